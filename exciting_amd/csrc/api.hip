@@ -1,0 +1,565 @@
+// C ABI of libexh (declared in include/exh.h): context, module state held
+// resident in HBM, and the match / H+O setup entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "exh_internal.h"
+
+namespace exh {
+
+int fail(exh_context *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(exh_context *c, hipError_t e, const char *where) {
+  if (c) c->err = std::string(where) + ": " + hipGetErrorString(e);
+  return (int)e > 0 ? (int)e : 1;
+}
+
+int ensure(exh_context *c, DevBuf &b, size_t bytes) {
+  if (bytes <= b.bytes && b.p) return 0;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+  }
+  size_t n = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, n);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return hip_fail(c, e, "hipMalloc");
+  }
+  b.bytes = n;
+  return 0;
+}
+
+static void release(DevBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+template <typename T>
+static int upload(exh_context *c, DevBuf &b, const T *src, size_t n) {
+  int rc = ensure(c, b, n * sizeof(T));
+  if (rc) return rc;
+  if (n == 0) return 0;
+  hipError_t e = hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "upload");
+}
+
+// polynom.f90 (m >= 0): m-th derivative at x of the degree np-1 polynomial
+// through (xa, ya).  Used for the boundary-derivative matrix of match.
+static double polynom_deriv(int m, int np, const double *xa, const double *ya, double x) {
+  double c[16];
+  if (np == 4) {
+    const double x1 = xa[1] - xa[0], x2 = xa[2] - xa[0], x3 = xa[3] - xa[0];
+    const double y1 = ya[1] - ya[0], y2 = ya[2] - ya[0], y3 = ya[3] - ya[0];
+    double t1 = 1.0 / (x1 * x2 * x3 * (x1 - x2) * (x1 - x3) * (x2 - x3));
+    const double t2 = x1 * x2 * y3, t3 = x2 * x3 * y1, t4 = x3 * x1 * y2;
+    const double t5 = x1 * x1, t6 = x2 * x2, t7 = x3 * x3;
+    c[1] = t1 * (t3 * (x3 * t6 - x2 * t7) + t4 * (x1 * t7 - x3 * t5) + t2 * (x2 * t5 - x1 * t6));
+    c[2] = t1 * (t3 * (t7 - t6) + t4 * (t5 - t7) + t2 * (t6 - t5));
+    c[3] = t1 * (t3 * (x2 - x3) + t4 * (x3 - x1) + t2 * (x1 - x2));
+    t1 = x - xa[0];
+    switch (m) {
+      case 0: return ya[0] + t1 * (c[1] + t1 * (c[2] + c[3] * t1));
+      case 1: return c[1] + t1 * (2.0 * c[2] + 3.0 * c[3] * t1);
+      case 2: return 6.0 * c[3] * t1 + 2.0 * c[2];
+      case 3: return 6.0 * c[3];
+      default: return 0.0;
+    }
+  }
+  if (m >= np) return 0.0;
+  for (int i = 0; i < np; i++) c[i] = ya[i];
+  for (int i = 2; i <= np; i++)
+    for (int j = np; j >= i; j--) c[j - 1] = (c[j - 1] - c[j - 2]) / (xa[j - 1] - xa[j - i]);
+  if (m == 0) {
+    double sum = c[0], t1 = 1.0;
+    for (int i = 2; i <= np; i++) {
+      t1 = t1 * (x - xa[i - 2]);
+      sum = sum + c[i - 1] * t1;
+    }
+    return sum;
+  }
+  const double x0 = xa[0];
+  for (int j = 1; j <= np - 1; j++)
+    for (int i = 1; i <= np - j; i++) {
+      const int k = np - i;
+      c[k - 1] = c[k - 1] + (x0 - xa[k - j]) * c[k];
+    }
+  for (int j = 1; j <= m; j++)
+    for (int i = m + 1; i <= np; i++) c[i - 1] = c[i - 1] * (double)(i - j);
+  double t1 = c[np - 1];
+  const double t2 = x - x0;
+  for (int i = np - 1; i >= m + 1; i--) t1 = t1 * t2 + c[i - 1];
+  return t1;
+}
+
+// zgetf2 on a real p x p matrix (leading dimension ld): partial pivoting by
+// largest |a|, reciprocal scaling of the column, rank-1 update.  Returns info.
+static int lu_real(int p, double *a, int ld, int *ipiv) {
+  int info = 0;
+  for (int j = 0; j < p; j++) {
+    int jp = j;
+    double best = std::fabs(a[j + ld * j]);
+    for (int i = j + 1; i < p; i++)
+      if (std::fabs(a[i + ld * j]) > best) {
+        best = std::fabs(a[i + ld * j]);
+        jp = i;
+      }
+    ipiv[j] = jp + 1;
+    if (a[jp + ld * j] != 0.0) {
+      if (jp != j)
+        for (int k = 0; k < p; k++) std::swap(a[j + ld * k], a[jp + ld * k]);
+      if (j < p - 1) {
+        const double r = 1.0 / a[j + ld * j];
+        for (int i = j + 1; i < p; i++) a[i + ld * j] = r * a[i + ld * j];
+      }
+    } else if (info == 0) {
+      info = j + 1;
+    }
+    for (int k = j + 1; k < p; k++) {
+      const double t = -a[j + ld * k];
+      for (int i = j + 1; i < p; i++) a[i + ld * k] += a[i + ld * j] * t;
+    }
+  }
+  return info;
+}
+
+// Rebuild the LU tables of match once both basis and radial data are known.
+static int build_match_tables(exh_context *c) {
+  constexpr int PC = 3;
+  const int L = c->lmaxapw, pmax = c->apwordmax;
+  const int np = std::max(pmax + 1, 4);
+  std::vector<double> lu((size_t)PC * PC * (L + 1) * c->natmtot, 0.0);
+  std::vector<int> pv((size_t)PC * (L + 1) * c->natmtot, 1);
+  for (int ias = 0; ias < c->natmtot; ias++) {
+    const int is = c->atom_species[ias];
+    const int ir = c->nrmt_h[is] - np;
+    if (ir < 0) return fail(c, EXH_EINVAL, "nrmt smaller than the polynomial order of match");
+    for (int l = 0; l <= L; l++) {
+      const int p = c->apword[l + (L + 1) * is];
+      double *a = lu.data() + (size_t)PC * PC * (l + (L + 1) * ias);
+      for (int io2 = 0; io2 < p; io2++) {
+        const double *ya = c->apwfr_h.data() + ir +
+                           (size_t)c->nrmtmax * (2 * (io2 + (size_t)pmax * (l + (size_t)(L + 1) * ias)));
+        for (int io1 = 0; io1 < p; io1++)
+          a[io1 + PC * io2] = polynom_deriv(io1, np, c->spr_h.data() + ir + (size_t)c->nrmtmax * is,
+                                            ya, c->rmt[is]);
+      }
+      int info = lu_real(p, a, PC, pv.data() + (size_t)PC * (l + (L + 1) * ias));
+      if (info != 0) {
+        char buf[160];
+        snprintf(buf, sizeof buf,
+                 "match: singular boundary-derivative matrix (species %d, atom %d, l %d, "
+                 "info %d)", is + 1, ias + 1, l, info);
+        return fail(c, EXH_EMATCH, buf);
+      }
+    }
+  }
+  int rc = upload(c, c->mt_match.lu, lu.data(), lu.size());
+  if (!rc) rc = upload(c, c->mt_match.piv, pv.data(), pv.size());
+  return rc;
+}
+
+}  // namespace exh
+
+using namespace exh;
+
+#define EXH_CHECK_CTX(c) \
+  if (!(c)) return EXH_EINVAL
+
+extern "C" {
+
+int exh_create(int device, exh_context **out) {
+  if (!out) return EXH_EINVAL;
+  *out = nullptr;
+  exh_context *c = new (std::nothrow) exh_context();
+  if (!c) return EXH_EINVAL;
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    int rc = (int)e;
+    delete c;
+    return rc > 0 ? rc : 1;
+  }
+  *out = c;
+  return 0;
+}
+
+void exh_destroy(exh_context *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  DevBuf *bufs[] = {&c->colA, &c->katom, &c->atom_species_d, &c->apword_d, &c->omax_d,
+                    &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
+                    &c->aa, &c->loa, &c->lolo, &c->salo, &c->ololo_eff, &c->h1aa,
+                    &c->ivg, &c->ivgig, &c->ist, &c->cf_stage, &c->vf_stage, &c->mf_stage,
+                    &c->m2_stage, &c->bt, &c->a2t, &c->ivgk, &c->s_apwalm, &c->s_igpig,
+                    &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac};
+  for (DevBuf *b : bufs) release(*b);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *exh_error(const exh_context *c) { return c ? c->err.c_str() : "null context"; }
+
+void *exh_stream(exh_context *c) { return c ? (void *)c->stream : nullptr; }
+
+int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, int lmaxmat,
+                  int apwordmax, const int *apword, int nlomax, const int *nlorb,
+                  const int *lorbl, int ngkmax) {
+  EXH_CHECK_CTX(c);
+  if (nspecies <= 0 || !natoms || !apword || !nlorb || (nlomax > 0 && !lorbl))
+    return fail(c, EXH_EINVAL, "set_basis: null or empty argument");
+  if (lmaxapw < 0 || lmaxmat < 0 || lmaxmat > lmaxapw)
+    return fail(c, EXH_EINVAL, "set_basis: need 0 <= lmaxmat <= lmaxapw");
+  if (lmaxapw > 12) return fail(c, EXH_ERANGE, "set_basis: lmaxapw > 12 not compiled");
+  if (apwordmax < 1 || apwordmax > 3)
+    return fail(c, EXH_ERANGE, "set_basis: apwordmax must be 1..3");
+  if (ngkmax <= 0) return fail(c, EXH_EINVAL, "set_basis: ngkmax <= 0");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  c->have_basis = c->have_radial = c->have_mt = false;
+  c->nspecies = nspecies;
+  c->lmaxapw = lmaxapw;
+  c->lmaxmat = lmaxmat;
+  c->apwordmax = apwordmax;
+  c->nlomax = nlomax;
+  c->ngkmax = ngkmax;
+  c->lmmax = (lmaxapw + 1) * (lmaxapw + 1);
+  c->natoms.assign(natoms, natoms + nspecies);
+  c->apword.assign(apword, apword + (size_t)(lmaxapw + 1) * nspecies);
+  c->nlorb.assign(nlorb, nlorb + nspecies);
+  c->lorbl.assign(nlomax > 0 ? lorbl : nlorb, nlomax > 0 ? lorbl + (size_t)nlomax * nspecies : nlorb);
+  c->natmtot = 0;
+  for (int is = 0; is < nspecies; is++) {
+    if (natoms[is] < 0) return fail(c, EXH_EINVAL, "set_basis: natoms < 0");
+    c->natmtot += natoms[is];
+    if (nlorb[is] < 0 || nlorb[is] > nlomax) return fail(c, EXH_EINVAL, "set_basis: nlorb");
+    for (int l = 0; l <= lmaxapw; l++) {
+      int p = apword[l + (lmaxapw + 1) * is];
+      if (p < 1 || p > apwordmax) return fail(c, EXH_EINVAL, "set_basis: apword out of 1..apwordmax");
+      if (p != apwordmax) c->uniform_apword = false;
+    }
+    for (int ilo = 0; ilo < nlorb[is]; ilo++) {
+      int l = lorbl[ilo + nlomax * is];
+      if (l < 0 || l > lmaxapw) return fail(c, EXH_EINVAL, "set_basis: lorbl out of range");
+    }
+  }
+  // per-species sizes (MaxAPWs, losize), per-atom offsets, stacked-K map
+  std::vector<int> Msp(nspecies, 0);
+  c->losize.assign(nspecies, 0);
+  c->maxaa = 0;
+  c->maxnlo = 0;
+  for (int is = 0; is < nspecies; is++) {
+    for (int l = 0; l <= lmaxmat; l++) Msp[is] += (2 * l + 1) * apword[l + (lmaxapw + 1) * is];
+    for (int ilo = 0; ilo < nlorb[is]; ilo++) c->losize[is] += 2 * lorbl[ilo + nlomax * is] + 1;
+    c->maxaa = std::max(c->maxaa, Msp[is]);
+    c->maxnlo = std::max(c->maxnlo, c->losize[is]);
+  }
+  if (c->maxnlo > 64) return fail(c, EXH_ERANGE, "set_basis: more than 64 LO functions per atom");
+  c->atom_species.clear();
+  c->Ms.clear();
+  c->Koff.clear();
+  c->lobase.clear();
+  c->colA_h.clear();
+  c->lorow_h.clear();
+  std::vector<int> kinfo;
+  int k = 0, lob = 0;
+  for (int is = 0; is < nspecies; is++)
+    for (int ia = 0; ia < natoms[is]; ia++) {
+      const int ias = (int)c->atom_species.size();
+      c->atom_species.push_back(is);
+      c->Ms.push_back(Msp[is]);
+      c->Koff.push_back(k);
+      c->lobase.push_back(lob);
+      int f = 0;
+      for (int l = 0; l <= lmaxmat; l++)
+        for (int m = -l; m <= l; m++) {
+          const int lm = l * (l + 1) + m;
+          const int p = apword[l + (lmaxapw + 1) * is];
+          for (int io = 0; io < p; io++) {
+            c->colA_h.push_back((int64_t)ngkmax * (io + (int64_t)apwordmax * (lm + (int64_t)c->lmmax * ias)));
+            kinfo.push_back(io | (p << 4) | (l << 8) | ((f - io) << 16));
+            f++;
+          }
+        }
+      k += Msp[is];
+      for (int ilo = 0; ilo < nlorb[is]; ilo++) {
+        const int l = lorbl[ilo + nlomax * is];
+        for (int mi = 0; mi <= 2 * l; mi++) c->lorow_h.push_back(LoRow{ias, ilo, mi, l});
+      }
+      lob += c->losize[is];
+    }
+  c->Ktot = k;
+  c->nlotot = lob;
+  // column table: [colA (Ktot) | k * ldb (Ktot)] with ldb fixed by ngkmax
+  const int64_t ldb = ((int64_t)ngkmax + 63) / 64 * 64;
+  std::vector<int64_t> cols(c->colA_h);
+  for (int kk = 0; kk < c->Ktot; kk++) cols.push_back((int64_t)kk * ldb);
+  std::vector<int> katom;
+  katom.insert(katom.end(), c->Koff.begin(), c->Koff.end());
+  katom.insert(katom.end(), c->Ms.begin(), c->Ms.end());
+  katom.insert(katom.end(), kinfo.begin(), kinfo.end());
+  std::vector<int> omax(nspecies, 0);
+  for (int is = 0; is < nspecies; is++)
+    for (int l = 0; l <= lmaxapw; l++) omax[is] = std::max(omax[is], apword[l + (lmaxapw + 1) * is]);
+  std::vector<int> lo;  // [lobase(natmtot), losize, nlorb, lorbl]
+  lo.insert(lo.end(), c->lobase.begin(), c->lobase.end());
+  lo.insert(lo.end(), c->losize.begin(), c->losize.end());
+  lo.insert(lo.end(), c->nlorb.begin(), c->nlorb.end());
+  lo.insert(lo.end(), c->lorbl.begin(), c->lorbl.end());
+  int rc = upload(c, c->colA, cols.data(), cols.size());
+  if (!rc) rc = upload(c, c->katom, katom.data(), katom.size());
+  if (!rc) rc = upload(c, c->atom_species_d, c->atom_species.data(), c->atom_species.size());
+  if (!rc) rc = upload(c, c->apword_d, c->apword.data(), c->apword.size());
+  if (!rc) rc = upload(c, c->omax_d, omax.data(), omax.size());
+  if (!rc) rc = upload(c, c->lobase_d, lo.data(), lo.size());
+  if (!rc) rc = upload(c, c->lorow, c->lorow_h.data(), c->lorow_h.size());
+  if (rc) return rc;
+  c->have_basis = true;
+  return 0;
+}
+
+int exh_set_radial(exh_context *c, double omega, const double *rmt, const int *nrmt,
+                   int nrmtmax, const double *spr, const double *apwfr) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_basis) return fail(c, EXH_ESTATE, "set_radial: call exh_set_basis first");
+  if (!(omega > 0.0) || !rmt || !nrmt || !spr || !apwfr || nrmtmax <= 0)
+    return fail(c, EXH_EINVAL, "set_radial: bad argument");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  c->omega = omega;
+  c->rmt.assign(rmt, rmt + c->nspecies);
+  c->nrmt_h.assign(nrmt, nrmt + c->nspecies);
+  for (int is = 0; is < c->nspecies; is++)
+    if (nrmt[is] > nrmtmax || !(rmt[is] > 0.0)) return fail(c, EXH_EINVAL, "set_radial: nrmt/rmt");
+  c->nrmtmax = nrmtmax;
+  c->spr_h.assign(spr, spr + (size_t)nrmtmax * c->nspecies);
+  c->apwfr_h.assign(apwfr, apwfr + (size_t)nrmtmax * 2 * c->apwordmax * (c->lmaxapw + 1) * c->natmtot);
+  int rc = upload(c, c->rmt_d, c->rmt.data(), c->rmt.size());
+  if (!rc) rc = build_match_tables(c);
+  if (rc) return rc;
+  c->have_radial = true;
+  return 0;
+}
+
+int exh_set_mt(exh_context *c, int maxaa, int maxnlo, const exh_z *aa, const exh_z *loa,
+               const exh_z *lolo, const double *oalo, const double *ololo, const double *h1aa,
+               const double *h1loa, const double *h1lolo) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_basis) return fail(c, EXH_ESTATE, "set_mt: call exh_set_basis first");
+  if (maxaa != c->maxaa || maxnlo != c->maxnlo)
+    return fail(c, EXH_EINVAL, "set_mt: maxaa/maxnlo differ from the basis");
+  if (!aa || (c->nlotot > 0 && (!loa || !lolo || !oalo || !ololo)))
+    return fail(c, EXH_EINVAL, "set_mt: null table");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  const size_t na = c->natmtot;
+  const size_t mnlo = std::max(maxnlo, 1), nlom = std::max(c->nlomax, 1);
+  int rc = upload(c, c->aa, (const double2 *)aa, (size_t)maxaa * maxaa * na);
+  std::vector<double2> zero;
+  if (!rc) {
+    if (loa) rc = upload(c, c->loa, (const double2 *)loa, mnlo * maxaa * na);
+    else rc = ensure(c, c->loa, 16);
+  }
+  if (!rc) {
+    if (lolo) rc = upload(c, c->lolo, (const double2 *)lolo, mnlo * mnlo * na);
+    else rc = ensure(c, c->lolo, 16);
+  }
+  // s = oalo + h1loa ; ololo_eff = ololo + h1lolo
+  std::vector<double> salo((size_t)c->apwordmax * nlom * na, 0.0), olo(nlom * nlom * na, 0.0);
+  if (oalo)
+    for (size_t i = 0; i < salo.size() && c->nlomax > 0; i++)
+      salo[i] = oalo[i] + (h1loa ? h1loa[i] : 0.0);
+  if (ololo)
+    for (size_t i = 0; i < olo.size() && c->nlomax > 0; i++)
+      olo[i] = ololo[i] + (h1lolo ? h1lolo[i] : 0.0);
+  if (!rc) rc = upload(c, c->salo, salo.data(), salo.size());
+  if (!rc) rc = upload(c, c->ololo_eff, olo.data(), olo.size());
+  c->have_h1aa = h1aa != nullptr;
+  if (!rc && h1aa)
+    rc = upload(c, c->h1aa, h1aa, (size_t)c->apwordmax * c->apwordmax * (c->lmaxapw + 1) * na);
+  if (rc) return rc;
+  c->have_mt = true;
+  return 0;
+}
+
+int exh_set_gvectors(exh_context *c, int ngrtot, int ngvec, const int *ivg, const int *intgv,
+                     const int *ivgig) {
+  EXH_CHECK_CTX(c);
+  if (ngrtot <= 0 || ngvec <= 0 || ngvec > ngrtot || !ivg || !intgv || !ivgig)
+    return fail(c, EXH_EINVAL, "set_gvectors: bad argument");
+  for (int d = 0; d < 3; d++)
+    if (intgv[3 + d] < intgv[d]) return fail(c, EXH_EINVAL, "set_gvectors: empty box");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  const size_t nbox = (size_t)(intgv[3] - intgv[0] + 1) * (intgv[4] - intgv[1] + 1) * (intgv[5] - intgv[2] + 1);
+  if (nbox > (size_t)1 << 31) return fail(c, EXH_ERANGE, "set_gvectors: box too large");
+  c->have_gvec = c->have_ist = false;
+  c->ngrtot = ngrtot;
+  c->ngvec = ngvec;
+  std::memcpy(c->intgv, intgv, sizeof(int) * 6);
+  int rc = upload(c, c->ivg, ivg, (size_t)3 * ngrtot);
+  if (!rc) rc = upload(c, c->ivgig, ivgig, nbox);
+  if (!rc) rc = ensure(c, c->ist, nbox * sizeof(IstEntry));
+  if (rc) return rc;
+  c->have_gvec = true;
+  return 0;
+}
+
+int exh_set_interstitial(exh_context *c, const exh_z *cfunig, const exh_z *veffig,
+                         const exh_z *meffig, const exh_z *m2effig, int valence_rel_none,
+                         int h1on) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_gvec) return fail(c, EXH_ESTATE, "set_interstitial: call exh_set_gvectors first");
+  if (!cfunig || !veffig || (!valence_rel_none && !meffig) || (h1on && !m2effig))
+    return fail(c, EXH_EINVAL, "set_interstitial: null array");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  int rc = upload(c, c->cf_stage, (const double2 *)cfunig, (size_t)c->ngrtot);
+  if (!rc) rc = upload(c, c->vf_stage, (const double2 *)veffig, (size_t)c->ngvec);
+  if (!rc && !valence_rel_none) rc = upload(c, c->mf_stage, (const double2 *)meffig, (size_t)c->ngvec);
+  if (!rc && h1on) rc = upload(c, c->m2_stage, (const double2 *)m2effig, (size_t)c->ngvec);
+  if (rc) return rc;
+  hipError_t e = launch_ist_table(c->stream, c, (const double2 *)c->cf_stage.p,
+                                  (const double2 *)c->vf_stage.p, (const double2 *)c->mf_stage.p,
+                                  (const double2 *)c->m2_stage.p, valence_rel_none ? 1 : 0,
+                                  h1on ? 1 : 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "set_interstitial");
+  c->have_ist = true;
+  return 0;
+}
+
+int exh_match_device(exh_context *c, void *stream, int ngp, const double *gpc,
+                     const double *tpgpc, const exh_z *sfacgp, exh_z *apwalm) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_basis || !c->have_radial)
+    return fail(c, EXH_ESTATE, "match: basis and radial data not set");
+  if (ngp < 0 || ngp > c->ngkmax) return fail(c, EXH_EINVAL, "match: ngp outside 0..ngkmax");
+  if (ngp > 0 && (!gpc || !tpgpc || !sfacgp || !apwalm))
+    return fail(c, EXH_EINVAL, "match: null array");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipError_t e = launch_match(s, c, ngp, gpc, tpgpc, (const double2 *)sfacgp, (double2 *)apwalm);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "match kernel");
+}
+
+int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_z *apwalm,
+                     const int *igpig, const double *vgpc, exh_z *h, exh_z *o, int flags) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_basis || !c->have_mt || !c->have_gvec || !c->have_ist)
+    return fail(c, EXH_ESTATE, "setup: basis, muffin-tin tables and interstitial data must be set");
+  if (ngp < 0 || ngp > c->ngkmax) return fail(c, EXH_EINVAL, "setup: ngp outside 0..ngkmax");
+  if (rank != ngp + c->nlotot) return fail(c, EXH_EINVAL, "setup: rank != ngp + nlotot");
+  if (!h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
+    return fail(c, EXH_EINVAL, "setup: null array");
+  if (flags & ~EXH_ACCUMULATE) return fail(c, EXH_ENOTSUP, "setup: unknown flag");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int ldb = (c->ngkmax + 63) / 64 * 64;
+  int rc = ensure(c, c->bt, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
+  if (!rc && c->have_h1aa) rc = ensure(c, c->a2t, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
+  if (!rc) rc = ensure(c, c->ivgk, (size_t)std::max(c->ngkmax, 1) * sizeof(int4));
+  if (rc) return rc;
+  const double2 *A = (const double2 *)apwalm;
+  hipError_t e = launch_ivgk(s, ngp, igpig, (const int *)c->ivg.p, c->ngrtot, (int4 *)c->ivgk.p);
+  if (e == hipSuccess) e = launch_bt(s, c, ngp, A, (double2 *)c->bt.p, ldb);
+  if (e == hipSuccess && c->have_h1aa) e = launch_a2t(s, c, ngp, A, (double2 *)c->a2t.p, ldb);
+  if (e == hipSuccess)
+    e = launch_ho(s, c, ngp, A, (const double2 *)c->bt.p,
+                  c->have_h1aa ? (const double2 *)c->a2t.p : nullptr, ldb,
+                  (const int4 *)c->ivgk.p, vgpc, (double2 *)h, (double2 *)o, rank, flags);
+  if (e == hipSuccess) e = launch_lo_border(s, c, ngp, A, (double2 *)h, (double2 *)o, rank, flags);
+  if (e == hipSuccess) e = launch_lolo(s, c, ngp, (double2 *)h, (double2 *)o, rank, flags);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "setup kernels");
+}
+
+int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
+              const exh_z *sfacgp, exh_z *apwalm) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_basis || !c->have_radial)
+    return fail(c, EXH_ESTATE, "match: basis and radial data not set");
+  if (ngp < 0 || ngp > c->ngkmax) return fail(c, EXH_EINVAL, "match: ngp outside 0..ngkmax");
+  if (ngp == 0) return 0;
+  if (!gpc || !tpgpc || !sfacgp || !apwalm) return fail(c, EXH_EINVAL, "match: null array");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  const size_t ld = c->ngkmax, ncol = (size_t)c->apwordmax * c->lmmax * c->natmtot;
+  int rc = ensure(c, c->s_gpc, ld * sizeof(double));
+  if (!rc) rc = ensure(c, c->s_tpgpc, 2 * ld * sizeof(double));
+  if (!rc) rc = ensure(c, c->s_sfac, ld * c->natmtot * sizeof(double2));
+  if (!rc) rc = ensure(c, c->s_apwalm, ld * ncol * sizeof(double2));
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  hipError_t e = hipMemcpyAsync(c->s_gpc.p, gpc, ngp * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->s_tpgpc.p, tpgpc, 2 * (size_t)ngp * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(c->s_sfac.p, ld * sizeof(double2), sfacgp, ld * sizeof(double2),
+                         ngp * sizeof(double2), c->natmtot, hipMemcpyHostToDevice, s);
+  // entries with io >= apword(l) are not written by match: keep the caller's
+  if (e == hipSuccess && !c->uniform_apword)
+    e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
+                         ngp * sizeof(double2), ncol, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(c, e, "match: upload");
+  e = launch_match(s, c, ngp, (const double *)c->s_gpc.p, (const double *)c->s_tpgpc.p,
+                   (const double2 *)c->s_sfac.p, (double2 *)c->s_apwalm.p);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(apwalm, ld * sizeof(double2), c->s_apwalm.p, ld * sizeof(double2),
+                         ngp * sizeof(double2), ncol, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "match");
+}
+
+int exh_hamiltonandoverlapsetup_ex(exh_context *c, int ngp, int rank, const exh_z *apwalm,
+                                   const int *igpig, const double *vgpc, exh_z *h, exh_z *o,
+                                   int flags) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_basis) return fail(c, EXH_ESTATE, "setup: basis not set");
+  if (ngp < 0 || ngp > c->ngkmax) return fail(c, EXH_EINVAL, "setup: ngp outside 0..ngkmax");
+  if (rank != ngp + c->nlotot) return fail(c, EXH_EINVAL, "setup: rank != ngp + nlotot");
+  if (!h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
+    return fail(c, EXH_EINVAL, "setup: null array");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  const size_t ld = c->ngkmax, ncol = (size_t)c->apwordmax * c->lmmax * c->natmtot;
+  const size_t nn = (size_t)rank * rank;
+  int rc = ensure(c, c->s_apwalm, ld * ncol * sizeof(double2));
+  if (!rc) rc = ensure(c, c->s_igpig, ld * sizeof(int));
+  if (!rc) rc = ensure(c, c->s_vgpc, 3 * ld * sizeof(double));
+  if (!rc) rc = ensure(c, c->s_h, std::max<size_t>(nn, 1) * sizeof(double2));
+  if (!rc) rc = ensure(c, c->s_o, std::max<size_t>(nn, 1) * sizeof(double2));
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  hipError_t e = hipSuccess;
+  if (ngp > 0) {
+    e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
+                         ngp * sizeof(double2), ncol, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->s_igpig.p, igpig, ngp * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->s_vgpc.p, vgpc, 3 * (size_t)ngp * sizeof(double), hipMemcpyHostToDevice, s);
+  }
+  if (e == hipSuccess && (flags & EXH_ACCUMULATE)) {
+    e = hipMemcpyAsync(c->s_h.p, h, nn * sizeof(double2), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->s_o.p, o, nn * sizeof(double2), hipMemcpyHostToDevice, s);
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "setup: upload");
+  rc = exh_setup_device(c, s, ngp, rank, (const exh_z *)c->s_apwalm.p, (const int *)c->s_igpig.p,
+                        (const double *)c->s_vgpc.p, (exh_z *)c->s_h.p, (exh_z *)c->s_o.p, flags);
+  if (rc) return rc;
+  e = hipMemcpyAsync(h, c->s_h.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(o, c->s_o.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "setup: download");
+}
+
+int exh_hamiltonandoverlapsetup(exh_context *c, int ngp, int rank, const exh_z *apwalm,
+                                const int *igpig, const double *vgpc, exh_z *h, exh_z *o) {
+  return exh_hamiltonandoverlapsetup_ex(c, ngp, rank, apwalm, igpig, vgpc, h, o, EXH_ACCUMULATE);
+}
+
+}  // extern "C"

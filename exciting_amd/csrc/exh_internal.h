@@ -1,0 +1,123 @@
+// Internal declarations shared by the HIP translation units of libexh.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/exh.h"
+
+namespace exh {
+
+// Device buffer owned by the context; grows, never shrinks.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+// Interstitial lookup entry for one point of the ivgig box:
+//   H(i,j) += veff + t1 * kin,  O(i,j) += ocf + t1 * okin,
+//   t1 = 0.5 * (G_i+k).(G_j+k).
+// 'none' relativity: kin = ocf = cfunig, okin = 0 (hmlistln.f90:73-84,
+// olpistln.f90:69-82).  Non-'none': kin = meffig (hmlistln.f90:61-71).
+// h1on: okin = alpha^2/2 * m2effig (olpistln.f90:53-66).  Points whose G
+// index is 0 or beyond ngvec hold zeros (the reference skips them).
+struct IstEntry {
+  double2 veff, kin, ocf, okin;
+};
+
+// Per LO row of the lo-lo block: owning atom, local orbital, m offset, l.
+struct LoRow {
+  int ias, ilo, mi, l;
+};
+
+struct MatchTables {
+  // LU factors of the boundary-derivative matrix D per (l, ias): pmax*pmax
+  // doubles (column-major, zgetf2 layout) and pmax 1-based pivots.
+  DevBuf lu, piv;
+};
+
+}  // namespace exh
+
+struct exh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // state flags
+  bool have_basis = false, have_radial = false, have_mt = false;
+  bool have_gvec = false, have_ist = false;
+
+  // basis (host)
+  int nspecies = 0, natmtot = 0, lmaxapw = 0, lmaxmat = 0, apwordmax = 0;
+  int nlomax = 0, ngkmax = 0, maxaa = 0, maxnlo = 0, nlotot = 0, lmmax = 0;
+  bool uniform_apword = true;
+  std::vector<int> natoms, apword, nlorb, lorbl;
+  std::vector<int> atom_species, losize, lobase;  // per species / per atom
+  std::vector<int> Ms, Koff;                     // per atom: K width, K offset
+  int Ktot = 0;
+  std::vector<int64_t> colA_h;                   // stacked K -> apwalm column
+  std::vector<exh::LoRow> lorow_h;
+
+  // device copies of basis tables
+  exh::DevBuf colA, katom, atom_species_d, apword_d, omax_d, lorow, lobase_d;
+
+  // match
+  double omega = 0.0;
+  std::vector<double> rmt;
+  std::vector<int> nrmt_h;
+  int nrmtmax = 0;
+  std::vector<double> spr_h, apwfr_h;
+  exh::DevBuf rmt_d;
+  exh::MatchTables mt_match;
+
+  // muffin-tin tables
+  exh::DevBuf aa, loa, lolo, salo, ololo_eff, h1aa;
+  bool have_h1aa = false;
+
+  // G vectors / interstitial
+  int ngrtot = 0, ngvec = 0;
+  int intgv[6] = {0, 0, 0, 0, 0, 0};
+  exh::DevBuf ivg, ivgig, ist;  // ist: IstEntry per box point
+  exh::DevBuf cf_stage, vf_stage, mf_stage, m2_stage;
+
+  // per-call scratch
+  exh::DevBuf bt, a2t, ivgk;
+  // host-API staging
+  exh::DevBuf s_apwalm, s_igpig, s_vgpc, s_h, s_o, s_gpc, s_tpgpc, s_sfac;
+};
+
+namespace exh {
+
+int fail(exh_context *c, int code, const std::string &msg);
+int hip_fail(exh_context *c, hipError_t e, const char *where);
+int ensure(exh_context *c, DevBuf &b, size_t bytes);
+
+// Kernel launchers (setup_kernels.hip / match_kernel.hip).
+hipError_t launch_ivgk(hipStream_t s, int ngp, const int *igpig, const int *ivg,
+                       int ngrtot, int4 *ivgk);
+hipError_t launch_ist_table(hipStream_t s, const exh_context *c,
+                            const double2 *cf, const double2 *vf,
+                            const double2 *mf, const double2 *m2, int relnone,
+                            int h1on);
+hipError_t launch_bt(hipStream_t s, const exh_context *c, int ngp,
+                     const double2 *apwalm, double2 *bt, int ldb);
+hipError_t launch_a2t(hipStream_t s, const exh_context *c, int ngp,
+                      const double2 *apwalm, double2 *a2t, int ldb);
+hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
+                     const double2 *apwalm, const double2 *bt,
+                     const double2 *a2t, int ldb, const int4 *ivgk,
+                     const double *vgpc, double2 *h, double2 *o, int ldh,
+                     int flags);
+hipError_t launch_lo_border(hipStream_t s, const exh_context *c, int ngp,
+                            const double2 *apwalm, double2 *h, double2 *o,
+                            int ldh, int flags);
+hipError_t launch_lolo(hipStream_t s, const exh_context *c, int ngp,
+                       double2 *h, double2 *o, int ldh, int flags);
+hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
+                        const double *gpc, const double *tpgpc,
+                        const double2 *sfacgp, double2 *apwalm);
+
+}  // namespace exh

@@ -1,0 +1,583 @@
+// FP64 kernels of the H/O setup (hamiltonsetup.f90 + overlapsetup.f90 +
+// hmlistln.f90 + olpistln.f90 + hamiltonoverlapocopy_UL.f90).
+//
+// Data layout in HBM (all column-major complex128 as double2):
+//   apwalm(ngkmax, apwordmax, lmmax, natmtot)  the caller's matching coeffs.
+//     Read as A^T: for stacked K index k = (atom, f) the column
+//     apwalm + colA[k] holds A_f(G) for G = 0..ngp-1 (contiguous).
+//   Bt[k * ldb + G] = (aa_atom . A_atom)(f, G)        per-call scratch.
+//   H, O (ldh = rank): written once, full square, upper computed, lower
+//     mirrored (lower := conj(upper)).
+//
+// MFMA: v_mfma_f64_16x16x4f64.  Operand lanes: A[i=lane%16][k=lane/16],
+// B[k=lane/16][j=lane%16]; result D[i=lane/16+4*r][j=lane%16], r=0..3
+// (established by the parity tests against the oracle).
+#include "exh_internal.h"
+
+namespace exh {
+namespace {
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dx4 mfma(double a, double b, dx4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int TM = 64;    // output tile edge (rows and columns)
+constexpr int TK = 16;    // K slab staged in LDS per step
+constexpr int NTH = 256;  // 4 waves; wave w owns a 32x32 quadrant
+
+__device__ __forceinline__ double2 z2(double r, double i) {
+  double2 v;
+  v.x = r;
+  v.y = i;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// ivg(:, igpig(i)) for the G+k set; w = 0 flags an igpig outside 1..ngrtot.
+// ---------------------------------------------------------------------------
+__global__ void ivgk_kernel(int ngp, const int *__restrict__ igpig,
+                            const int *__restrict__ ivg, int ngrtot,
+                            int4 *__restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ngp) return;
+  int g = igpig[i] - 1;
+  int4 v = make_int4(0, 0, 0, 0);
+  if (g >= 0 && g < ngrtot) {
+    v.x = ivg[3 * g];
+    v.y = ivg[3 * g + 1];
+    v.z = ivg[3 * g + 2];
+    v.w = 1;
+  }
+  out[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Interstitial lookup table over the ivgig box (one gather per element).
+// ---------------------------------------------------------------------------
+__global__ void ist_table_kernel(int nbox, const int *__restrict__ ivgig,
+                                 int ngvec, const double2 *__restrict__ cf,
+                                 const double2 *__restrict__ vf,
+                                 const double2 *__restrict__ mf,
+                                 const double2 *__restrict__ m2, int relnone,
+                                 int h1on, double a2, IstEntry *__restrict__ T) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbox) return;
+  int ig = ivgig[b];
+  IstEntry e;
+  e.veff = z2(0, 0);
+  e.kin = z2(0, 0);
+  e.ocf = z2(0, 0);
+  e.okin = z2(0, 0);
+  if (ig > 0 && ig <= ngvec) {
+    double2 c = cf[ig - 1];
+    e.veff = vf[ig - 1];
+    e.ocf = c;
+    e.kin = relnone ? c : mf[ig - 1];
+    if (h1on) {
+      double2 m = m2[ig - 1];
+      e.okin = z2(a2 * m.x, a2 * m.y);
+    }
+  }
+  T[b] = e;
+}
+
+// ---------------------------------------------------------------------------
+// Bt = (aa . A)^T per atom:  Bt[Koff+f][G] = sum_f' A^T[G][f'] aa(f, f')
+// (hamiltonsetup.f90:75-88, zm = aa * apwi).  Grid (G tiles, f tiles, atoms).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTH) void bt_kernel(
+    int ngp, const double2 *__restrict__ apwalm,
+    const long long *__restrict__ colA, const int *__restrict__ Koff,
+    const int *__restrict__ Ms, const double2 *__restrict__ aa, int maxaa,
+    double2 *__restrict__ bt, int ldb) {
+  const int ias = blockIdx.z;
+  const int M = Ms[ias];
+  const int F0 = blockIdx.y * TM;
+  if (F0 >= M) return;
+  const int I0 = blockIdx.x * TM;
+  const int koff = Koff[ias];
+  const long long *col = colA + koff;
+  const double2 *aat = aa + (size_t)ias * maxaa * maxaa;
+
+  __shared__ double sXr[TK][TM], sXi[TK][TM], sYr[TK][TM], sYi[TK][TM];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, lr = lane & 15, lk = lane >> 4;
+
+  dx4 cre[2][2], cim[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      cre[a][b] = dx4{0, 0, 0, 0};
+      cim[a][b] = dx4{0, 0, 0, 0};
+    }
+
+  double2 px[4], py[4];
+  auto load = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int e = tid + NTH * q, kk = e >> 6, ii = e & 63, k = kb + kk;
+      bool kv = k < M;
+      px[q] = (kv && I0 + ii < ngp) ? apwalm[col[k] + I0 + ii] : z2(0, 0);
+      py[q] = (kv && F0 + ii < M) ? aat[(F0 + ii) + (size_t)maxaa * k] : z2(0, 0);
+    }
+  };
+  load(0);
+  for (int kb = 0; kb < M; kb += TK) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int e = tid + NTH * q, kk = e >> 6, ii = e & 63;
+      sXr[kk][ii] = px[q].x;
+      sXi[kk][ii] = px[q].y;
+      sYr[kk][ii] = py[q].x;
+      sYi[kk][ii] = py[q].y;
+    }
+    __syncthreads();
+    if (kb + TK < M) load(kb + TK);
+#pragma unroll
+    for (int k4 = 0; k4 < TK / 4; k4++) {
+      const int kk = 4 * k4 + lk;
+      double xr[2], xi[2], nxi[2], yr[2], yi[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        xr[a] = sXr[kk][wr * 32 + a * 16 + lr];
+        xi[a] = sXi[kk][wr * 32 + a * 16 + lr];
+        nxi[a] = -xi[a];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        yr[b] = sYr[kk][wc * 32 + b * 16 + lr];
+        yi[b] = sYi[kk][wc * 32 + b * 16 + lr];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          cre[a][b] = mfma(xr[a], yr[b], cre[a][b]);
+          cre[a][b] = mfma(nxi[a], yi[b], cre[a][b]);
+          cim[a][b] = mfma(xr[a], yi[b], cim[a][b]);
+          cim[a][b] = mfma(xi[a], yr[b], cim[a][b]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int i = I0 + wr * 32 + a * 16 + lk + 4 * r;
+        int f = F0 + wc * 32 + b * 16 + lr;
+        if (i < ngp && f < M)
+          bt[(size_t)(koff + f) * ldb + i] = z2(cre[a][b][r], cim[a][b][r]);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// A2t = ((1 + h1) . A)^T: the overlap operand when h1aa is present
+// (overlapsetup.f90:82-94).  h1aa(io1, io2, l, ias) couples the radial
+// orders of one (l, m).  One thread per (G, atom, f2).
+// ---------------------------------------------------------------------------
+__global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apwalm,
+                           const long long *__restrict__ colA,
+                           const int *__restrict__ Koff, const int *__restrict__ Ms,
+                           const int *__restrict__ kinfo, const double *__restrict__ h1aa,
+                           int pmax, int lmaxapw, double2 *__restrict__ a2t, int ldb) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  int ias = blockIdx.z;
+  int f2 = blockIdx.y;
+  if (g >= ngp || f2 >= Ms[ias]) return;
+  int koff = Koff[ias];
+  // kinfo[k] = io | p << 4 | l << 8 | (first f of this (l,m)) << 16
+  const int info = kinfo[koff + f2];
+  const int io2 = info & 15, p = (info >> 4) & 15, l = (info >> 8) & 255, fb = info >> 16;
+  double2 v = z2(0, 0);
+  for (int io1 = 0; io1 < p; io1++) {
+    double hv = h1aa[io1 + pmax * (io2 + (size_t)pmax * (l + (size_t)(lmaxapw + 1) * ias))];
+    double2 a = apwalm[colA[koff + fb + io1] + g];
+    v.x += hv * a.x;
+    v.y += hv * a.y;
+  }
+  double2 a = apwalm[colA[koff + f2] + g];
+  v.x += a.x;
+  v.y += a.y;
+  a2t[(size_t)(koff + f2) * ldb + g] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Fused upper-triangle H and O tiles.
+//   H(i,j) = sum_k conj(A^T[i][k]) Bt[k][j]  + IST_H(i,j)   i <= j < ngp
+//   O(i,j) = sum_k conj(A^T[i][k]) YA[k][j]  + IST_O(i,j)
+// K runs over all atoms stacked (hamiltonsetup.f90:90-103, overlapsetup
+// .f90:95-108); the epilogue adds the interstitial terms (hmlistln.f90,
+// olpistln.f90) and writes lower := conj(upper) (hamiltonoverlapocopy_UL).
+// One workgroup per upper tile pair (I <= J); tiles are dealt to the 8 XCDs
+// in contiguous ranges so co-resident tiles share column slabs in L2.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTH) void ho_kernel(
+    int ngp, int Ktot, const double2 *__restrict__ xa,
+    const long long *__restrict__ colX, const double2 *__restrict__ yb, int ldb,
+    const double2 *__restrict__ ya, const long long *__restrict__ colY,
+    const int4 *__restrict__ ivgk, const double *__restrict__ vgpc,
+    const IstEntry *__restrict__ T, int lo0, int lo1, int lo2, int n0, int n1,
+    int n2, double2 *__restrict__ H, double2 *__restrict__ O, long long ldh,
+    int accumulate) {
+  // ---- tile decode (XCD-contiguous remap, then J-major pair order) ----
+  const int bid = blockIdx.x, nb = gridDim.x;
+  const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
+  const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
+  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((J + 1) * (J + 2) / 2 <= t) J++;
+  while (J * (J + 1) / 2 > t) J--;
+  const int I = t - J * (J + 1) / 2;
+  const int I0 = I * TM, J0 = J * TM;
+
+  __shared__ double sXr[TK][TM], sXi[TK][TM];
+  __shared__ double sBr[TK][TM], sBi[TK][TM];
+  __shared__ double sAr[TK][TM], sAi[TK][TM];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, lr = lane & 15, lk = lane >> 4;
+
+  dx4 hre[2][2], him[2][2], ore[2][2], oim[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      hre[a][b] = dx4{0, 0, 0, 0};
+      him[a][b] = dx4{0, 0, 0, 0};
+      ore[a][b] = dx4{0, 0, 0, 0};
+      oim[a][b] = dx4{0, 0, 0, 0};
+    }
+
+  double2 px[4], pb[4], pa[4];
+  auto load = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int e = tid + NTH * q, kk = e >> 6, ii = e & 63, k = kb + kk;
+      bool kv = k < Ktot;
+      long long cx = kv ? colX[k] : 0;
+      long long cy = kv ? colY[k] : 0;
+      px[q] = (kv && I0 + ii < ngp) ? xa[cx + I0 + ii] : z2(0, 0);
+      pb[q] = (kv && J0 + ii < ngp) ? yb[(size_t)k * ldb + J0 + ii] : z2(0, 0);
+      pa[q] = (kv && J0 + ii < ngp) ? ya[cy + J0 + ii] : z2(0, 0);
+    }
+  };
+  load(0);
+  for (int kb = 0; kb < Ktot; kb += TK) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int e = tid + NTH * q, kk = e >> 6, ii = e & 63;
+      sXr[kk][ii] = px[q].x;
+      sXi[kk][ii] = px[q].y;
+      sBr[kk][ii] = pb[q].x;
+      sBi[kk][ii] = pb[q].y;
+      sAr[kk][ii] = pa[q].x;
+      sAi[kk][ii] = pa[q].y;
+    }
+    __syncthreads();
+    if (kb + TK < Ktot) load(kb + TK);
+#pragma unroll
+    for (int k4 = 0; k4 < TK / 4; k4++) {
+      const int kk = 4 * k4 + lk;
+      double xr[2], xi[2], nxi[2], br[2], bi[2], ar[2], ai[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        xr[a] = sXr[kk][wr * 32 + a * 16 + lr];
+        xi[a] = sXi[kk][wr * 32 + a * 16 + lr];
+        nxi[a] = -xi[a];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        br[b] = sBr[kk][wc * 32 + b * 16 + lr];
+        bi[b] = sBi[kk][wc * 32 + b * 16 + lr];
+        ar[b] = sAr[kk][wc * 32 + b * 16 + lr];
+        ai[b] = sAi[kk][wc * 32 + b * 16 + lr];
+      }
+      // conj(x) * y = (xr yr + xi yi) + i (xr yi - xi yr)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          hre[a][b] = mfma(xr[a], br[b], hre[a][b]);
+          hre[a][b] = mfma(xi[a], bi[b], hre[a][b]);
+          him[a][b] = mfma(xr[a], bi[b], him[a][b]);
+          him[a][b] = mfma(nxi[a], br[b], him[a][b]);
+          ore[a][b] = mfma(xr[a], ar[b], ore[a][b]);
+          ore[a][b] = mfma(xi[a], ai[b], ore[a][b]);
+          oim[a][b] = mfma(xr[a], ai[b], oim[a][b]);
+          oim[a][b] = mfma(nxi[a], ar[b], oim[a][b]);
+        }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue: interstitial gather, accumulate, write upper + mirror ----
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    const int j = J0 + wc * 32 + b * 16 + lr;
+    if (j >= ngp) continue;
+    const int4 gj = ivgk[j];
+    const double vj0 = vgpc[3 * j], vj1 = vgpc[3 * j + 1], vj2 = vgpc[3 * j + 2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = I0 + wr * 32 + a * 16 + lk + 4 * r;
+        if (i >= ngp || i > j) continue;
+        double hr = hre[a][b][r], hi = him[a][b][r];
+        double orr = ore[a][b][r], oi = oim[a][b][r];
+        const size_t up = (size_t)i + (size_t)j * ldh;
+        if (accumulate) {
+          double2 h0 = H[up], o0 = O[up];
+          hr += h0.x;
+          hi += h0.y;
+          orr += o0.x;
+          oi += o0.y;
+        }
+        const int4 gi = ivgk[i];
+        if (gi.w & gj.w) {
+          const int d0 = gi.x - gj.x - lo0, d1 = gi.y - gj.y - lo1, d2 = gi.z - gj.z - lo2;
+          if ((unsigned)d0 < (unsigned)n0 && (unsigned)d1 < (unsigned)n1 &&
+              (unsigned)d2 < (unsigned)n2) {
+            const IstEntry e = T[d0 + (size_t)n0 * (d1 + (size_t)n1 * d2)];
+            const double t1 = 0.5 * (vgpc[3 * i] * vj0 + vgpc[3 * i + 1] * vj1 +
+                                     vgpc[3 * i + 2] * vj2);
+            hr += e.veff.x + t1 * e.kin.x;
+            hi += e.veff.y + t1 * e.kin.y;
+            orr += e.ocf.x + t1 * e.okin.x;
+            oi += e.ocf.y + t1 * e.okin.y;
+          }
+        }
+        H[up] = z2(hr, hi);
+        O[up] = z2(orr, oi);
+        if (i != j) {
+          const size_t lo = (size_t)j + (size_t)i * ldh;
+          H[lo] = z2(hr, -hi);
+          O[lo] = z2(orr, -oi);
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// APW-lo border (hamiltonsetup.f90:124-151, overlapsetup.f90:116-133).
+// One thread per (G, atom):  H(lo rows, G) = loa . A   (+ mirror),
+//                            O(G, lo cols) = sum_io conj(A_io,lm s_io) (+ mirror)
+// with s = oalo + h1loa.
+// ---------------------------------------------------------------------------
+template <int MAXLO>
+__global__ __launch_bounds__(64) void lo_border_kernel(
+    int ngp, const double2 *__restrict__ apwalm,
+    const long long *__restrict__ colA, const int *__restrict__ Koff,
+    const int *__restrict__ Ms, const int *__restrict__ atom_species,
+    const int *__restrict__ losize, const int *__restrict__ lobase,
+    const double2 *__restrict__ loa, int maxnlo, int maxaa,
+    const int *__restrict__ nlorb, const int *__restrict__ lorbl, int nlomax,
+    const int *__restrict__ apword, int lmaxapw, int pmax, int lmmax, int ngkmax,
+    const double *__restrict__ salo, double2 *__restrict__ H,
+    double2 *__restrict__ O, long long ldh, int accumulate) {
+  const int ias = blockIdx.y;
+  const int is = atom_species[ias];
+  const int los = losize[is];
+  const int g = blockIdx.x * 64 + threadIdx.x;
+  if (los == 0 || g >= ngp) return;
+  const int r0 = ngp + lobase[ias];
+  const long long *col = colA + Koff[ias];
+  const int M = Ms[ias];
+  const double2 *L = loa + (size_t)ias * maxnlo * maxaa;
+
+  double2 acc[MAXLO];
+#pragma unroll
+  for (int r = 0; r < MAXLO; r++) acc[r] = z2(0, 0);
+  for (int f = 0; f < M; f++) {
+    const double2 a = apwalm[col[f] + g];
+    const double2 *Lf = L + (size_t)maxnlo * f;
+#pragma unroll
+    for (int r = 0; r < MAXLO; r++)
+      if (r < los) {
+        const double2 lv = Lf[r];
+        acc[r].x += a.x * lv.x - a.y * lv.y;
+        acc[r].y += a.x * lv.y + a.y * lv.x;
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < MAXLO; r++)
+    if (r < los) {
+      const size_t e = (size_t)(r0 + r) + (size_t)g * ldh;
+      double2 v = acc[r];
+      if (accumulate) {
+        double2 h0 = H[e];
+        v.x += h0.x;
+        v.y += h0.y;
+      }
+      H[e] = v;
+      H[(size_t)g + (size_t)(r0 + r) * ldh] = z2(v.x, -v.y);
+    }
+
+  int c = r0;
+  for (int ilo = 0; ilo < nlorb[is]; ilo++) {
+    const int l = lorbl[ilo + nlomax * is];
+    const int p = apword[l + (lmaxapw + 1) * is];
+    for (int mi = 0; mi <= 2 * l; mi++, c++) {
+      const int lm = l * l + mi;
+      const size_t e = (size_t)g + (size_t)c * ldh;
+      double2 v = accumulate ? O[e] : z2(0, 0);
+      for (int io = 0; io < p; io++) {
+        const double2 a = apwalm[g + (size_t)ngkmax * (io + (size_t)pmax * (lm + (size_t)lmmax * ias))];
+        const double s = salo[io + pmax * (ilo + (size_t)nlomax * ias)];
+        v.x += a.x * s;
+        v.y += -(a.y * s);
+      }
+      O[e] = v;
+      O[(size_t)c + (size_t)g * ldh] = z2(v.x, -v.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// lo-lo block (hamiltonsetup.f90:153-161, overlapsetup.f90:134-151) over the
+// whole nlotot x nlotot corner: H += lolo (same atom), O += ololo + h1lolo
+// (same atom, same l, same m).  Upper threads write both triangles.
+// ---------------------------------------------------------------------------
+__global__ void lolo_kernel(int ngp, int nlotot, const LoRow *__restrict__ lorow,
+                            const int *__restrict__ lobase,
+                            const double2 *__restrict__ lolo, int maxnlo,
+                            const double *__restrict__ ololo_eff, int nlomax,
+                            double2 *__restrict__ H, double2 *__restrict__ O,
+                            long long ldh, int accumulate) {
+  const int r = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int c = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (r >= nlotot || c >= nlotot || r > c) return;
+  const LoRow P = lorow[r], Q = lorow[c];
+  const size_t up = (size_t)(ngp + r) + (size_t)(ngp + c) * ldh;
+  double2 hv = z2(0, 0), ov = z2(0, 0);
+  if (accumulate) {
+    hv = H[up];
+    ov = O[up];
+  }
+  if (P.ias == Q.ias) {
+    const int b = lobase[P.ias];
+    const double2 lv = lolo[(r - b) + (size_t)maxnlo * ((c - b) + (size_t)maxnlo * P.ias)];
+    hv.x += lv.x;
+    hv.y += lv.y;
+    if (P.l == Q.l && P.mi == Q.mi)
+      ov.x += ololo_eff[P.ilo + nlomax * (Q.ilo + (size_t)nlomax * P.ias)];
+  }
+  H[up] = hv;
+  O[up] = ov;
+  if (r != c) {
+    const size_t lo = (size_t)(ngp + c) + (size_t)(ngp + r) * ldh;
+    H[lo] = z2(hv.x, -hv.y);
+    O[lo] = z2(ov.x, -ov.y);
+  }
+}
+
+inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_ivgk(hipStream_t s, int ngp, const int *igpig, const int *ivg,
+                       int ngrtot, int4 *ivgk) {
+  if (ngp <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ivgk_kernel, dim3(cdiv(ngp, 256)), dim3(256), 0, s, ngp,
+                     igpig, ivg, ngrtot, ivgk);
+  return hipGetLastError();
+}
+
+hipError_t launch_ist_table(hipStream_t s, const exh_context *c,
+                            const double2 *cf, const double2 *vf,
+                            const double2 *mf, const double2 *m2, int relnone,
+                            int h1on) {
+  const int n0 = c->intgv[3] - c->intgv[0] + 1, n1 = c->intgv[4] - c->intgv[1] + 1,
+            n2 = c->intgv[5] - c->intgv[2] + 1;
+  const int nbox = n0 * n1 * n2;
+  const double alpha = 1.0 / 137.03599911, a2 = 0.5 * alpha * alpha;
+  hipLaunchKernelGGL(ist_table_kernel, dim3(cdiv(nbox, 256)), dim3(256), 0, s, nbox,
+                     (const int *)c->ivgig.p, c->ngvec, cf, vf, mf, m2, relnone,
+                     h1on, a2, (IstEntry *)c->ist.p);
+  return hipGetLastError();
+}
+
+hipError_t launch_bt(hipStream_t s, const exh_context *c, int ngp,
+                     const double2 *apwalm, double2 *bt, int ldb) {
+  if (ngp <= 0 || c->natmtot <= 0) return hipSuccess;
+  dim3 grid(cdiv(ngp, TM), cdiv(c->maxaa, TM), c->natmtot);
+  hipLaunchKernelGGL(bt_kernel, grid, dim3(NTH), 0, s, ngp, apwalm,
+                     (const long long *)c->colA.p, (const int *)c->katom.p,
+                     (const int *)c->katom.p + c->natmtot, (const double2 *)c->aa.p,
+                     c->maxaa, bt, ldb);
+  return hipGetLastError();
+}
+
+hipError_t launch_a2t(hipStream_t s, const exh_context *c, int ngp,
+                      const double2 *apwalm, double2 *a2t, int ldb) {
+  if (ngp <= 0 || c->natmtot <= 0) return hipSuccess;
+  dim3 grid(cdiv(ngp, 256), c->maxaa, c->natmtot);
+  hipLaunchKernelGGL(a2t_kernel, grid, dim3(256), 0, s, ngp, c->natmtot, apwalm,
+                     (const long long *)c->colA.p, (const int *)c->katom.p,
+                     (const int *)c->katom.p + c->natmtot,
+                     (const int *)c->katom.p + 2 * c->natmtot, (const double *)c->h1aa.p,
+                     c->apwordmax, c->lmaxapw, a2t, ldb);
+  return hipGetLastError();
+}
+
+hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
+                     const double2 *apwalm, const double2 *bt,
+                     const double2 *a2t, int ldb, const int4 *ivgk,
+                     const double *vgpc, double2 *h, double2 *o, int ldh,
+                     int flags) {
+  if (ngp <= 0) return hipSuccess;
+  const long long nt = (ngp + TM - 1) / TM;
+  const long long npairs = nt * (nt + 1) / 2;
+  const int lo0 = c->intgv[0], lo1 = c->intgv[1], lo2 = c->intgv[2];
+  const int n0 = c->intgv[3] - lo0 + 1, n1 = c->intgv[4] - lo1 + 1, n2 = c->intgv[5] - lo2 + 1;
+  const double2 *ya = a2t ? a2t : apwalm;
+  const long long *colY = a2t ? (const long long *)c->colA.p + c->Ktot
+                              : (const long long *)c->colA.p;
+  hipLaunchKernelGGL(ho_kernel, dim3((unsigned)npairs), dim3(NTH), 0, s, ngp, c->Ktot,
+                     apwalm, (const long long *)c->colA.p, bt, ldb, ya, colY, ivgk,
+                     vgpc, (const IstEntry *)c->ist.p, lo0, lo1, lo2, n0, n1, n2, h,
+                     o, (long long)ldh, flags & EXH_ACCUMULATE);
+  return hipGetLastError();
+}
+
+hipError_t launch_lo_border(hipStream_t s, const exh_context *c, int ngp,
+                            const double2 *apwalm, double2 *h, double2 *o,
+                            int ldh, int flags) {
+  if (ngp <= 0 || c->nlotot == 0) return hipSuccess;
+  dim3 grid(cdiv(ngp, 64), c->natmtot);
+  const int *ki = (const int *)c->katom.p;
+  const int *lo = (const int *)c->lobase_d.p;  // [lobase(natmtot), losize(nspecies), nlorb, lorbl]
+  const int *lobase = lo, *losz = lo + c->natmtot, *nlorb = losz + c->nspecies,
+            *lorbl = nlorb + c->nspecies;
+#define EXH_LOB(N)                                                                     \
+  hipLaunchKernelGGL(lo_border_kernel<N>, grid, dim3(64), 0, s, ngp, apwalm,           \
+                     (const long long *)c->colA.p, ki, ki + c->natmtot,                \
+                     (const int *)c->atom_species_d.p, losz, lobase,                   \
+                     (const double2 *)c->loa.p, c->maxnlo, c->maxaa, nlorb, lorbl,     \
+                     c->nlomax, (const int *)c->apword_d.p, c->lmaxapw, c->apwordmax,  \
+                     c->lmmax, c->ngkmax, (const double *)c->salo.p, h, o,             \
+                     (long long)ldh, flags & EXH_ACCUMULATE)
+  if (c->maxnlo <= 16) EXH_LOB(16);
+  else if (c->maxnlo <= 32) EXH_LOB(32);
+  else EXH_LOB(64);
+#undef EXH_LOB
+  return hipGetLastError();
+}
+
+hipError_t launch_lolo(hipStream_t s, const exh_context *c, int ngp, double2 *h,
+                       double2 *o, int ldh, int flags) {
+  if (c->nlotot == 0) return hipSuccess;
+  dim3 grid(cdiv(c->nlotot, 16), cdiv(c->nlotot, 16));
+  hipLaunchKernelGGL(lolo_kernel, grid, dim3(256), 0, s, ngp, c->nlotot,
+                     (const LoRow *)c->lorow.p, (const int *)c->lobase_d.p,
+                     (const double2 *)c->lolo.p, c->maxnlo,
+                     (const double *)c->ololo_eff.p, c->nlomax, h, o, (long long)ldh,
+                     flags & EXH_ACCUMULATE);
+  return hipGetLastError();
+}
+
+}  // namespace exh

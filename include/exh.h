@@ -1,0 +1,148 @@
+/*
+ * exh.h -- C ABI of the MI355X-native LAPW Hamiltonian/overlap setup.
+ *
+ * One library context per GPU holds the module state the reference reads from
+ * Fortran globals (basis sizes, muffin-tin tables, G-vector maps, interstitial
+ * functions) resident in HBM.  The two hot-path entry points replace
+ *
+ *   Subroutine match (ngp, gpc, tpgpc, sfacgp, apwalm)          src/match.f90:13
+ *   Subroutine hamiltonandoverlapsetup (system, ngp, apwalm, igpig, vgpc)
+ *                             src/src_eigensystem/hamiltonandoverlapsetup.f90:8
+ *
+ * with the semantics of the live pair hamiltonsetup + overlapsetup called at
+ * src/seceqnfv.f90:76-77 (src/src_eigensystem/hamiltonsetup.f90:8,
+ * overlapsetup.f90:8), including hamiltonoverlapocopy_UL.
+ *
+ * Conventions
+ *   - Arrays use the reference's column-major layouts and 1-based index
+ *     VALUES (igpig, ivgig, ivg), exactly as the Fortran caller holds them.
+ *   - Atoms are numbered species-major (idxas(ia,is) = sum natoms(<is) + ia).
+ *   - Return codes: 0 ok; < 0 invalid argument or state (EXH_E*); > 0 a HIP
+ *     runtime error code.  exh_error() returns the last message.
+ *   - Functions ending in _device take device pointers and enqueue work on
+ *     `stream` (a hipStream_t, NULL = the context's stream) without host
+ *     synchronisation.  The others take host pointers and return when the
+ *     result is back in host memory.
+ *   - A context is used by one host thread at a time.
+ */
+#ifndef EXH_H
+#define EXH_H
+
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+typedef struct {
+  double re, im;
+} exh_z;
+#else
+typedef double _Complex exh_z;
+#endif
+
+typedef struct exh_context exh_context;
+
+enum {
+  EXH_OK = 0,
+  EXH_EINVAL = -1,   /* bad argument                         */
+  EXH_ESTATE = -2,   /* required set_* call missing           */
+  EXH_ERANGE = -3,   /* size beyond what the kernels support  */
+  EXH_ENOTSUP = -4,  /* option not implemented                */
+  EXH_EMATCH = -5    /* singular boundary-derivative matrix   */
+};
+
+/* Flags for exh_setup_device / exh_hamiltonandoverlapsetup_ex. */
+enum {
+  /* Add to the H and O already in memory, as the reference does (zgemm with
+   * beta=1 and += gathers; the upper triangle and the lo rows below the APW
+   * block are read).  Without it H and O are overwritten, which equals the
+   * reference when called right after newsystem() zeroed them
+   * (seceqnfv.f90:73). */
+  EXH_ACCUMULATE = 1
+};
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart: the reference keeps this state in modules. */
+int exh_create(int device, exh_context **ctx);
+void exh_destroy(exh_context *ctx);
+const char *exh_error(const exh_context *ctx);
+/* The context's own stream (hipStream_t), created at default priority. */
+void *exh_stream(exh_context *ctx);
+
+/* ---- module state ------------------------------------------------------- */
+/* Basis layout: mod_atoms (nspecies, natoms), mod_APW_LO (apwordmax, apword,
+ * nlorb, lorbl), input%groundstate%lmaxapw / lmaxmat, mod_Gkvector ngkmax.
+ * apword is apword(0:lmaxapw, nspecies); lorbl is lorbl(nlomax, nspecies).
+ * Derived here as the reference derives them: idxlo (genidxlo.f90:27-45),
+ * losize/maxnlo (mod_eigensystem.F90:206-230), maxaa (MaxAPWs :151-184). */
+int exh_set_basis(exh_context *ctx, int nspecies, const int *natoms,
+                  int lmaxapw, int lmaxmat, int apwordmax, const int *apword,
+                  int nlomax, const int *nlorb, const int *lorbl, int ngkmax);
+
+/* Radial boundary data read by match (match.f90:141-148): omega
+ * (mod_lattice), rmt(nspecies), nrmt(nspecies), spr(nrmtmax, nspecies) and
+ * apwfr(nrmtmax, 2, apwordmax, 0:lmaxapw, natmtot) (mod_APW_LO; only the last
+ * max(apwordmax+1,4) points of component 1 are read).  Changes every SCF
+ * iteration (genapwfr, scf_cycle.f90:210). */
+int exh_set_radial(exh_context *ctx, double omega, const double *rmt,
+                   const int *nrmt, int nrmtmax, const double *spr,
+                   const double *apwfr);
+
+/* Muffin-tin tables for the current SCF iteration (hmlint/olprad,
+ * scf_cycle.f90:210-222): mt_hscf%main%aa(maxaa,maxaa,natmtot),
+ * %loa(maxnlo,maxaa,natmtot), %lolo(maxnlo,maxnlo,natmtot) (complex);
+ * oalo(apwordmax,nlomax,natmtot), ololo(nlomax,nlomax,natmtot) (real);
+ * h1aa(apwordmax,apwordmax,0:lmaxapw,natmtot), h1loa(apwordmax,nlomax,natmtot),
+ * h1lolo(nlomax,nlomax,natmtot) (real, NULL = zero, the non-iora* case).
+ * maxaa and maxnlo must equal the values exh_set_basis derived. */
+int exh_set_mt(exh_context *ctx, int maxaa, int maxnlo, const exh_z *aa,
+               const exh_z *loa, const exh_z *lolo, const double *oalo,
+               const double *ololo, const double *h1aa, const double *h1loa,
+               const double *h1lolo);
+
+/* G-vector maps (gengvec.f90): ivg(3,ngrtot), the ivgig box with bounds
+ * intgv(3,2) (column-major: lo1,lo2,lo3,hi1,hi2,hi3), ngvec.  Once per
+ * structure. */
+int exh_set_gvectors(exh_context *ctx, int ngrtot, int ngvec, const int *ivg,
+                     const int *intgv, const int *ivgig);
+
+/* Interstitial functions for the current SCF iteration: cfunig(ngrtot),
+ * veffig(ngvec) (genveffig), meffig(ngvec) read when valence_rel_none == 0
+ * (hmlistln.f90:61-71), m2effig(ngvec) read when h1on != 0
+ * (olpistln.f90:53-66). */
+int exh_set_interstitial(exh_context *ctx, const exh_z *cfunig,
+                         const exh_z *veffig, const exh_z *meffig,
+                         const exh_z *m2effig, int valence_rel_none, int h1on);
+
+/* ---- hot path ----------------------------------------------------------- */
+/* match.f90:13-194.  gpc(ngkmax), tpgpc(2,ngkmax), sfacgp(ngkmax,natmtot) in;
+ * apwalm(ngkmax,apwordmax,(lmaxapw+1)^2,natmtot) out (entries with io beyond
+ * apword(l) are left untouched, as in the reference). */
+int exh_match(exh_context *ctx, int ngp, const double *gpc,
+              const double *tpgpc, const exh_z *sfacgp, exh_z *apwalm);
+int exh_match_device(exh_context *ctx, void *stream, int ngp,
+                     const double *gpc, const double *tpgpc,
+                     const exh_z *sfacgp, exh_z *apwalm);
+
+/* hamiltonandoverlapsetup.f90:8 signature with hamiltonsetup.f90 +
+ * overlapsetup.f90 semantics.  `system` unpacks to rank = nmat (the order of
+ * H and O, ngp + nlotot) and the two rank x rank column-major matrices
+ * system%hamilton%za and system%overlap%za.  apwalm(ngkmax,...) as produced
+ * by match, igpig(ngkmax) (1-based G-vector indices), vgpc(3,ngkmax).  H and
+ * O are accumulated into (reference semantics). */
+int exh_hamiltonandoverlapsetup(exh_context *ctx, int ngp, int rank,
+                                const exh_z *apwalm, const int *igpig,
+                                const double *vgpc, exh_z *h, exh_z *o);
+/* Same with EXH_* flags (0 = overwrite). */
+int exh_hamiltonandoverlapsetup_ex(exh_context *ctx, int ngp, int rank,
+                                   const exh_z *apwalm, const int *igpig,
+                                   const double *vgpc, exh_z *h, exh_z *o,
+                                   int flags);
+/* Device-resident form: every pointer is device memory, ldh = rank. */
+int exh_setup_device(exh_context *ctx, void *stream, int ngp, int rank,
+                     const exh_z *apwalm, const int *igpig, const double *vgpc,
+                     exh_z *h, exh_z *o, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

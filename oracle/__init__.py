@@ -1,0 +1,208 @@
+"""ctypes front end of the oracle (test infrastructure only).
+
+The oracle is the CPU restatement of exciting's H/O setup in oracle.c.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module; it is the checker and the CPU baseline, never the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_srclib.so")
+
+_i = C.c_int
+_d = C.c_double
+_p = C.c_void_p
+
+
+class OrcSystem(C.Structure):
+    _fields_ = [
+        ("nspecies", _i), ("natmtot", _i), ("natoms", _p),
+        ("lmaxapw", _i), ("lmaxmat", _i), ("apwordmax", _i), ("apword", _p),
+        ("nlomax", _i), ("nlorb", _p), ("lorbl", _p), ("ngkmax", _i),
+        ("omega", _d), ("rmt", _p), ("nrmt", _p), ("nrmtmax", _i), ("spr", _p),
+        ("apwfr", _p),
+        ("maxaa", _i), ("maxnlo", _i), ("aa", _p), ("loa", _p), ("lolo", _p),
+        ("oalo", _p), ("ololo", _p), ("h1aa", _p), ("h1loa", _p), ("h1lolo", _p),
+        ("ngrtot", _i), ("ngvec", _i), ("ivg", _p), ("intgv", _i * 6),
+        ("ivgig", _p), ("cfunig", _p), ("veffig", _p), ("meffig", _p),
+        ("m2effig", _p), ("valence_rel_none", _i), ("h1on", _i),
+    ]
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so (and _ref when the reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build(ref=False)
+        L = C.CDLL(LIB_PATH)
+        L.orc_sbessel.argtypes = [_i, _d, _p]
+        L.orc_sbesseldm.argtypes = [_i, _i, _d, _p]
+        L.orc_genylm.argtypes = [_i, _p, _p]
+        L.orc_polynom.argtypes = [_i, _i, _p, _p, _p, _d]
+        L.orc_polynom.restype = _d
+        L.orc_sphcrd.argtypes = [_p, _p, _p]
+        L.orc_factnm.argtypes = [_i, _i]
+        L.orc_factnm.restype = _d
+        L.orc_factr.argtypes = [_i, _i]
+        L.orc_factr.restype = _d
+        L.orc_zgesv.argtypes = [_i, _i, _p, _i, _p, _p, _i]
+        L.orc_match.argtypes = [_p, _i, _p, _p, _p, _p]
+        for name in ("orc_hamiltonandoverlapsetup",):
+            getattr(L, name).argtypes = [_p, _i, _p, _p, _p, _i, _p, _p, _p]
+        for name in ("orc_hamiltonsetup", "orc_overlapsetup"):
+            getattr(L, name).argtypes = [_p, _i, _p, _p, _p, _i, _p, _p]
+        for name in ("orc_hmlistln", "orc_olpistln"):
+            getattr(L, name).argtypes = [_p, _i, _p, _p, _i, _p]
+        L.orc_copy_ul.argtypes = [_i, _p, _p]
+        L.orc_set_fault.argtypes = [_i]
+        L.orc_nlotot.argtypes = [_p]
+        L.orc_maxapw.argtypes = [_p]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own src_lib routines compiled by flang, or None."""
+    if not os.path.exists(REF_PATH):
+        return None
+    R = C.CDLL(REF_PATH)
+    return R
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class System:
+    """Holds an OrcSystem and keeps the numpy arrays it points at alive."""
+
+    def __init__(self, s):
+        self.s = s
+        keep = []
+
+        def arr(x, dt):
+            if x is None:
+                return None
+            y = np.asfortranarray(x, dtype=dt)
+            keep.append(y)
+            return y.ctypes.data
+
+        o = OrcSystem()
+        o.nspecies = s.nspecies
+        o.natmtot = s.natmtot
+        o.natoms = arr(s.natoms, np.int32)
+        o.lmaxapw = s.lmaxapw
+        o.lmaxmat = s.lmaxmat
+        o.apwordmax = s.apwordmax
+        o.apword = arr(s.apword, np.int32)
+        o.nlomax = s.nlomax
+        o.nlorb = arr(s.nlorb, np.int32)
+        o.lorbl = arr(s.lorbl, np.int32)
+        o.ngkmax = s.ngkmax
+        o.omega = s.omega
+        o.rmt = arr(s.rmt, np.float64)
+        o.nrmt = arr(s.nrmt, np.int32)
+        o.nrmtmax = s.nrmtmax
+        o.spr = arr(s.spr, np.float64)
+        o.apwfr = arr(s.apwfr, np.float64)
+        o.maxaa = s.maxaa
+        o.maxnlo = s.maxnlo
+        o.aa = arr(s.aa, np.complex128)
+        o.loa = arr(s.loa, np.complex128)
+        o.lolo = arr(s.lolo, np.complex128)
+        o.oalo = arr(s.oalo, np.float64)
+        o.ololo = arr(s.ololo, np.float64)
+        o.h1aa = arr(s.h1aa, np.float64)
+        o.h1loa = arr(s.h1loa, np.float64)
+        o.h1lolo = arr(s.h1lolo, np.float64)
+        o.ngrtot = s.ngrtot
+        o.ngvec = s.ngvec
+        o.ivg = arr(s.ivg, np.int32)
+        iv = np.asfortranarray(s.intgv, dtype=np.int32).ravel(order="F")
+        for k in range(6):
+            o.intgv[k] = int(iv[k])
+        o.ivgig = arr(s.ivgig, np.int32)
+        o.cfunig = arr(s.cfunig, np.complex128)
+        o.veffig = arr(s.veffig, np.complex128)
+        o.meffig = arr(s.meffig, np.complex128)
+        o.m2effig = arr(s.m2effig, np.complex128)
+        o.valence_rel_none = 1 if s.valence_rel_none else 0
+        o.h1on = 1 if s.h1on else 0
+        self.o = o
+        self._keep = keep
+
+    @property
+    def ptr(self):
+        return C.addressof(self.o)
+
+
+def scipy_zgemm():
+    """Function pointer of the optimised zgemm bundled with SciPy (OpenBLAS),
+    or None."""
+    try:
+        import scipy.linalg.cython_blas as cb
+    except Exception:
+        return None
+    cap = cb.__pyx_capi__["zgemm"]
+    get = C.pythonapi.PyCapsule_GetPointer
+    get.restype = C.c_void_p
+    get.argtypes = [C.py_object, C.c_char_p]
+    name = C.pythonapi.PyCapsule_GetName
+    name.restype = C.c_char_p
+    name.argtypes = [C.py_object]
+    return get(cap, name(cap))
+
+
+def match(sys_, ik: int) -> np.ndarray:
+    """match.f90 for k-point ik: apwalm(ngkmax, apwordmax, lmmaxapw, natmtot)."""
+    s = sys_.s
+    k = s.kpts[ik]
+    lmm = (s.lmaxapw + 1) ** 2
+    apwalm = np.zeros((s.ngkmax, s.apwordmax, lmm, s.natmtot), dtype=np.complex128, order="F")
+    gpc = np.ascontiguousarray(k.gpc)
+    tp = np.asfortranarray(k.tpgpc)
+    sf = np.asfortranarray(k.sfacgp)
+    rc = lib().orc_match(sys_.ptr, k.ngk, gpc.ctypes.data, tp.ctypes.data, sf.ctypes.data,
+                         apwalm.ctypes.data)
+    if rc < 0:
+        raise RuntimeError(f"orc_match failed: {rc}")
+    return apwalm
+
+
+def setup(sys_, ik: int, apwalm: np.ndarray, fast: bool = True):
+    """hamiltonsetup + overlapsetup (+UL copy) for k-point ik: (H, O)."""
+    s = sys_.s
+    k = s.kpts[ik]
+    n = k.ngk + s.nlotot
+    h = np.zeros((n, n), dtype=np.complex128, order="F")
+    o = np.zeros((n, n), dtype=np.complex128, order="F")
+    zg = scipy_zgemm() if fast else None
+    a = np.asfortranarray(apwalm)
+    ig = np.ascontiguousarray(k.igpig, dtype=np.int32)
+    vg = np.asfortranarray(k.vgpc)
+    rc = lib().orc_hamiltonandoverlapsetup(sys_.ptr, k.ngk, a.ctypes.data, ig.ctypes.data,
+                                           vg.ctypes.data, n, h.ctypes.data, o.ctypes.data, zg)
+    if rc != 0:
+        raise RuntimeError(f"orc_hamiltonandoverlapsetup failed: {rc}")
+    return h, o
+
+
+def set_fault(code: int) -> None:
+    lib().orc_set_fault(code)

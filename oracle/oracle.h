@@ -1,0 +1,142 @@
+/*
+ * oracle.h -- CPU restatement of exciting's LAPW Hamiltonian/overlap setup.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path in
+ * exciting_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only to compute expected values or to time
+ * the CPU baseline.  No product code path links, loads or falls back to it.
+ *
+ * Every routine follows one reference routine, cited file:line against the
+ * exciting source tree.  Arrays use the reference's column-major layouts and
+ * keep its 1-based index VALUES (igpig, ivg, ivgig hold Fortran indices).
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   orc_sbessel, orc_sbesseldm : scipy.special.spherical_jn (+ derivative via
+ *                                the recurrence identity), the compiled
+ *                                reference routines in oracle/_ref, and the
+ *                                x<1e-8 / x<l / x>=l branch special cases.
+ *   orc_genylm                 : scipy.special.sph_harm (Condon-Shortley), the
+ *                                compiled reference genylm, orthonormality.
+ *   orc_polynom                : exactness on cubics, the compiled reference.
+ *   orc_sphcrd                 : the compiled reference, round trip r(tp)->v.
+ *   orc_match                  : the continuity identity sum_j D_ij A_j = b_i,
+ *                                the Rayleigh plane-wave identity at r=R.
+ *   orc_hamiltonandoverlapsetup: brute-force numpy triple loops on tiny
+ *                                inputs, Hermiticity, O = A^H A + ... >= 0, the
+ *                                empty-cell closed form, and a planted-mistake
+ *                                test (tests/test_oracle_setup.py).
+ */
+#ifndef EXH_ORACLE_H
+#define EXH_ORACLE_H
+
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double _Complex orc_z;
+
+/* Fortran-ABI zgemm (as exported by scipy.linalg.cython_blas, no hidden
+ * string lengths).  NULL selects the oracle's own netlib-ordered loops. */
+typedef void (*orc_zgemm_t)(const char *transa, const char *transb,
+                            const int *m, const int *n, const int *k,
+                            const orc_z *alpha, const orc_z *a, const int *lda,
+                            const orc_z *b, const int *ldb, const orc_z *beta,
+                            orc_z *c, const int *ldc);
+
+/* Module state the reference routines read (mod_atoms, mod_APW_LO,
+ * mod_muffin_tin, mod_eigensystem, mod_Gvector, mod_potential_and_density).
+ * Atoms are numbered species-major: ias = sum_{is'<is} natoms(is') + ia. */
+typedef struct orc_system {
+  int nspecies, natmtot;
+  const int *natoms;         /* [nspecies]                                   */
+  int lmaxapw, lmaxmat;      /* lmaxmat <= lmaxapw                            */
+  int apwordmax;
+  const int *apword;         /* apword(0:lmaxapw, nspecies)                   */
+  int nlomax;
+  const int *nlorb;          /* nlorb(nspecies)                               */
+  const int *lorbl;          /* lorbl(nlomax, nspecies)                       */
+  int ngkmax;                /* leading dimension of apwalm, igpig, vgpc      */
+  /* match data */
+  double omega;
+  const double *rmt;         /* rmt(nspecies)                                 */
+  const int *nrmt;           /* nrmt(nspecies)                                */
+  int nrmtmax;
+  const double *spr;         /* spr(nrmtmax, nspecies)                        */
+  const double *apwfr;       /* apwfr(nrmtmax, 2, apwordmax, 0:lmaxapw, natmtot) */
+  /* muffin-tin tables (mt_hscf%main) and overlap radial integrals */
+  int maxaa, maxnlo;
+  const orc_z *aa;           /* aa(maxaa, maxaa, natmtot)                     */
+  const orc_z *loa;          /* loa(maxnlo, maxaa, natmtot)                   */
+  const orc_z *lolo;         /* lolo(maxnlo, maxnlo, natmtot)                 */
+  const double *oalo;        /* oalo(apwordmax, nlomax, natmtot)              */
+  const double *ololo;       /* ololo(nlomax, nlomax, natmtot)                */
+  const double *h1aa;        /* h1aa(apwordmax,apwordmax,0:lmaxapw,natmtot) or NULL */
+  const double *h1loa;       /* h1loa(apwordmax, nlomax, natmtot) or NULL     */
+  const double *h1lolo;      /* h1lolo(nlomax, nlomax, natmtot) or NULL       */
+  /* interstitial */
+  int ngrtot, ngvec;
+  const int *ivg;            /* ivg(3, ngrtot)                                */
+  int intgv[6];              /* intgv(3,2) column-major: lo1,lo2,lo3,hi1,hi2,hi3 */
+  const int *ivgig;          /* ivgig(intgv(1,1):intgv(1,2), ...)             */
+  const orc_z *cfunig;       /* cfunig(ngrtot)                                */
+  const orc_z *veffig;       /* veffig(ngvec)                                 */
+  const orc_z *meffig;       /* meffig(ngvec), read when valence_rel_none==0  */
+  const orc_z *m2effig;      /* m2effig(ngvec), read when h1on!=0             */
+  int valence_rel_none;      /* input%groundstate%ValenceRelativity == 'none' */
+  int h1on;                  /* ValenceRelativity == 'iora*'                  */
+} orc_system;
+
+/* src/src_lib helpers */
+int orc_sbessel(int lmax, double x, double *jl);
+int orc_sbesseldm(int m, int lmax, double x, double *djl);
+int orc_genylm(int lmax, const double *tp, orc_z *ylm);
+double orc_polynom(int m, int np, const double *xa, const double *ya, double *c,
+                   double x);
+void orc_sphcrd(const double *v, double *r, double *tp);
+double orc_factnm(int n, int m);
+double orc_factr(int n, int d);
+
+/* zgesv with LAPACK 3.4.0 zgetf2/zgetrs semantics; returns info. */
+int orc_zgesv(int n, int nrhs, orc_z *a, int lda, int *ipiv, orc_z *b, int ldb);
+
+/* Derived sizes: losize(is), idxlo(lm, ilo, ias) (0 if unused), nlotot,
+ * following mod_eigensystem.F90 MTInitAll and genidxlo.f90. */
+int orc_losize(const orc_system *s, int is);
+int orc_nlotot(const orc_system *s);
+int orc_maxapw(const orc_system *s);
+
+/* match.f90:13-194 */
+int orc_match(const orc_system *s, int ngp, const double *gpc,
+              const double *tpgpc, const orc_z *sfacgp, orc_z *apwalm);
+
+/* hamiltonsetup.f90 then overlapsetup.f90 (incl. hamiltonoverlapocopy_UL),
+ * the live pair called at seceqnfv.f90:76-77.  h and o are rank x rank
+ * column-major and are accumulated into, as the reference does. */
+int orc_hamiltonandoverlapsetup(const orc_system *s, int ngp,
+                                const orc_z *apwalm, const int *igpig,
+                                const double *vgpc, int rank, orc_z *h,
+                                orc_z *o, orc_zgemm_t zgemm);
+
+/* Individual stages, exposed for the planted-mistake and stage tests. */
+int orc_hamiltonsetup(const orc_system *s, int ngp, const orc_z *apwalm,
+                      const int *igpig, const double *vgpc, int rank, orc_z *h,
+                      orc_zgemm_t zgemm);
+int orc_overlapsetup(const orc_system *s, int ngp, const orc_z *apwalm,
+                     const int *igpig, const double *vgpc, int rank, orc_z *o,
+                     orc_zgemm_t zgemm);
+void orc_hmlistln(const orc_system *s, int ngp, const int *igpig,
+                  const double *vgpc, int rank, orc_z *h);
+void orc_olpistln(const orc_system *s, int ngp, const int *igpig,
+                  const double *vgpc, int rank, orc_z *o);
+void orc_copy_ul(int rank, orc_z *h, orc_z *o);
+
+/* Planted-mistake switch for tests: 0 = faithful.  Nonzero values inject one
+ * deliberate error so tests can show the checks catch it. */
+void orc_set_fault(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,37 @@
+"""The C ABI library loads and exports every function include/exh.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from exciting_amd import lapw
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "exh.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(exh_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "exh_hamiltonandoverlapsetup" in names and "exh_match" in names
+    assert set(names) == set(lapw.EXPORTS)
+
+
+def test_library_exports_all_symbols():
+    if not os.path.exists(lapw.LIB_PATH):
+        pytest.fail("libexh.so not built (python -m exciting_amd.build)")
+    L = ctypes.CDLL(lapw.LIB_PATH)
+    for n in declared():
+        assert hasattr(L, n), n
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(lapw, "_lib", None)
+    monkeypatch.setattr(lapw, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(lapw.ExhError, match="not built"):
+        lapw.lib()
