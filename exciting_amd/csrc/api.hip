@@ -169,6 +169,45 @@ static int build_match_tables(exh_context *c) {
   return rc;
 }
 
+static hipEvent_t take_event(exh_context *c) {
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Bracket one launch with events when profiling is on.
+template <typename F>
+static hipError_t timed(exh_context *c, int k, hipStream_t s, F &&launch) {
+  if (!c->prof) return launch();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, s);
+  if (e == hipSuccess) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(b, s);
+  c->recs.push_back({k, a, b});
+  return e;
+}
+
+static int drain_records(exh_context *c) {
+  for (auto &r : c->recs) {
+    hipError_t e = hipEventSynchronize(r.b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
+    if (e != hipSuccess) return hip_fail(c, e, "profile");
+    c->prof_ms[r.k] += ms;
+    c->prof_n[r.k] += 1;
+    c->evpool.push_back(r.a);
+    c->evpool.push_back(r.b);
+  }
+  c->recs.clear();
+  return 0;
+}
+
 }  // namespace exh
 
 using namespace exh;
@@ -206,6 +245,11 @@ void exh_destroy(exh_context *c) {
                     &c->m2_stage, &c->bt, &c->a2t, &c->ivgk, &c->s_apwalm, &c->s_igpig,
                     &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac};
   for (DevBuf *b : bufs) release(*b);
+  for (auto &r : c->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -445,7 +489,9 @@ int exh_match_device(exh_context *c, void *stream, int ngp, const double *gpc,
   if (ngp > 0 && (!gpc || !tpgpc || !sfacgp || !apwalm))
     return fail(c, EXH_EINVAL, "match: null array");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  hipError_t e = launch_match(s, c, ngp, gpc, tpgpc, (const double2 *)sfacgp, (double2 *)apwalm);
+  hipError_t e = timed(c, EXH_K_MATCH, s, [&] {
+    return launch_match(s, c, ngp, gpc, tpgpc, (const double2 *)sfacgp, (double2 *)apwalm);
+  });
   return e == hipSuccess ? 0 : hip_fail(c, e, "match kernel");
 }
 
@@ -467,15 +513,25 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
   if (!rc) rc = ensure(c, c->ivgk, (size_t)std::max(c->ngkmax, 1) * sizeof(int4));
   if (rc) return rc;
   const double2 *A = (const double2 *)apwalm;
-  hipError_t e = launch_ivgk(s, ngp, igpig, (const int *)c->ivg.p, c->ngrtot, (int4 *)c->ivgk.p);
-  if (e == hipSuccess) e = launch_bt(s, c, ngp, A, (double2 *)c->bt.p, ldb);
-  if (e == hipSuccess && c->have_h1aa) e = launch_a2t(s, c, ngp, A, (double2 *)c->a2t.p, ldb);
+  hipError_t e = timed(c, EXH_K_IVGK, s, [&] {
+    return launch_ivgk(s, ngp, igpig, (const int *)c->ivg.p, c->ngrtot, (int4 *)c->ivgk.p);
+  });
   if (e == hipSuccess)
-    e = launch_ho(s, c, ngp, A, (const double2 *)c->bt.p,
-                  c->have_h1aa ? (const double2 *)c->a2t.p : nullptr, ldb,
-                  (const int4 *)c->ivgk.p, vgpc, (double2 *)h, (double2 *)o, rank, flags);
-  if (e == hipSuccess) e = launch_lo_border(s, c, ngp, A, (double2 *)h, (double2 *)o, rank, flags);
-  if (e == hipSuccess) e = launch_lolo(s, c, ngp, (double2 *)h, (double2 *)o, rank, flags);
+    e = timed(c, EXH_K_BT, s, [&] { return launch_bt(s, c, ngp, A, (double2 *)c->bt.p, ldb); });
+  if (e == hipSuccess && c->have_h1aa)
+    e = timed(c, EXH_K_A2T, s, [&] { return launch_a2t(s, c, ngp, A, (double2 *)c->a2t.p, ldb); });
+  if (e == hipSuccess)
+    e = timed(c, EXH_K_HO, s, [&] {
+      return launch_ho(s, c, ngp, A, (const double2 *)c->bt.p,
+                       c->have_h1aa ? (const double2 *)c->a2t.p : nullptr, ldb,
+                       (const int4 *)c->ivgk.p, vgpc, (double2 *)h, (double2 *)o, rank, flags);
+    });
+  if (e == hipSuccess)
+    e = timed(c, EXH_K_LOBORDER, s, [&] {
+      return launch_lo_border(s, c, ngp, A, (double2 *)h, (double2 *)o, rank, flags);
+    });
+  if (e == hipSuccess)
+    e = timed(c, EXH_K_LOLO, s, [&] { return launch_lolo(s, c, ngp, (double2 *)h, (double2 *)o, rank, flags); });
   return e == hipSuccess ? 0 : hip_fail(c, e, "setup kernels");
 }
 
@@ -506,8 +562,10 @@ int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
     e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
                          ngp * sizeof(double2), ncol, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(c, e, "match: upload");
-  e = launch_match(s, c, ngp, (const double *)c->s_gpc.p, (const double *)c->s_tpgpc.p,
-                   (const double2 *)c->s_sfac.p, (double2 *)c->s_apwalm.p);
+  e = timed(c, EXH_K_MATCH, s, [&] {
+    return launch_match(s, c, ngp, (const double *)c->s_gpc.p, (const double *)c->s_tpgpc.p,
+                        (const double2 *)c->s_sfac.p, (double2 *)c->s_apwalm.p);
+  });
   if (e == hipSuccess)
     e = hipMemcpy2DAsync(apwalm, ld * sizeof(double2), c->s_apwalm.p, ld * sizeof(double2),
                          ngp * sizeof(double2), ncol, hipMemcpyDeviceToHost, s);
@@ -555,6 +613,32 @@ int exh_hamiltonandoverlapsetup_ex(exh_context *c, int ngp, int rank, const exh_
   if (e == hipSuccess) e = hipMemcpyAsync(o, c->s_o.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "setup: download");
+}
+
+int exh_profile_enable(exh_context *c, int on) {
+  EXH_CHECK_CTX(c);
+  c->prof = on != 0;
+  return 0;
+}
+
+int exh_profile_read(exh_context *c, int kernel, double *total_ms, int *launches) {
+  EXH_CHECK_CTX(c);
+  if (kernel < 0 || kernel >= EXH_K_COUNT) return fail(c, EXH_EINVAL, "profile_read: kernel id");
+  int rc = drain_records(c);
+  if (rc) return rc;
+  if (total_ms) *total_ms = c->prof_ms[kernel];
+  if (launches) *launches = c->prof_n[kernel];
+  return 0;
+}
+
+int exh_profile_reset(exh_context *c) {
+  EXH_CHECK_CTX(c);
+  int rc = drain_records(c);
+  for (int k = 0; k < EXH_K_COUNT; k++) {
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  return rc;
 }
 
 int exh_hamiltonandoverlapsetup(exh_context *c, int ngp, int rank, const exh_z *apwalm,

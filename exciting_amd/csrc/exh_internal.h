@@ -83,6 +83,17 @@ struct exh_context {
   exh::DevBuf ivg, ivgig, ist;  // ist: IstEntry per box point
   exh::DevBuf cf_stage, vf_stage, mf_stage, m2_stage;
 
+  // instrumentation: (kernel id, start, stop) per recorded launch
+  bool prof = false;
+  struct Rec {
+    int k;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> evpool;
+  double prof_ms[EXH_K_COUNT] = {0};
+  int prof_n[EXH_K_COUNT] = {0};
+
   // per-call scratch
   exh::DevBuf bt, a2t, ivgk;
   // host-API staging

@@ -73,6 +73,9 @@ def lib():
     L.exh_hamiltonandoverlapsetup.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
     L.exh_hamiltonandoverlapsetup_ex.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p, _i]
     L.exh_setup_device.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
+    L.exh_profile_enable.argtypes = [_p, _i]
+    L.exh_profile_read.argtypes = [_p, _i, C.POINTER(_d), C.POINTER(_i)]
+    L.exh_profile_reset.argtypes = [_p]
     _lib = L
     return L
 
@@ -81,8 +84,11 @@ EXPORTS = (
     "exh_create", "exh_destroy", "exh_error", "exh_stream", "exh_set_basis",
     "exh_set_radial", "exh_set_mt", "exh_set_gvectors", "exh_set_interstitial",
     "exh_match", "exh_match_device", "exh_hamiltonandoverlapsetup",
-    "exh_hamiltonandoverlapsetup_ex", "exh_setup_device",
+    "exh_hamiltonandoverlapsetup_ex", "exh_setup_device", "exh_profile_enable",
+    "exh_profile_read", "exh_profile_reset",
 )
+
+KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo")
 
 
 def _is_torch(a) -> bool:
@@ -172,6 +178,23 @@ class Context:
     @property
     def stream(self) -> int:
         return lib().exh_stream(self._h)
+
+    # ---- instrumentation ------------------------------------------------
+    def profile(self, on: bool = True):
+        self._check(lib().exh_profile_enable(self._h, 1 if on else 0), "exh_profile_enable")
+
+    def profile_reset(self):
+        self._check(lib().exh_profile_reset(self._h), "exh_profile_reset")
+
+    def profile_read(self):
+        """{kernel name: (total device ms, launches)} since the last reset."""
+        out = {}
+        for k, name in enumerate(KERNELS):
+            ms, n = _d(), _i()
+            self._check(lib().exh_profile_read(self._h, k, C.byref(ms), C.byref(n)),
+                        "exh_profile_read")
+            out[name] = (ms.value, n.value)
+        return out
 
     # ---- module state ---------------------------------------------------
     def set_basis(self, natoms, lmaxapw, lmaxmat, apwordmax, apword, nlomax, nlorb,

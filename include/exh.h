@@ -142,6 +142,26 @@ int exh_setup_device(exh_context *ctx, void *stream, int ngp, int rank,
                      const exh_z *apwalm, const int *igpig, const double *vgpc,
                      exh_z *h, exh_z *o, int flags);
 
+/* ---- instrumentation ----------------------------------------------------- */
+/* No reference counterpart (the reference accumulates wall-clock timers such
+ * as timemat, hamiltonsetup.f90:190).  When enabled, every hot-path kernel
+ * launch is bracketed by HIP events on the stream it runs on; read returns
+ * the summed device time (ms) and launch count of one kernel since the last
+ * reset, synchronising on the recorded events. */
+enum {
+  EXH_K_MATCH = 0,    /* match kernel                                  */
+  EXH_K_IVGK = 1,     /* G+k -> G index prep                           */
+  EXH_K_BT = 2,       /* aa . A per atom (FP64 MFMA)                    */
+  EXH_K_A2T = 3,      /* (1 + h1aa) . A (iora* only)                    */
+  EXH_K_HO = 4,       /* fused upper H/O contraction + interstitial     */
+  EXH_K_LOBORDER = 5, /* APW-lo rows/columns                           */
+  EXH_K_LOLO = 6,     /* lo-lo block                                   */
+  EXH_K_COUNT = 7
+};
+int exh_profile_enable(exh_context *ctx, int on);
+int exh_profile_read(exh_context *ctx, int kernel, double *total_ms, int *launches);
+int exh_profile_reset(exh_context *ctx);
+
 #ifdef __cplusplus
 }
 #endif
