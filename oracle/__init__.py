@@ -38,7 +38,7 @@ class OrcSystem(C.Structure):
 
 def build(ref: bool = True) -> None:
     """Compile liboracle.so (and _ref when the reference tree is present)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "all"], check=True)
     if ref:
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
