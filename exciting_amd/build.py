@@ -27,6 +27,10 @@ def flags():
     return ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             # packed FP32 VALU ops stay off in all device code
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            # keep MFMA accumulators in the unified VGPR file: the default
+            # heuristic copies every loop-carried accumulator AGPR<->VGPR per
+            # iteration of the K loop
+            "-mllvm", "-amdgpu-mfma-vgpr-form=1",
             "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
             "-I", os.path.join(HERE, "..", "include")]
 

@@ -242,7 +242,7 @@ void exh_destroy(exh_context *c) {
                     &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
                     &c->aa, &c->loa, &c->lolo, &c->salo, &c->ololo_eff, &c->h1aa,
                     &c->ivg, &c->ivgig, &c->ist, &c->cf_stage, &c->vf_stage, &c->mf_stage,
-                    &c->m2_stage, &c->bt, &c->a2t, &c->ivgk, &c->s_apwalm, &c->s_igpig,
+                    &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->s_apwalm, &c->s_igpig,
                     &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac};
   for (DevBuf *b : bufs) release(*b);
   for (auto &r : c->recs) {
@@ -345,6 +345,9 @@ int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, 
     }
   c->Ktot = k;
   c->nlotot = lob;
+  c->contig = true;
+  for (int kk = 0; kk < c->Ktot; kk++)
+    if (c->colA_h[kk] != (int64_t)kk * ngkmax) c->contig = false;
   // column table: [colA (Ktot) | k * ldb (Ktot)] with ldb fixed by ngkmax
   const int64_t ldb = ((int64_t)ngkmax + 63) / 64 * 64;
   std::vector<int64_t> cols(c->colA_h);
@@ -510,12 +513,15 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
   const int ldb = (c->ngkmax + 63) / 64 * 64;
   int rc = ensure(c, c->bt, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
   if (!rc && c->have_h1aa) rc = ensure(c, c->a2t, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
+  if (!rc && !c->contig) rc = ensure(c, c->at, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
   if (!rc) rc = ensure(c, c->ivgk, (size_t)std::max(c->ngkmax, 1) * sizeof(int4));
   if (rc) return rc;
   const double2 *A = (const double2 *)apwalm;
   hipError_t e = timed(c, EXH_K_IVGK, s, [&] {
     return launch_ivgk(s, ngp, igpig, (const int *)c->ivg.p, c->ngrtot, (int4 *)c->ivgk.p);
   });
+  if (e == hipSuccess && !c->contig)
+    e = timed(c, EXH_K_PACK, s, [&] { return launch_pack(s, c, ngp, A); });
   if (e == hipSuccess)
     e = timed(c, EXH_K_BT, s, [&] { return launch_bt(s, c, ngp, A, (double2 *)c->bt.p, ldb); });
   if (e == hipSuccess && c->have_h1aa)

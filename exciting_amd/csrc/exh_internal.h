@@ -95,7 +95,8 @@ struct exh_context {
   int prof_n[EXH_K_COUNT] = {0};
 
   // per-call scratch
-  exh::DevBuf bt, a2t, ivgk;
+  exh::DevBuf bt, a2t, at, ivgk;
+  bool contig = true;  // colA[k] == k * ngkmax
   // host-API staging
   exh::DevBuf s_apwalm, s_igpig, s_vgpc, s_h, s_o, s_gpc, s_tpgpc, s_sfac;
 };
@@ -113,6 +114,8 @@ hipError_t launch_ist_table(hipStream_t s, const exh_context *c,
                             const double2 *cf, const double2 *vf,
                             const double2 *mf, const double2 *m2, int relnone,
                             int h1on);
+hipError_t launch_pack(hipStream_t s, const exh_context *c, int ngp,
+                       const double2 *apwalm);
 hipError_t launch_bt(hipStream_t s, const exh_context *c, int ngp,
                      const double2 *apwalm, double2 *bt, int ldb);
 hipError_t launch_a2t(hipStream_t s, const exh_context *c, int ngp,

@@ -4,7 +4,9 @@
 // Data layout in HBM (all column-major complex128 as double2):
 //   apwalm(ngkmax, apwordmax, lmmax, natmtot)  the caller's matching coeffs.
 //     Read as A^T: for stacked K index k = (atom, f) the column
-//     apwalm + colA[k] holds A_f(G) for G = 0..ngp-1 (contiguous).
+//     apwalm + colA[k] holds A_f(G) for G = 0..ngp-1 (contiguous).  When
+//     colA[k] = k * ngkmax (uniform apword, lmaxmat = lmaxapw) apwalm is used
+//     in place as a K-major matrix; otherwise it is packed into At first.
 //   Bt[k * ldb + G] = (aa_atom . A_atom)(f, G)        per-call scratch.
 //   H, O (ldh = rank): written once, full square, upper computed, lower
 //     mirrored (lower := conj(upper)).
@@ -84,26 +86,42 @@ __global__ void ist_table_kernel(int nbox, const int *__restrict__ ivgig,
 }
 
 // ---------------------------------------------------------------------------
+// At[k * ldb + G] = apwalm[colA[k] + G]: the K-major operand, packed only
+// when the stacked (atom, l, m, io) columns are not already equally spaced
+// in apwalm (lmaxmat < lmaxapw or an apword(l) below apwordmax).
+// ---------------------------------------------------------------------------
+__global__ void pack_kernel(int ngp, const double2 *__restrict__ apwalm,
+                            const long long *__restrict__ colA,
+                            double2 *__restrict__ at, int ldb) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (g < ngp) at[(size_t)k * ldb + g] = apwalm[colA[k] + g];
+}
+
+// ---------------------------------------------------------------------------
 // Bt = (aa . A)^T per atom:  Bt[Koff+f][G] = sum_f' A^T[G][f'] aa(f, f')
-// (hamiltonsetup.f90:75-88, zm = aa * apwi).  Grid (G tiles, f tiles, atoms).
+// (hamiltonsetup.f90:75-88, zm = aa * apwi).  X holds A^T K-major with
+// leading dimension ldx.  Grid (G tiles, f tiles, atoms).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NTH) void bt_kernel(
-    int ngp, const double2 *__restrict__ apwalm,
-    const long long *__restrict__ colA, const int *__restrict__ Koff,
-    const int *__restrict__ Ms, const double2 *__restrict__ aa, int maxaa,
-    double2 *__restrict__ bt, int ldb) {
+    int ngp, const double2 *__restrict__ X, long long ldx,
+    const int *__restrict__ Koff, const int *__restrict__ Ms,
+    const double2 *__restrict__ aa, int maxaa, double2 *__restrict__ bt, int ldb) {
   const int ias = blockIdx.z;
   const int M = Ms[ias];
   const int F0 = blockIdx.y * TM;
   if (F0 >= M) return;
   const int I0 = blockIdx.x * TM;
   const int koff = Koff[ias];
-  const long long *col = colA + koff;
   const double2 *aat = aa + (size_t)ias * maxaa * maxaa;
 
-  __shared__ double sXr[TK][TM], sXi[TK][TM], sYr[TK][TM], sYi[TK][TM];
+  __shared__ double sXr[TK][TM + 16], sXi[TK][TM + 16], sYr[TK][TM + 16], sYi[TK][TM + 16];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = w >> 1, wc = w & 1, lr = lane & 15, lk = lane >> 4;
+  const int ii = tid & 63, kq = tid >> 6;
+  const bool vi = I0 + ii < ngp, vf = F0 + ii < M;
+  const double2 *xp = X + (size_t)koff * ldx + (vi ? I0 + ii : ngp - 1);
+  const double2 *yp = aat + (vf ? F0 + ii : M - 1);
 
   dx4 cre[2][2], cim[2][2];
 #pragma unroll
@@ -118,17 +136,20 @@ __global__ __launch_bounds__(NTH) void bt_kernel(
   auto load = [&](int kb) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int e = tid + NTH * q, kk = e >> 6, ii = e & 63, k = kb + kk;
-      bool kv = k < M;
-      px[q] = (kv && I0 + ii < ngp) ? apwalm[col[k] + I0 + ii] : z2(0, 0);
-      py[q] = (kv && F0 + ii < M) ? aat[(F0 + ii) + (size_t)maxaa * k] : z2(0, 0);
+      const int k = kb + kq + 4 * q;
+      const bool kv = k < M;
+      const int kc = kv ? k : M - 1;
+      const double2 x = xp[(size_t)kc * ldx];
+      const double2 y = yp[(size_t)maxaa * kc];
+      px[q] = (kv && vi) ? x : z2(0, 0);
+      py[q] = (kv && vf) ? y : z2(0, 0);
     }
   };
   load(0);
   for (int kb = 0; kb < M; kb += TK) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int e = tid + NTH * q, kk = e >> 6, ii = e & 63;
+      const int kk = kq + 4 * q;
       sXr[kk][ii] = px[q].x;
       sXi[kk][ii] = px[q].y;
       sYr[kk][ii] = py[q].x;
@@ -209,18 +230,25 @@ __global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apw
 
 // ---------------------------------------------------------------------------
 // Fused upper-triangle H and O tiles.
-//   H(i,j) = sum_k conj(A^T[i][k]) Bt[k][j]  + IST_H(i,j)   i <= j < ngp
-//   O(i,j) = sum_k conj(A^T[i][k]) YA[k][j]  + IST_O(i,j)
-// K runs over all atoms stacked (hamiltonsetup.f90:90-103, overlapsetup
-// .f90:95-108); the epilogue adds the interstitial terms (hmlistln.f90,
-// olpistln.f90) and writes lower := conj(upper) (hamiltonoverlapocopy_UL).
-// One workgroup per upper tile pair (I <= J); tiles are dealt to the 8 XCDs
-// in contiguous ranges so co-resident tiles share column slabs in L2.
+//   H(i,j) = sum_k conj(X[k][i]) YB[k][j]  + IST_H(i,j)   i <= j < ngp
+//   O(i,j) = sum_k conj(X[k][i]) YA[k][j]  + IST_O(i,j)
+// X = A^T, YB = (aa A)^T, YA = A^T (or ((1+h1) A)^T), all K-major with K
+// running over every atom's (l, m, io) (hamiltonsetup.f90:90-103,
+// overlapsetup.f90:95-108).  The epilogue adds the interstitial terms
+// (hmlistln.f90, olpistln.f90) and writes lower := conj(upper)
+// (hamiltonoverlapocopy_UL.f90).  One workgroup per upper tile pair I <= J;
+// tiles are dealt to the 8 XCDs in contiguous ranges so co-resident tiles
+// share column slabs in L2.  K slabs are double-buffered in LDS (one barrier
+// per slab) and prefetched two slabs ahead into registers with clamped,
+// branch-free loads.  LDS rows are padded to 80 doubles so the four k rows
+// read by one MFMA operand fetch alternate LDS bank halves.
 // ---------------------------------------------------------------------------
+constexpr int LDSP = TM + 16;
+
 __global__ __launch_bounds__(NTH) void ho_kernel(
-    int ngp, int Ktot, const double2 *__restrict__ xa,
-    const long long *__restrict__ colX, const double2 *__restrict__ yb, int ldb,
-    const double2 *__restrict__ ya, const long long *__restrict__ colY,
+    int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
+    const double2 *__restrict__ YB, long long ldb,
+    const double2 *__restrict__ YA, long long lda,
     const int4 *__restrict__ ivgk, const double *__restrict__ vgpc,
     const IstEntry *__restrict__ T, int lo0, int lo1, int lo2, int n0, int n1,
     int n2, double2 *__restrict__ H, double2 *__restrict__ O, long long ldh,
@@ -235,11 +263,15 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   const int I = t - J * (J + 1) / 2;
   const int I0 = I * TM, J0 = J * TM;
 
-  __shared__ double sXr[TK][TM], sXi[TK][TM];
-  __shared__ double sBr[TK][TM], sBi[TK][TM];
-  __shared__ double sAr[TK][TM], sAi[TK][TM];
+  // [buffer][Xr, Xi, Br, Bi, Ar, Ai][k][row]
+  __shared__ double sm[2][6][TK][LDSP];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = w >> 1, wc = w & 1, lr = lane & 15, lk = lane >> 4;
+  const int ii = tid & 63, kq = tid >> 6;
+  const int ri = min(I0 + ii, ngp - 1), rj = min(J0 + ii, ngp - 1);
+  const double2 *xp = X + ri;
+  const double2 *bp = YB + rj;
+  const double2 *ap = YA + rj;
 
   dx4 hre[2][2], him[2][2], ore[2][2], oim[2][2];
 #pragma unroll
@@ -252,49 +284,54 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
       oim[a][b] = dx4{0, 0, 0, 0};
     }
 
+  // Rows at or past ngp only feed output rows/columns the epilogue never
+  // writes, so addresses are clamped but values are not masked; the K tail
+  // is zeroed by a 0/1 factor so that every load is unconditional.
   double2 px[4], pb[4], pa[4];
   auto load = [&](int kb) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int e = tid + NTH * q, kk = e >> 6, ii = e & 63, k = kb + kk;
-      bool kv = k < Ktot;
-      long long cx = kv ? colX[k] : 0;
-      long long cy = kv ? colY[k] : 0;
-      px[q] = (kv && I0 + ii < ngp) ? xa[cx + I0 + ii] : z2(0, 0);
-      pb[q] = (kv && J0 + ii < ngp) ? yb[(size_t)k * ldb + J0 + ii] : z2(0, 0);
-      pa[q] = (kv && J0 + ii < ngp) ? ya[cy + J0 + ii] : z2(0, 0);
+      const int k = kb + kq + 4 * q;
+      const double f = k < Ktot ? 1.0 : 0.0;
+      const size_t kc = k < Ktot ? k : Ktot - 1;
+      const double2 x = xp[kc * ldx];
+      const double2 b = bp[kc * ldb];
+      const double2 a = ap[kc * lda];
+      px[q] = z2(f * x.x, f * x.y);
+      pb[q] = z2(f * b.x, f * b.y);
+      pa[q] = z2(f * a.x, f * a.y);
     }
   };
-  load(0);
-  for (int kb = 0; kb < Ktot; kb += TK) {
+  auto store = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int e = tid + NTH * q, kk = e >> 6, ii = e & 63;
-      sXr[kk][ii] = px[q].x;
-      sXi[kk][ii] = px[q].y;
-      sBr[kk][ii] = pb[q].x;
-      sBi[kk][ii] = pb[q].y;
-      sAr[kk][ii] = pa[q].x;
-      sAi[kk][ii] = pa[q].y;
+      const int kk = kq + 4 * q;
+      const double2 a = pa[q];
+      sm[buf][0][kk][ii] = px[q].x;
+      sm[buf][1][kk][ii] = px[q].y;
+      sm[buf][2][kk][ii] = pb[q].x;
+      sm[buf][3][kk][ii] = pb[q].y;
+      sm[buf][4][kk][ii] = a.x;
+      sm[buf][5][kk][ii] = a.y;
     }
-    __syncthreads();
-    if (kb + TK < Ktot) load(kb + TK);
+  };
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int k4 = 0; k4 < TK / 4; k4++) {
       const int kk = 4 * k4 + lk;
       double xr[2], xi[2], nxi[2], br[2], bi[2], ar[2], ai[2];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
-        xr[a] = sXr[kk][wr * 32 + a * 16 + lr];
-        xi[a] = sXi[kk][wr * 32 + a * 16 + lr];
+        xr[a] = sm[buf][0][kk][wr * 32 + a * 16 + lr];
+        xi[a] = sm[buf][1][kk][wr * 32 + a * 16 + lr];
         nxi[a] = -xi[a];
       }
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        br[b] = sBr[kk][wc * 32 + b * 16 + lr];
-        bi[b] = sBi[kk][wc * 32 + b * 16 + lr];
-        ar[b] = sAr[kk][wc * 32 + b * 16 + lr];
-        ai[b] = sAi[kk][wc * 32 + b * 16 + lr];
+        br[b] = sm[buf][2][kk][wc * 32 + b * 16 + lr];
+        bi[b] = sm[buf][3][kk][wc * 32 + b * 16 + lr];
+        ar[b] = sm[buf][4][kk][wc * 32 + b * 16 + lr];
+        ai[b] = sm[buf][5][kk][wc * 32 + b * 16 + lr];
       }
       // conj(x) * y = (xr yr + xi yi) + i (xr yi - xi yr)
 #pragma unroll
@@ -302,15 +339,33 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
 #pragma unroll
         for (int b = 0; b < 2; b++) {
           hre[a][b] = mfma(xr[a], br[b], hre[a][b]);
-          hre[a][b] = mfma(xi[a], bi[b], hre[a][b]);
           him[a][b] = mfma(xr[a], bi[b], him[a][b]);
-          him[a][b] = mfma(nxi[a], br[b], him[a][b]);
           ore[a][b] = mfma(xr[a], ar[b], ore[a][b]);
-          ore[a][b] = mfma(xi[a], ai[b], ore[a][b]);
           oim[a][b] = mfma(xr[a], ai[b], oim[a][b]);
+        }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          hre[a][b] = mfma(xi[a], bi[b], hre[a][b]);
+          him[a][b] = mfma(nxi[a], br[b], him[a][b]);
+          ore[a][b] = mfma(xi[a], ai[b], ore[a][b]);
           oim[a][b] = mfma(nxi[a], ar[b], oim[a][b]);
         }
     }
+  };
+
+  // Loads past the last slab are clamped and zero-weighted, so the loop body
+  // is one straight-line block (no per-iteration conditions).
+  const int nslab = (Ktot + TK - 1) / TK;
+  load(0);
+  store(0);
+  __syncthreads();
+  load(TK);
+  for (int sl = 0; sl < nslab; sl++) {
+    compute(sl & 1);
+    store((sl + 1) & 1);
+    load((sl + 2) * TK);
     __syncthreads();
   }
 
@@ -501,14 +556,51 @@ hipError_t launch_ist_table(hipStream_t s, const exh_context *c,
   return hipGetLastError();
 }
 
+// K-major view of A for the MFMA kernels: apwalm itself when its stacked
+// columns are equally spaced (ld = ngkmax), else the packed copy in c->at.
+static const double2 *k_major_A(hipStream_t s, const exh_context *c, int ngp,
+                                const double2 *apwalm, long long *ld, hipError_t *err) {
+  *err = hipSuccess;
+  if (c->contig) {
+    *ld = c->ngkmax;
+    return apwalm;
+  }
+  const int ldb = (c->ngkmax + 63) / 64 * 64;
+  *ld = ldb;
+  hipLaunchKernelGGL(pack_kernel, dim3(cdiv(ngp, 256), c->Ktot), dim3(256), 0, s, ngp,
+                     apwalm, (const long long *)c->colA.p, (double2 *)c->at.p, ldb);
+  *err = hipGetLastError();
+  return (const double2 *)c->at.p;
+}
+
+hipError_t launch_pack(hipStream_t s, const exh_context *c, int ngp,
+                       const double2 *apwalm) {
+  if (ngp <= 0 || c->contig || c->Ktot == 0) return hipSuccess;
+  long long ld;
+  hipError_t e;
+  (void)k_major_A(s, c, ngp, apwalm, &ld, &e);
+  return e;
+}
+
+static const double2 *k_major_view(const exh_context *c, const double2 *apwalm,
+                                   long long *ld) {
+  if (c->contig) {
+    *ld = c->ngkmax;
+    return apwalm;
+  }
+  *ld = (c->ngkmax + 63) / 64 * 64;
+  return (const double2 *)c->at.p;
+}
+
 hipError_t launch_bt(hipStream_t s, const exh_context *c, int ngp,
                      const double2 *apwalm, double2 *bt, int ldb) {
   if (ngp <= 0 || c->natmtot <= 0) return hipSuccess;
+  long long ldx;
+  const double2 *X = k_major_view(c, apwalm, &ldx);
   dim3 grid(cdiv(ngp, TM), cdiv(c->maxaa, TM), c->natmtot);
-  hipLaunchKernelGGL(bt_kernel, grid, dim3(NTH), 0, s, ngp, apwalm,
-                     (const long long *)c->colA.p, (const int *)c->katom.p,
-                     (const int *)c->katom.p + c->natmtot, (const double2 *)c->aa.p,
-                     c->maxaa, bt, ldb);
+  hipLaunchKernelGGL(bt_kernel, grid, dim3(NTH), 0, s, ngp, X, ldx,
+                     (const int *)c->katom.p, (const int *)c->katom.p + c->natmtot,
+                     (const double2 *)c->aa.p, c->maxaa, bt, ldb);
   return hipGetLastError();
 }
 
@@ -534,13 +626,14 @@ hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
   const long long npairs = nt * (nt + 1) / 2;
   const int lo0 = c->intgv[0], lo1 = c->intgv[1], lo2 = c->intgv[2];
   const int n0 = c->intgv[3] - lo0 + 1, n1 = c->intgv[4] - lo1 + 1, n2 = c->intgv[5] - lo2 + 1;
-  const double2 *ya = a2t ? a2t : apwalm;
-  const long long *colY = a2t ? (const long long *)c->colA.p + c->Ktot
-                              : (const long long *)c->colA.p;
+  long long ldx;
+  const double2 *X = k_major_view(c, apwalm, &ldx);
+  const double2 *YA = a2t ? a2t : X;
+  const long long lda = a2t ? (long long)ldb : ldx;
   hipLaunchKernelGGL(ho_kernel, dim3((unsigned)npairs), dim3(NTH), 0, s, ngp, c->Ktot,
-                     apwalm, (const long long *)c->colA.p, bt, ldb, ya, colY, ivgk,
-                     vgpc, (const IstEntry *)c->ist.p, lo0, lo1, lo2, n0, n1, n2, h,
-                     o, (long long)ldh, flags & EXH_ACCUMULATE);
+                     X, ldx, bt, (long long)ldb, YA, lda, ivgk, vgpc,
+                     (const IstEntry *)c->ist.p, lo0, lo1, lo2, n0, n1, n2, h, o,
+                     (long long)ldh, flags & EXH_ACCUMULATE);
   return hipGetLastError();
 }
 

@@ -88,7 +88,7 @@ EXPORTS = (
     "exh_profile_read", "exh_profile_reset",
 )
 
-KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo")
+KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack")
 
 
 def _is_torch(a) -> bool:

@@ -156,7 +156,8 @@ enum {
   EXH_K_HO = 4,       /* fused upper H/O contraction + interstitial     */
   EXH_K_LOBORDER = 5, /* APW-lo rows/columns                           */
   EXH_K_LOLO = 6,     /* lo-lo block                                   */
-  EXH_K_COUNT = 7
+  EXH_K_PACK = 7,     /* K-major copy of apwalm (non-uniform layouts)  */
+  EXH_K_COUNT = 8
 };
 int exh_profile_enable(exh_context *ctx, int on);
 int exh_profile_read(exh_context *ctx, int kernel, double *total_ms, int *launches);
