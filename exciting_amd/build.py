@@ -63,5 +63,41 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "exh_driver")
+
+
+def flang() -> str:
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang", shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return ""
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The ISO_C_BINDING module and its driver program, linked to libexh.so.
+    Skipped (returns "") when no Fortran compiler is installed."""
+    fc = flang()
+    if not fc:
+        return ""
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for src in ("exh_lapw.f90", "exh_driver.f90"):
+        obj = os.path.join(bdir, src.replace(".f90", ".o"))
+        cmd = [fc, "-O2", "-c", os.path.join(FORTRAN, src), "-module-dir", bdir, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = DRIVER + ".tmp"
+    cmd = [fc, "-o", tmp, *objs, "-L", HERE, "-lexh", "-Wl,-rpath,$ORIGIN/.."]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVER)
+    shutil.rmtree(bdir, ignore_errors=True)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True) or "no Fortran compiler: driver skipped")

@@ -1,0 +1,141 @@
+! ISO_C_BINDING interface to libexh (include/exh.h).
+!
+! A Fortran caller (exciting's seceqn/seceqnfv) uses this module to reach the
+! MI355X H/O setup.  Every C entry point has an interface here with the same
+! argument order; arrays are passed by address (c_loc, or implicitly as
+! assumed-size dummies) in the reference's own column-major layouts.
+!
+!   exh_match                    replaces  match (ngp, gpc, tpgpc, sfacgp, apwalm)
+!                                          src/match.f90:13
+!   exh_hamiltonandoverlapsetup  replaces  hamiltonandoverlapsetup (system, ngp,
+!                                          apwalm, igpig, vgpc)
+!                                          src/src_eigensystem/hamiltonandoverlapsetup.f90:8
+!                                with the semantics of hamiltonsetup followed by
+!                                overlapsetup (seceqnfv.f90:76-77).
+module exh_lapw
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: EXH_ACCUMULATE = 1
+
+  public :: exh_create, exh_destroy, exh_set_basis, exh_set_radial, exh_set_mt, &
+            exh_set_gvectors, exh_set_interstitial, exh_match, &
+            exh_hamiltonandoverlapsetup, exh_hamiltonandoverlapsetup_ex, &
+            exh_error_message
+
+  interface
+    integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    subroutine exh_destroy(ctx) bind(C, name="exh_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+
+    type(c_ptr) function exh_error_c(ctx) bind(C, name="exh_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function exh_set_basis(ctx, nspecies, natoms, lmaxapw, lmaxmat, &
+        apwordmax, apword, nlomax, nlorb, lorbl, ngkmax) bind(C, name="exh_set_basis")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nspecies, lmaxapw, lmaxmat, apwordmax, nlomax, ngkmax
+      integer(c_int), intent(in) :: natoms(*), apword(*), nlorb(*), lorbl(*)
+    end function
+
+    integer(c_int) function exh_set_radial(ctx, omega, rmt, nrmt, nrmtmax, spr, apwfr) &
+        bind(C, name="exh_set_radial")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: omega
+      real(c_double), intent(in) :: rmt(*), spr(*), apwfr(*)
+      integer(c_int), intent(in) :: nrmt(*)
+      integer(c_int), value :: nrmtmax
+    end function
+
+    integer(c_int) function exh_set_mt(ctx, maxaa, maxnlo, aa, loa, lolo, oalo, ololo, &
+        h1aa, h1loa, h1lolo) bind(C, name="exh_set_mt")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: maxaa, maxnlo
+      type(c_ptr), value :: aa, loa, lolo, oalo, ololo, h1aa, h1loa, h1lolo
+    end function
+
+    integer(c_int) function exh_set_gvectors(ctx, ngrtot, ngvec, ivg, intgv, ivgig) &
+        bind(C, name="exh_set_gvectors")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngrtot, ngvec
+      integer(c_int), intent(in) :: ivg(*), intgv(*), ivgig(*)
+    end function
+
+    integer(c_int) function exh_set_interstitial(ctx, cfunig, veffig, meffig, m2effig, &
+        valence_rel_none, h1on) bind(C, name="exh_set_interstitial")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, cfunig, veffig, meffig, m2effig
+      integer(c_int), value :: valence_rel_none, h1on
+    end function
+
+    integer(c_int) function exh_match(ctx, ngp, gpc, tpgpc, sfacgp, apwalm) &
+        bind(C, name="exh_match")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp
+      real(c_double), intent(in) :: gpc(*), tpgpc(*)
+      complex(c_double_complex), intent(in) :: sfacgp(*)
+      complex(c_double_complex), intent(inout) :: apwalm(*)
+    end function
+
+    integer(c_int) function exh_hamiltonandoverlapsetup(ctx, ngp, rank, apwalm, igpig, vgpc, &
+        h, o) bind(C, name="exh_hamiltonandoverlapsetup")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp, rank
+      complex(c_double_complex), intent(in) :: apwalm(*)
+      integer(c_int), intent(in) :: igpig(*)
+      real(c_double), intent(in) :: vgpc(*)
+      complex(c_double_complex), intent(inout) :: h(*), o(*)
+    end function
+
+    integer(c_int) function exh_hamiltonandoverlapsetup_ex(ctx, ngp, rank, apwalm, igpig, &
+        vgpc, h, o, flags) bind(C, name="exh_hamiltonandoverlapsetup_ex")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp, rank, flags
+      complex(c_double_complex), intent(in) :: apwalm(*)
+      integer(c_int), intent(in) :: igpig(*)
+      real(c_double), intent(in) :: vgpc(*)
+      complex(c_double_complex), intent(inout) :: h(*), o(*)
+    end function
+
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  !> The library's last error message as a Fortran string.
+  function exh_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    p = exh_error_c(ctx)
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate (character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = chars(i)
+    end do
+  end function
+
+end module exh_lapw

@@ -1,0 +1,59 @@
+"""The Fortran ISO_C_BINDING path (exciting_amd/fortran/exh_lapw.f90) end to
+end: a flang-built program calls match and hamiltonandoverlapsetup through
+the binding module on the GPU; its apwalm, H and O are checked against the
+oracle to the same 1e-12 relative Frobenius bar as the Python path."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from exciting_amd import build as exhbuild
+from tests import _cases
+
+
+def write_input(path, s, ik=0):
+    k = s.kpts[ik]
+    nbox = int(np.prod(s.ivgig.shape))
+    i32 = lambda a: np.asarray(a, dtype=np.int32).ravel(order="F").tobytes()
+    f64 = lambda a: np.asarray(a, dtype=np.float64).ravel(order="F").tobytes()
+    c128 = lambda a: np.asarray(a, dtype=np.complex128).ravel(order="F").tobytes()
+    hdr = [s.nspecies, s.natmtot, s.lmaxapw, s.lmaxmat, s.apwordmax, s.nlomax, s.ngkmax,
+           s.nrmtmax, s.maxaa, max(s.maxnlo, 1), s.ngrtot, s.ngvec, k.ngk, nbox, s.nlotot]
+    with open(path, "wb") as f:
+        f.write(i32(hdr))
+        f.write(i32(s.natoms) + i32(s.apword) + i32(s.nlorb) + i32(s.lorbl) + f64([s.omega])
+                + f64(s.rmt) + i32(s.nrmt) + f64(s.spr) + f64(s.apwfr))
+        f.write(c128(s.aa) + c128(s.loa) + c128(s.lolo) + f64(s.oalo) + f64(s.ololo))
+        f.write(i32(s.ivg) + i32(s.intgv) + i32(s.ivgig) + c128(s.cfunig) + c128(s.veffig))
+        f.write(f64(k.gpc) + f64(k.tpgpc) + c128(k.sfacgp) + i32(k.igpig) + f64(k.vgpc))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tiny", "fcc2"])
+def test_fortran_driver(tmp_path, name):
+    if not os.path.exists(exhbuild.DRIVER):
+        pytest.fail("Fortran driver not built (python -m exciting_amd.build)")
+    s = _cases.system(name)
+    k = s.kpts[0]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, s)
+    print(f"[fortran] case {name}", flush=True)
+    r = subprocess.run([exhbuild.DRIVER, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    lmm = (s.lmaxapw + 1) ** 2
+    n = k.ngk + s.nlotot
+    raw = np.fromfile(fout, dtype=np.complex128)
+    na = s.ngkmax * s.apwordmax * lmm * s.natmtot
+    A = raw[:na].reshape((s.ngkmax, s.apwordmax, lmm, s.natmtot), order="F")
+    H = raw[na:na + n * n].reshape((n, n), order="F")
+    O = raw[na + n * n:].reshape((n, n), order="F")
+    S = oracle.System(s)
+    A0 = oracle.match(S, 0)
+    H0, O0 = oracle.setup(S, 0, A0)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"  fortran {name}: A {rel(A, A0):.2e} H {rel(H, H0):.2e} O {rel(O, O0):.2e}")
+    assert rel(A, A0) <= 1e-12 and rel(H, H0) <= 1e-12 and rel(O, O0) <= 1e-12
