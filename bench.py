@@ -44,6 +44,20 @@ def flops_ho(natm, M, G):
     return natm * 8.0 * G * (G + 1) * M
 
 
+PROFILE = os.path.join(ROOT, "profiles", "ho_kernel_sc64_fp64.md")
+
+
+def measured_traffic(config_index: int):
+    """HBM-side bytes per ho_kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by a calibration copy).
+    Only the 64-atom workload was counted; other workloads report null."""
+    if config_index != 3 or not os.path.exists(PROFILE):
+        return None
+    import re
+    m = re.search(r"\|\s*traffic_bytes_per_launch\s*\|\s*([0-9.eE+-]+)", open(PROFILE).read())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(s, ik: int):
     """The oracle (C restatement + SciPy's OpenBLAS zgemm) on one k-point."""
     import oracle
@@ -82,6 +96,9 @@ def main():
     ap.add_argument("--config", type=int, default=3, help="workload index (0..3)")
     ap.add_argument("--kpoints", type=int, default=0, help="use the first N k-points (0 = all)")
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the timed region, run one device-to-device copy of this "
+                         "many MiB (a known-size kernel for calibrating HBM byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -115,6 +132,14 @@ def main():
     Obuf = torch.empty(nmax * nmax, dtype=torch.complex128, device="cuda")
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(ctx.stream)
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"calibration copy of {args.copy_calibration_mib} MiB done")
 
     def one_setup(d):
         ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, apwalm)
@@ -193,7 +218,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(args.config),
         },
     }
     if args.cpu_baseline == "on":
