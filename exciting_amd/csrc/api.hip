@@ -6,7 +6,7 @@
 #include <cstring>
 #include <new>
 
-#include "exh_internal.h"
+#include "exh_device.h"
 
 namespace exh {
 
@@ -242,7 +242,7 @@ void exh_destroy(exh_context *c) {
                     &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
                     &c->aa, &c->loa, &c->lolo, &c->salo, &c->ololo_eff, &c->h1aa,
                     &c->ivg, &c->ivgig, &c->ist, &c->cf_stage, &c->vf_stage, &c->mf_stage,
-                    &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->s_apwalm, &c->s_igpig,
+                    &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex, &c->ozeb, &c->s_apwalm, &c->s_igpig,
                     &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac};
   for (DevBuf *b : bufs) release(*b);
   for (auto &r : c->recs) {
@@ -507,7 +507,12 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
   if (rank != ngp + c->nlotot) return fail(c, EXH_EINVAL, "setup: rank != ngp + nlotot");
   if (!h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
     return fail(c, EXH_EINVAL, "setup: null array");
-  if (flags & ~EXH_ACCUMULATE) return fail(c, EXH_ENOTSUP, "setup: unknown flag");
+  const int nmod = (flags >> 8) & 0xff;
+  if (flags & ~(EXH_ACCUMULATE | 0xff00)) return fail(c, EXH_ENOTSUP, "setup: unknown flag");
+  if (nmod && (nmod < 6 || nmod > kOzMaxMod))
+    return fail(c, EXH_EINVAL, "setup: EXH_SPLIT(n) needs 6 <= n <= 12");
+  if (nmod && c->have_h1aa)
+    return fail(c, EXH_ENOTSUP, "setup: split precision with h1aa (iora*) is not implemented");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const int ldb = (c->ngkmax + 63) / 64 * 64;
@@ -526,12 +531,40 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
     e = timed(c, EXH_K_BT, s, [&] { return launch_bt(s, c, ngp, A, (double2 *)c->bt.p, ldb); });
   if (e == hipSuccess && c->have_h1aa)
     e = timed(c, EXH_K_A2T, s, [&] { return launch_a2t(s, c, ngp, A, (double2 *)c->a2t.p, ldb); });
-  if (e == hipSuccess)
+  if (e == hipSuccess && nmod == 0)
     e = timed(c, EXH_K_HO, s, [&] {
       return launch_ho(s, c, ngp, A, (const double2 *)c->bt.p,
                        c->have_h1aa ? (const double2 *)c->a2t.p : nullptr, ldb,
                        (const int4 *)c->ivgk.p, vgpc, (double2 *)h, (double2 *)o, rank, flags);
     });
+  if (e == hipSuccess && nmod > 0) {
+    const OzLayout L = oz_layout(c->ngkmax, c->Ktot, nmod);
+    const int bits = oz_bits(nmod, c->Ktot);
+    if (!c->oz_consts) {
+      e = oz_upload_constants(nmod);
+      c->oz_consts = e == hipSuccess;
+    }
+    int rc2 = 0;
+    if (e == hipSuccess) rc2 = ensure(c, c->ozqx, L.qx_bytes);
+    if (!rc2) rc2 = ensure(c, c->ozqb, L.qb_bytes);
+    if (!rc2) rc2 = ensure(c, c->ozr, L.r_bytes);
+    if (!rc2) rc2 = ensure(c, c->ozex, (size_t)c->ngkmax * sizeof(int));
+    if (!rc2) rc2 = ensure(c, c->ozeb, (size_t)c->ngkmax * sizeof(int));
+    if (rc2) return rc2;
+    long long ldx;
+    const double2 *X = k_major_operand(c, A, &ldx);
+    const int lo0 = c->intgv[0], lo1 = c->intgv[1], lo2 = c->intgv[2];
+    IstBox box{(const int4 *)c->ivgk.p, vgpc, (const IstEntry *)c->ist.p, lo0, lo1, lo2,
+               c->intgv[3] - lo0 + 1, c->intgv[4] - lo1 + 1, c->intgv[5] - lo2 + 1};
+    const int kid[3] = {EXH_K_OZRES, EXH_K_OZGEMM, EXH_K_OZCOMB};
+    for (int st = 0; st < 3 && e == hipSuccess; st++)
+      e = timed(c, kid[st], s, [&] {
+        return launch_ozaki(s, c, ngp, X, ldx, (const double2 *)c->bt.p, ldb, nmod, bits, L,
+                            (int8_t *)c->ozqx.p, (int8_t *)c->ozqb.p, (uint8_t *)c->ozr.p,
+                            (int *)c->ozex.p, (int *)c->ozeb.p, box, (double2 *)h, (double2 *)o,
+                            rank, flags & EXH_ACCUMULATE, 1 << st);
+      });
+  }
   if (e == hipSuccess)
     e = timed(c, EXH_K_LOBORDER, s, [&] {
       return launch_lo_border(s, c, ngp, A, (double2 *)h, (double2 *)o, rank, flags);

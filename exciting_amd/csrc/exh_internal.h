@@ -33,6 +33,16 @@ struct LoRow {
   int ias, ilo, mi, l;
 };
 
+// Split-precision (Ozaki scheme II) path.
+constexpr int kOzMaxMod = 12;
+extern const int kOzPrimes[kOzMaxMod];
+struct OzLayout {
+  int Gpad = 0, Kpad = 0;       // G rounded to the 128 tile, K to the 64 stage
+  size_t qstride = 0;           // bytes of one residue array (Kpad * Gpad)
+  long long max_pairs = 0;      // upper tile pairs at ngkmax
+  size_t qx_bytes = 0, qb_bytes = 0, r_bytes = 0;
+};
+
 struct MatchTables {
   // LU factors of the boundary-derivative matrix D per (l, ias): pmax*pmax
   // doubles (column-major, zgetf2 layout) and pmax 1-based pivots.
@@ -96,6 +106,8 @@ struct exh_context {
 
   // per-call scratch
   exh::DevBuf bt, a2t, at, ivgk;
+  exh::DevBuf ozqx, ozqb, ozr, ozex, ozeb;
+  bool oz_consts = false;
   bool contig = true;  // colA[k] == k * ngkmax
   // host-API staging
   exh::DevBuf s_apwalm, s_igpig, s_vgpc, s_h, s_o, s_gpc, s_tpgpc, s_sfac;
@@ -130,6 +142,16 @@ hipError_t launch_lo_border(hipStream_t s, const exh_context *c, int ngp,
                             int ldh, int flags);
 hipError_t launch_lolo(hipStream_t s, const exh_context *c, int ngp,
                        double2 *h, double2 *o, int ldh, int flags);
+const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, long long *ld);
+int oz_bits(int nmod, int Ktot);
+hipError_t oz_upload_constants(int nmod);
+OzLayout oz_layout(int ngkmax, int Ktot, int nmod);
+struct IstBox;
+hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const double2 *X,
+                        long long ldx, const double2 *B, long long ldb, int nmod, int bits,
+                        const OzLayout &L, int8_t *QX, int8_t *QB, uint8_t *R, int *ex,
+                        int *eb, const IstBox &box, double2 *h, double2 *o, int ldh,
+                        int accumulate, int stage_mask);
 hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
                         const double *gpc, const double *tpgpc,
                         const double2 *sfacgp, double2 *apwalm);

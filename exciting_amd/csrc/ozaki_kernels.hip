@@ -1,0 +1,414 @@
+// Split-precision muffin-tin contraction: Ozaki scheme II on int8 MFMA.
+//
+// The FP64 products H_MT = X^H B and O_MT = X^H X (X = A^T, B = (aa A)^T,
+// K = every atom's (l, m, io) stacked) are emulated with integer arithmetic:
+//
+//  1. Each G column of X and of B gets a power-of-two scale 2^s so that
+//     |Re|, |Im| <= 2^b; the scaled values are rounded to integers.  This
+//     rounding is the only approximation: the error is <= 2^-b of the
+//     column's largest entry.
+//  2. The integers are reduced modulo N primes p_m <= 251 into int8 residues
+//     in [-(p-1)/2, (p-1)/2], stored in 16-byte k blocks per G row: the
+//     operand format of v_mfma_i32_32x32x32_i8.
+//  3. For each modulus and each of six real products (Karatsuba for the
+//     conjugated complex product, exact in modular arithmetic):
+//        H: P1 = xr.br, P2 = xi.bi, P3 = (xr+xi).(br-bi)
+//        O: Q1 = xr.xr, Q2 = xi.xi, Q3 = (xr+xi).(xr-xi)
+//     an upper-triangular int8 GEMM accumulates exactly in int32 and stores
+//     the result modulo p_m as one byte per element.
+//  4. Re = P1+P2 and Im = P1-P2-P3 are formed modulo each p_m and turned
+//     back into exact integers by Garner's mixed-radix algorithm with
+//     symmetric digits (no cancellation for small results), evaluated in
+//     FP64 and rescaled by 2^-(s_i+s_j).  The interstitial terms and the
+//     conj mirror are added as in the FP64 path.
+//
+// b is chosen from N and K so that |6 K 2^(2b)| < prod(p)/2: every product is
+// recovered exactly from its residues.
+#include <climits>
+#include <cmath>
+
+#include "exh_device.h"
+
+namespace exh {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int OZ_T = 128;   // output tile edge
+constexpr int OZ_KS = 64;   // k per LDS stage (4 blocks of 16)
+
+__constant__ int c_p[kOzMaxMod];
+__constant__ float c_rp[kOzMaxMod];          // 1/p
+__constant__ int c_w[kOzMaxMod];             // (p_0..p_{k-1})^{-1} mod p_k
+__constant__ int c_c[kOzMaxMod][kOzMaxMod];  // (p_0..p_{j-1}) * w_k mod p_k, j < k
+
+// ---------------------------------------------------------------------------
+// Column exponents: emax[g] = max over k of the binary exponent of |Re|,|Im|.
+// Grid (G blocks, K chunks); atomicMax merges the chunks.
+// ---------------------------------------------------------------------------
+__global__ void colexp_kernel(int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
+                              int *__restrict__ emax) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngp) return;
+  const int k0 = blockIdx.y * 256, k1 = min(Ktot, k0 + 256);
+  double mx = 0.0;
+  for (int k = k0; k < k1; k++) {
+    const double2 v = X[(size_t)k * ldx + g];
+    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+  }
+  if (mx > 0.0) {
+    int e;
+    (void)frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1): |values| <= 2^e
+    atomicMax(emax + g, e);
+  }
+}
+
+__device__ __forceinline__ int smod(double x, int m) {
+  // symmetric residue of the integer-valued double x modulo the odd prime p_m
+  const double p = (double)c_p[m];
+  return (int)(x - p * rint(x * (1.0 / p)));
+}
+
+// ---------------------------------------------------------------------------
+// Residues: one thread per (16-k block, G row).  X gives four arrays
+// (xr, xi, xr+xi, xr-xi), B gives three (br, bi, br-bi), per modulus.
+// Rows >= ngp and k >= Ktot are written as zeros (no masking downstream).
+// Layout: Q[(m * narr + t) * qstride + (kblk * Gpad + g) * 16 + byte].
+// ---------------------------------------------------------------------------
+template <int NARR>
+__global__ void residue_kernel(int ngp, int Ktot, int Gpad, const double2 *__restrict__ X,
+                               long long ldx, const int *__restrict__ emax, int bits, int nmod,
+                               size_t qstride, int8_t *__restrict__ Q) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int kb = blockIdx.y;
+  if (g >= Gpad) return;
+  double re[16], im[16];
+  int s = 0;
+  if (g < ngp) {
+    const int e = emax[g];
+    s = (e == INT_MIN) ? 0 : bits - e;
+  }
+#pragma unroll
+  for (int t = 0; t < 16; t++) {
+    const int k = kb * 16 + t;
+    double2 v = z2(0.0, 0.0);
+    if (g < ngp && k < Ktot) v = X[(size_t)k * ldx + g];
+    re[t] = rint(ldexp(v.x, s));
+    im[t] = rint(ldexp(v.y, s));
+  }
+  const size_t off = ((size_t)kb * Gpad + g) * 16;
+  for (int m = 0; m < nmod; m++) {
+#pragma unroll
+    for (int a = 0; a < NARR; a++) {
+      int w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        unsigned packed = 0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int k = 4 * q + t;
+          double x;
+          if (a == 0) x = re[k];
+          else if (a == 1) x = im[k];
+          else if (a == 2) x = (NARR == 4) ? re[k] + im[k] : re[k] - im[k];
+          else x = re[k] - im[k];
+          packed |= (unsigned)(smod(x, m) & 0xff) << (8 * t);
+        }
+        w[q] = (int)packed;
+      }
+      *(v4i *)(Q + (size_t)(m * NARR + a) * qstride + off) = v4i{w[0], w[1], w[2], w[3]};
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Upper-triangular int8 GEMM, one (product, modulus) per workgroup:
+//   C[i][j] = sum_k A[k][i] Bm[k][j]  (int32, exact), stored mod p as bytes.
+// 128x128 tiles, 4 waves of 64x64 (2x2 blocks of v_mfma_i32_32x32x32_i8).
+// Operand lane layout: row/column = lane%32, 16 k bytes from k block
+// 2*ks + lane/32; result D[8*(r/4) + 4*(lane/32) + r%4][lane%32].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ozgemm_kernel(int npairs, int nslab, int Gpad, int nmod,
+                                                     const int8_t *__restrict__ QX,
+                                                     const int8_t *__restrict__ QB,
+                                                     size_t qstride, uint8_t *__restrict__ R) {
+  // XCD-contiguous remap over the whole grid, then (product*nmod+m, tile)
+  const int bid = blockIdx.x, nb = gridDim.x;
+  const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
+  const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
+  const int pm = t / npairs, tile = t - pm * npairs;
+  const int prod = pm / nmod, m = pm - prod * nmod;
+  int J = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+  while ((J + 1) * (J + 2) / 2 <= tile) J++;
+  while (J * (J + 1) / 2 > tile) J--;
+  const int I = tile - J * (J + 1) / 2;
+  const int I0 = I * OZ_T, J0 = J * OZ_T;
+
+  // operands: rows from X arrays {xr, xi, xs}; columns from B {br, bi, bd}
+  // (H products 0-2) or X {xr, xi, xd} (O products 3-5)
+  const int ra = prod % 3;
+  const int8_t *Ab = QX + (size_t)(m * 4 + ra) * qstride;
+  const int8_t *Bb = prod < 3 ? QB + (size_t)(m * 3 + ra) * qstride
+                              : QX + (size_t)(m * 4 + (ra == 2 ? 3 : ra)) * qstride;
+
+  __shared__ v4i sA[2][4][OZ_T], sB[2][4][OZ_T];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
+
+  v16i acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = v16i{0};
+
+  // global -> register prefetch: 2 chunks of 16 B per operand per thread
+  v4i pa[2], pb[2];
+  auto load = [&](int s) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int c = tid + 256 * q, kk = c >> 7, row = c & 127;
+      const size_t kblk = (size_t)s * 4 + kk;
+      pa[q] = *(const v4i *)(Ab + (kblk * Gpad + I0 + row) * 16);
+      pb[q] = *(const v4i *)(Bb + (kblk * Gpad + J0 + row) * 16);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int c = tid + 256 * q, kk = c >> 7, row = c & 127;
+      sA[buf][kk][row] = pa[q];
+      sB[buf][kk][row] = pb[q];
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  if (nslab > 1) load(1);
+  for (int s = 0; s < nslab; s++) {
+    const int buf = s & 1;
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      const int kk = 2 * ks + lh;
+      v4i av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) av[a] = sA[buf][kk][wr * 64 + a * 32 + l32];
+#pragma unroll
+      for (int b = 0; b < 2; b++) bv[b] = sB[buf][kk][wc * 64 + b * 32 + l32];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    if (s + 1 < nslab) {
+      store(buf ^ 1);
+      if (s + 2 < nslab) load(s + 2);
+    }
+    __syncthreads();
+  }
+
+  // residues mod p_m, four consecutive rows packed per 32-bit store
+  const int p = c_p[m];
+  const float rp = c_rp[m];
+  uint8_t *out = R + ((size_t)pm * npairs + tile) * (OZ_T * OZ_T);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        unsigned packed = 0;
+#pragma unroll
+        for (int t4 = 0; t4 < 4; t4++) {
+          const int x = acc[a][b][4 * h + t4];
+          int q = (int)floorf((float)x * rp);
+          int r = x - q * p;
+          r += (r < 0) ? p : 0;
+          r -= (r >= p) ? p : 0;
+          packed |= (unsigned)r << (8 * t4);
+        }
+        const int il = wr * 64 + a * 32 + 8 * h + 4 * lh;
+        const int jl = wc * 64 + b * 32 + l32;
+        *(unsigned *)(out + il + OZ_T * jl) = packed;
+      }
+}
+
+// Garner with symmetric digits: the integer x in (-M/2, M/2) with residues r_k.
+__device__ __forceinline__ double garner(const int *r, int nmod) {
+  int v[kOzMaxMod];
+  double x = 0.0;
+  for (int k = 0; k < nmod; k++) {
+    const int p = c_p[k];
+    int acc = r[k] * c_w[k];
+    for (int j = 0; j < k; j++) acc -= v[j] * c_c[j][k];
+    int q = (int)floorf((float)acc * c_rp[k]);
+    int d = acc - q * p;
+    d += (d < 0) ? p : 0;
+    d -= (d >= p) ? p : 0;
+    if (d > (p >> 1)) d -= p;
+    v[k] = d;
+  }
+  for (int k = nmod - 1; k >= 0; k--) x = fma(x, (double)c_p[k], (double)v[k]);
+  return x;
+}
+
+__device__ __forceinline__ int modp(int x, int k) {
+  const int p = c_p[k];
+  int r = x % p;
+  return r < 0 ? r + p : r;
+}
+
+// ---------------------------------------------------------------------------
+// Residues -> FP64 H and O upper elements (+ interstitial, + mirror).
+// One workgroup per tile; thread handles elements tid, tid+256, ...
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ozcombine_kernel(
+    int ngp, int npairs, int nmod, const uint8_t *__restrict__ R, const int *__restrict__ ex,
+    const int *__restrict__ eb, int bits, IstBox box, double2 *__restrict__ H,
+    double2 *__restrict__ O, long long ldh, int accumulate) {
+  const int tile = blockIdx.x;
+  int J = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+  while ((J + 1) * (J + 2) / 2 <= tile) J++;
+  while (J * (J + 1) / 2 > tile) J--;
+  const int I = tile - J * (J + 1) / 2;
+  const int I0 = I * OZ_T, J0 = J * OZ_T;
+  const size_t TT = (size_t)OZ_T * OZ_T;
+  for (int e = threadIdx.x; e < OZ_T * OZ_T; e += 256) {
+    const int il = e & (OZ_T - 1), jl = e >> 7;
+    const int i = I0 + il, j = J0 + jl;
+    if (i >= ngp || j >= ngp || i > j) continue;
+    int hre[kOzMaxMod], him[kOzMaxMod], ore[kOzMaxMod], oim[kOzMaxMod];
+    for (int k = 0; k < nmod; k++) {
+      int r[6];
+#pragma unroll
+      for (int pr = 0; pr < 6; pr++)
+        r[pr] = R[((size_t)(pr * nmod + k) * npairs + tile) * TT + e];
+      hre[k] = modp(r[0] + r[1], k);
+      him[k] = modp(r[0] - r[1] - r[2], k);
+      ore[k] = modp(r[3] + r[4], k);
+      oim[k] = modp(r[3] - r[4] - r[5], k);
+    }
+    const int si = ex[i] == INT_MIN ? 0 : bits - ex[i];
+    const int sjb = eb[j] == INT_MIN ? 0 : bits - eb[j];
+    const int sjx = ex[j] == INT_MIN ? 0 : bits - ex[j];
+    double hr = ldexp(garner(hre, nmod), -(si + sjb));
+    double hi = ldexp(garner(him, nmod), -(si + sjb));
+    double orr = ldexp(garner(ore, nmod), -(si + sjx));
+    double oi = ldexp(garner(oim, nmod), -(si + sjx));
+    if (accumulate) {
+      const size_t up = (size_t)i + (size_t)j * ldh;
+      const double2 h0 = H[up], o0 = O[up];
+      hr += h0.x;
+      hi += h0.y;
+      orr += o0.x;
+      oi += o0.y;
+    }
+    ist_add(box, i, j, hr, hi, orr, oi);
+    store_upper(H, O, ldh, i, j, hr, hi, orr, oi, 0);
+  }
+}
+
+__global__ void fill_int_kernel(int n, int v, int *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+const int kOzPrimes[kOzMaxMod] = {251, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191};
+
+int oz_bits(int nmod, int Ktot) {
+  double lm = 0.0;
+  for (int k = 0; k < nmod; k++) lm += std::log2((double)kOzPrimes[k]);
+  // |Im| <= |P1| + |P2| + |P3| <= 6 K 2^(2b) must stay below prod(p)/2
+  const double b = std::floor((lm - 1.0 - std::log2(6.0 * std::max(Ktot, 1))) * 0.5);
+  return (int)std::min(30.0, b);
+}
+
+hipError_t oz_upload_constants(int nmod) {
+  int p[kOzMaxMod], w[kOzMaxMod], cc[kOzMaxMod][kOzMaxMod];
+  float rp[kOzMaxMod];
+  auto inv = [](long long a, int m) {  // a^-1 mod m, m prime
+    long long r = 1, b = ((a % m) + m) % m, e = m - 2;
+    while (e) {
+      if (e & 1) r = r * b % m;
+      b = b * b % m;
+      e >>= 1;
+    }
+    return (int)r;
+  };
+  for (int k = 0; k < kOzMaxMod; k++) {
+    p[k] = kOzPrimes[k];
+    rp[k] = 1.0f / (float)p[k];
+    long long P = 1;  // p_0 ... p_{k-1} mod p_k
+    for (int j = 0; j < k; j++) P = P * kOzPrimes[j] % kOzPrimes[k];
+    w[k] = k == 0 ? 1 : inv(P, kOzPrimes[k]);
+    long long Pj = 1;  // p_0 ... p_{j-1} mod p_k
+    for (int j = 0; j < kOzMaxMod; j++) {
+      cc[j][k] = (j < k) ? (int)(Pj * w[k] % kOzPrimes[k]) : 0;
+      if (j < k) Pj = Pj * kOzPrimes[j] % kOzPrimes[k];
+    }
+  }
+  (void)nmod;
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_p), p, sizeof p);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_rp), rp, sizeof rp);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_w), w, sizeof w);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_c), cc, sizeof cc);
+  return e;
+}
+
+OzLayout oz_layout(int ngkmax, int Ktot, int nmod) {
+  OzLayout L;
+  L.Gpad = (ngkmax + OZ_T - 1) / OZ_T * OZ_T;
+  L.Kpad = (Ktot + OZ_KS - 1) / OZ_KS * OZ_KS;
+  L.qstride = (size_t)L.Kpad * L.Gpad;
+  const long long nt = L.Gpad / OZ_T;
+  L.max_pairs = nt * (nt + 1) / 2;
+  L.qx_bytes = (size_t)nmod * 4 * L.qstride;
+  L.qb_bytes = (size_t)nmod * 3 * L.qstride;
+  L.r_bytes = (size_t)6 * nmod * L.max_pairs * OZ_T * OZ_T;
+  return L;
+}
+
+hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const double2 *X,
+                        long long ldx, const double2 *B, long long ldb, int nmod, int bits,
+                        const OzLayout &L, int8_t *QX, int8_t *QB, uint8_t *R, int *ex, int *eb,
+                        const IstBox &box, double2 *h, double2 *o, int ldh, int accumulate,
+                        int stage_mask) {
+  if (ngp <= 0) return hipSuccess;
+  const int K = c->Ktot;
+  const long long nt = (ngp + OZ_T - 1) / OZ_T;
+  const long long npairs = nt * (nt + 1) / 2;
+  const int nslab = L.Kpad / OZ_KS;
+  if (stage_mask & 1) {
+    hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(ngp, 256)), dim3(256), 0, s, ngp, INT_MIN, ex);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(ngp, 256)), dim3(256), 0, s, ngp, INT_MIN, eb);
+    dim3 ge(cdiv(ngp, 256), cdiv(K, 256));
+    hipLaunchKernelGGL(colexp_kernel, ge, dim3(256), 0, s, ngp, K, X, ldx, ex);
+    hipLaunchKernelGGL(colexp_kernel, ge, dim3(256), 0, s, ngp, K, B, ldb, eb);
+    dim3 gr(cdiv(L.Gpad, 256), L.Kpad / 16);
+    hipLaunchKernelGGL(residue_kernel<4>, gr, dim3(256), 0, s, ngp, K, L.Gpad, X, ldx, ex, bits,
+                       nmod, L.qstride, QX);
+    hipLaunchKernelGGL(residue_kernel<3>, gr, dim3(256), 0, s, ngp, K, L.Gpad, B, ldb, eb, bits,
+                       nmod, L.qstride, QB);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (stage_mask & 2) {
+    const long long nblk = npairs * 6 * nmod;
+    hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)npairs, nslab,
+                       L.Gpad, nmod, QX, QB, L.qstride, R);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (stage_mask & 4) {
+    hipLaunchKernelGGL(ozcombine_kernel, dim3((unsigned)npairs), dim3(256), 0, s, ngp,
+                       (int)npairs, nmod, R, ex, eb, bits, box, h, o, (long long)ldh, accumulate);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace exh

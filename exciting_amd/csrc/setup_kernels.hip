@@ -14,7 +14,7 @@
 // MFMA: v_mfma_f64_16x16x4f64.  Operand lanes: A[i=lane%16][k=lane/16],
 // B[k=lane/16][j=lane%16]; result D[i=lane/16+4*r][j=lane%16], r=0..3
 // (established by the parity tests against the oracle).
-#include "exh_internal.h"
+#include "exh_device.h"
 
 namespace exh {
 namespace {
@@ -28,13 +28,6 @@ __device__ __forceinline__ dx4 mfma(double a, double b, dx4 c) {
 constexpr int TM = 64;    // output tile edge (rows and columns)
 constexpr int TK = 16;    // K slab staged in LDS per step
 constexpr int NTH = 256;  // 4 waves; wave w owns a 32x32 quadrant
-
-__device__ __forceinline__ double2 z2(double r, double i) {
-  double2 v;
-  v.x = r;
-  v.y = i;
-  return v;
-}
 
 // ---------------------------------------------------------------------------
 // ivg(:, igpig(i)) for the G+k set; w = 0 flags an igpig outside 1..ngrtot.
@@ -369,13 +362,14 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
     __syncthreads();
   }
 
-  // ---- epilogue: interstitial gather, accumulate, write upper + mirror ----
+  // ---- epilogue: accumulate, interstitial gather, write upper + mirror ----
+  // (accumulate adds the old upper value before the interstitial term, the
+  // reference order: zgemm beta = 1, then hmlistln/olpistln)
+  IstBox box{ivgk, vgpc, T, lo0, lo1, lo2, n0, n1, n2};
 #pragma unroll
   for (int b = 0; b < 2; b++) {
     const int j = J0 + wc * 32 + b * 16 + lr;
     if (j >= ngp) continue;
-    const int4 gj = ivgk[j];
-    const double vj0 = vgpc[3 * j], vj1 = vgpc[3 * j + 1], vj2 = vgpc[3 * j + 2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -384,35 +378,16 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
         if (i >= ngp || i > j) continue;
         double hr = hre[a][b][r], hi = him[a][b][r];
         double orr = ore[a][b][r], oi = oim[a][b][r];
-        const size_t up = (size_t)i + (size_t)j * ldh;
         if (accumulate) {
-          double2 h0 = H[up], o0 = O[up];
+          const size_t up = (size_t)i + (size_t)j * ldh;
+          const double2 h0 = H[up], o0 = O[up];
           hr += h0.x;
           hi += h0.y;
           orr += o0.x;
           oi += o0.y;
         }
-        const int4 gi = ivgk[i];
-        if (gi.w & gj.w) {
-          const int d0 = gi.x - gj.x - lo0, d1 = gi.y - gj.y - lo1, d2 = gi.z - gj.z - lo2;
-          if ((unsigned)d0 < (unsigned)n0 && (unsigned)d1 < (unsigned)n1 &&
-              (unsigned)d2 < (unsigned)n2) {
-            const IstEntry e = T[d0 + (size_t)n0 * (d1 + (size_t)n1 * d2)];
-            const double t1 = 0.5 * (vgpc[3 * i] * vj0 + vgpc[3 * i + 1] * vj1 +
-                                     vgpc[3 * i + 2] * vj2);
-            hr += e.veff.x + t1 * e.kin.x;
-            hi += e.veff.y + t1 * e.kin.y;
-            orr += e.ocf.x + t1 * e.okin.x;
-            oi += e.ocf.y + t1 * e.okin.y;
-          }
-        }
-        H[up] = z2(hr, hi);
-        O[up] = z2(orr, oi);
-        if (i != j) {
-          const size_t lo = (size_t)j + (size_t)i * ldh;
-          H[lo] = z2(hr, -hi);
-          O[lo] = z2(orr, -oi);
-        }
+        ist_add(box, i, j, hr, hi, orr, oi);
+        store_upper(H, O, ldh, i, j, hr, hi, orr, oi, 0);
       }
   }
 }
@@ -590,6 +565,10 @@ static const double2 *k_major_view(const exh_context *c, const double2 *apwalm,
   }
   *ld = (c->ngkmax + 63) / 64 * 64;
   return (const double2 *)c->at.p;
+}
+
+const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, long long *ld) {
+  return k_major_view(c, apwalm, ld);
 }
 
 hipError_t launch_bt(hipStream_t s, const exh_context *c, int ngp,

@@ -88,7 +88,14 @@ EXPORTS = (
     "exh_profile_read", "exh_profile_reset",
 )
 
-KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack")
+KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
+           "oz_gemm", "oz_combine")
+
+
+def SPLIT(nmod: int) -> int:
+    """Flag selecting the split-precision (int8 Ozaki scheme II) muffin-tin
+    contraction with nmod moduli (6..12)."""
+    return nmod << 8
 
 
 def _is_torch(a) -> bool:

@@ -59,6 +59,14 @@ enum {
    * (seceqnfv.f90:73). */
   EXH_ACCUMULATE = 1
 };
+/* Split precision: EXH_SPLIT(n), 6 <= n <= 12, OR-ed into flags, computes the
+ * muffin-tin APW-APW products with the Ozaki scheme II on int8 matrix cores:
+ * A and aa.A are rounded per G column to b-bit integers (b is the largest
+ * value for which n primes <= 251 recover every product exactly; b = 26 for
+ * n = 9 at K = 15488) and multiplied exactly through n modular int8 GEMMs.
+ * The rounding is the only approximation (relative 2^-b per column).  Match,
+ * the lo blocks and the interstitial stay FP64.  Not combined with h1aa. */
+#define EXH_SPLIT(n) ((n) << 8)
 
 /* ---- context ------------------------------------------------------------ */
 /* No reference counterpart: the reference keeps this state in modules. */
@@ -157,7 +165,10 @@ enum {
   EXH_K_LOBORDER = 5, /* APW-lo rows/columns                           */
   EXH_K_LOLO = 6,     /* lo-lo block                                   */
   EXH_K_PACK = 7,     /* K-major copy of apwalm (non-uniform layouts)  */
-  EXH_K_COUNT = 8
+  EXH_K_OZRES = 8,    /* split: column exponents + int8 residues        */
+  EXH_K_OZGEMM = 9,   /* split: modular int8 GEMMs (MFMA)               */
+  EXH_K_OZCOMB = 10,  /* split: Garner/CRT + interstitial + mirror      */
+  EXH_K_COUNT = 11
 };
 int exh_profile_enable(exh_context *ctx, int on);
 int exh_profile_read(exh_context *ctx, int kernel, double *total_ms, int *launches);
