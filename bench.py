@@ -71,29 +71,14 @@ def oz_bits(nmod, K):
 
 
 def gen_eigvals(H, O, n, nst):
-    """Lowest nst eigenvalues of H x = e O x (FP64), from column-major device
-    buffers (the tensors hold H^T).  GPU: Cholesky, L^-1 by a general inverse,
-    two ZGEMMs and a Hermitian eigensolve; LAPACK on the host if the device
-    solver refuses the size."""
-    import torch
-    Hm = H[:n * n].view(n, n).transpose(0, 1)
-    Om = O[:n * n].view(n, n).transpose(0, 1)
-    # the upper triangles are authoritative (ZHEGVX 'U'); rebuild Hermitian
-    Hm = torch.triu(Hm) + torch.triu(Hm, 1).conj().transpose(0, 1)
-    Om = torch.triu(Om) + torch.triu(Om, 1).conj().transpose(0, 1)
-    try:
-        L = torch.linalg.cholesky(Om)
-        Li = torch.linalg.inv(L)
-        Ht = Li @ Hm @ Li.conj().transpose(0, 1)
-        Ht = 0.5 * (Ht + Ht.conj().transpose(0, 1))
-        w = torch.linalg.eigvalsh(Ht)
-        return w[:nst].cpu().numpy(), "gpu"
-    except RuntimeError as e:
-        log(f"device eigensolve unavailable ({str(e)[:80]}); using LAPACK on the host")
-        import scipy.linalg as sl
-        w = sl.eigh(Hm.cpu().numpy(), Om.cpu().numpy(), eigvals_only=True,
-                    subset_by_index=[0, nst - 1], driver="gvx")
-        return w, "host"
+    """Lowest nst eigenvalues of H x = e O x from the column-major device
+    buffers (the tensors hold H^T): LAPACK ZHEGVX semantics ('I', upper
+    triangles) on the host, as solvewithlapack does (modfvsystem.f90:691)."""
+    import scipy.linalg as sl
+    Hm = H[:n * n].view(n, n).transpose(0, 1).cpu().numpy()
+    Om = O[:n * n].view(n, n).transpose(0, 1).cpu().numpy()
+    return sl.eigh(Hm, Om, lower=False, eigvals_only=True, subset_by_index=[0, nst - 1],
+                   driver="gvx"), "host LAPACK"
 
 
 def accuracy_check(ctx, d, apwalm, one_setup, nmod, nst):

@@ -128,3 +128,4 @@ def test_errors_are_reported(ctxs):
     sysg = lapw.newsystem(k.ngk + s.nlotot + 1)  # wrong rank
     with pytest.raises(lapw.ExhError, match="rank"):
         ctx.hamiltonandoverlapsetup(sysg, k.ngk, np.zeros((1,), np.complex128), k.igpig, k.vgpc)
+
