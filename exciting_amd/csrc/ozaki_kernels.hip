@@ -26,6 +26,7 @@
 // recovered exactly from its residues.
 #include <climits>
 #include <cmath>
+#include <mutex>
 
 #include "exh_device.h"
 
@@ -328,7 +329,18 @@ int oz_bits(int nmod, int Ktot) {
   return (int)std::min(30.0, b);
 }
 
+// The tables cover all kOzMaxMod moduli whatever nmod a call uses, so they are
+// written once per device and process; the mutex keeps contexts on other host
+// threads from writing them while this one does.
 hipError_t oz_upload_constants(int nmod) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static bool done[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> guard(mu);
+  if (dev < kMaxDevices && done[dev]) return hipSuccess;
   int p[kOzMaxMod], w[kOzMaxMod], cc[kOzMaxMod][kOzMaxMod];
   float rp[kOzMaxMod];
   auto inv = [](long long a, int m) {  // a^-1 mod m, m prime
@@ -353,10 +365,11 @@ hipError_t oz_upload_constants(int nmod) {
     }
   }
   (void)nmod;
-  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_p), p, sizeof p);
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_p), p, sizeof p);
   if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_rp), rp, sizeof rp);
   if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_w), w, sizeof w);
   if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_c), cc, sizeof cc);
+  if (e == hipSuccess && dev < kMaxDevices) done[dev] = true;
   return e;
 }
 

@@ -299,7 +299,7 @@ class Context:
 class DeviceKPoint:
     """Per-k inputs of one k-point resident in HBM (torch tensors)."""
 
-    def __init__(self, s: synth.LapwSystem, ik: int, device="cuda"):
+    def __init__(self, s: synth.LapwSystem, ik: int, device="cuda", with_apwalm: bool = True):
         import torch
         k = s.kpts[ik]
         self.ngk = k.ngk
@@ -312,5 +312,5 @@ class DeviceKPoint:
         self.vgpc = t(k.vgpc, torch.float64)
         lmm = (s.lmaxapw + 1) ** 2
         self.apwalm = torch.zeros(s.natmtot * lmm * s.apwordmax * s.ngkmax,
-                                  dtype=torch.complex128, device=device)
+                                  dtype=torch.complex128, device=device) if with_apwalm else None
         self.apwalm_shape = (s.ngkmax, s.apwordmax, lmm, s.natmtot)

@@ -1,0 +1,440 @@
+"""Continuous-batching request loop for setup + eigensolve requests.
+
+A request names a cell (one of a fixed set whose module state -- radial
+data, muffin-tin tables, G-vector maps, interstitial functions -- stays
+resident in HBM across requests), a list of its k-points, a precision
+(FP64 or split with n moduli) and the number of states.  For every k-point
+the server runs match, the H/O setup and the generalized eigensolve on the
+GPU and returns the lowest eigenvalues.
+
+Scheduling:
+  * The device budget is the free HBM reported by hipMemGetInfo when the
+    server starts, minus a reserve.  Every library context is charged its
+    whole footprint (apwalm, H and O of the cell's largest k-point, setup
+    scratch for FP64 and split precision, resident per-k inputs and the
+    eigensolve temporaries, which PyTorch's allocator keeps cached per
+    stream) for as long as it exists.
+  * Requests are admitted in arrival order as soon as an execution slot is
+    free and either an idle context of their cell exists or the budget has
+    room for a new one; idle contexts of other cells are evicted, least
+    recently used first, to make room.  A request that does not fit waits,
+    and later requests that fit are admitted around it (bypass), so one large
+    request does not stall the queue.  A request whose context alone exceeds
+    the budget fails at once.
+  * Each admitted request runs on its own context (own HIP stream and
+    scratch) in its own host thread, so requests that arrive while others are
+    on the GPU start at once and their kernels overlap on the device.  The
+    eigensolves (PyTorch's ROCm linear algebra, not re-entrant across host
+    threads) are serialized; they overlap with other requests' match and
+    setup kernels.
+  * Results do not depend on what else runs: every setup kernel has a fixed
+    reduction order and owns its output tiles (no atomics on H or O), and
+    the eigensolve sees bitwise identical inputs.
+
+Failure handling, forced from the host and tested on the CPU with a fake
+executor: a queued request can be cancelled, a request larger than the
+budget is refused, an executor error fails only its request (the context
+goes back to the pool), and the loop drains in-flight work before it
+returns.
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import hashlib
+import json
+import sys
+import threading
+import time
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    cell: str
+    kpoints: List[int]
+    nmod: int = 0            # 0 = FP64, else split precision with nmod moduli
+    nstfv: int = 40
+    arrival: float = 0.0     # seconds after the start of the run
+    admitted: float = -1.0
+    finished: float = -1.0
+    status: str = "queued"   # queued | running | done | cancelled | failed
+    error: str = ""
+    eigenvalues: Optional[List[np.ndarray]] = None
+    digests: Optional[List[str]] = None   # per k: hash of H and O (if asked)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def context_bytes(sys_, nmod_max: int = 12) -> int:
+    """HBM one context holds while it exists, for a cell (LapwSystem)."""
+    n = sys_.ngkmax + sys_.nlotot
+    lmm = (sys_.lmaxapw + 1) ** 2
+    apwalm = 16 * sys_.ngkmax * sys_.apwordmax * lmm * sys_.natmtot
+    ho = 2 * 16 * n * n
+    K = sys_.natmtot * sys_.maxaa
+    ldb = (sys_.ngkmax + 63) // 64 * 64
+    scratch = 16 * K * ldb                       # aa.A operand
+    if nmod_max:
+        gpad = (sys_.ngkmax + 127) // 128 * 128
+        kpad = (K + 63) // 64 * 64
+        nt = gpad // 128
+        scratch += 7 * nmod_max * kpad * gpad    # int8 residues of A and aa.A
+        scratch += 6 * nmod_max * nt * (nt + 1) // 2 * 128 * 128   # product residues
+    perk = len(sys_.kpts) * sys_.ngkmax * (16 * sys_.natmtot + 8 * 7 + 4)
+    solve = 5 * 16 * n * n                       # Cholesky factor, reduced matrix, workspace
+    return apwalm + ho + scratch + perk + solve
+
+
+class ContextPool:
+    """Per-cell pools of idle library contexts with the cell's tables
+    resident; tracks the bytes of all contexts (idle and in use)."""
+
+    def __init__(self, make_context: Callable, close_context: Callable):
+        self.make = make_context
+        self.close = close_context
+        self.idle: Dict[str, list] = {}
+        self.lru: List[tuple] = []   # (last use, cell, ctx, bytes) of idle contexts
+        self.bytes = 0
+        self.created = 0
+        self.evicted = 0
+
+    def take_idle(self, cell: str):
+        lst = self.idle.get(cell)
+        if not lst:
+            return None
+        ctx = lst.pop()
+        self.lru = [e for e in self.lru if e[2] is not ctx]
+        return ctx
+
+    def evict_one(self) -> bool:
+        if not self.lru:
+            return False
+        self.lru.sort(key=lambda e: e[0])
+        _, cell, ctx, nbytes = self.lru.pop(0)
+        self.idle[cell].remove(ctx)
+        self.close(ctx)
+        self.bytes -= nbytes
+        self.evicted += 1
+        return True
+
+    def give_back(self, cell: str, ctx, nbytes: int, now: float) -> None:
+        self.idle.setdefault(cell, []).append(ctx)
+        self.lru.append((now, cell, ctx, nbytes))
+
+    def close_all(self) -> None:
+        while self.evict_one():
+            pass
+
+
+_NEW = object()      # admission result: build a new context in the worker
+_NEVER = object()    # admission result: the context alone exceeds the budget
+
+
+class Server:
+    """Admission + execution loop.
+
+    ``execute(req, ctx)`` runs one request on one context and returns its
+    per-k eigenvalues.  ``make_context(cell)``/``close_context(ctx)`` build
+    and free a context with a cell's tables resident; ``footprint(cell)`` is
+    the bytes a context of that cell holds.  Everything is injectable so the
+    scheduling logic is testable without a GPU."""
+
+    def __init__(self, cells: Dict[str, object], execute: Callable, make_context: Callable,
+                 close_context: Callable, budget_bytes: int, max_inflight: int = 8,
+                 footprint: Callable = context_bytes,
+                 clock: Callable[[], float] = time.perf_counter,
+                 on_finish: Optional[Callable[[Request], None]] = None):
+        self.cells = cells
+        self.execute = execute
+        self.pool = ContextPool(make_context, close_context)
+        self.budget = int(budget_bytes)
+        self.max_inflight = max_inflight
+        self.footprint = footprint
+        self.clock = clock
+        self.on_finish = on_finish
+        self.lock = threading.Lock()
+        self.inflight: Dict[int, threading.Thread] = {}
+        self.cancelled = set()
+        self.peak_inflight = 0
+        self.peak_bytes = 0
+
+    def cancel(self, rid: int) -> None:
+        """Cancel a request that has not been admitted yet."""
+        with self.lock:
+            self.cancelled.add(rid)
+
+    def _admit(self, req: Request):
+        """An idle context, _NEW (bytes reserved), None (not now) or
+        _NEVER.  Called with the lock held."""
+        ctx = self.pool.take_idle(req.cell)
+        if ctx is not None:
+            return ctx
+        need = self.footprint(self.cells[req.cell])
+        if need > self.budget:
+            return _NEVER
+        idle_bytes = sum(e[3] for e in self.pool.lru)
+        if self.pool.bytes - idle_bytes + need > self.budget:
+            return None          # would not fit even with every idle context evicted
+        while self.pool.bytes + need > self.budget:
+            self.pool.evict_one()
+        self.pool.bytes += need
+        self.pool.created += 1
+        self.peak_bytes = max(self.peak_bytes, self.pool.bytes)
+        return _NEW
+
+    def _worker(self, req: Request, t0: float, ctx):
+        cell = self.cells[req.cell]
+        nbytes = self.footprint(cell)
+        try:
+            if ctx is _NEW:
+                ctx = self.pool.make(cell)
+            req.eigenvalues = self.execute(req, ctx)
+            req.status = "done"
+        except Exception as e:  # reported per request; the loop keeps serving
+            req.status = "failed"
+            req.error = f"{type(e).__name__}: {e}"
+        finally:
+            req.finished = self.clock() - t0
+            with self.lock:
+                if ctx is _NEW:          # building the context failed
+                    self.pool.bytes -= nbytes
+                else:
+                    self.pool.give_back(req.cell, ctx, nbytes, req.finished)
+                self.inflight.pop(req.rid, None)
+            if self.on_finish is not None:
+                self.on_finish(req)
+
+    def run(self, requests: List[Request], poll: float = 0.0005) -> List[Request]:
+        """Serve ``requests`` (arrival times relative to the start) to
+        completion; fills in admission and finish times and results."""
+        arrivals = sorted(requests, key=lambda r: r.arrival)
+        queue: List[Request] = []
+        t0 = self.clock()
+        nxt = 0
+        while True:
+            with self.lock:
+                busy = bool(self.inflight)
+            if nxt >= len(arrivals) and not queue and not busy:
+                break
+            now = self.clock() - t0
+            while nxt < len(arrivals) and arrivals[nxt].arrival <= now:
+                queue.append(arrivals[nxt])
+                nxt += 1
+            for req in list(queue):
+                with self.lock:
+                    if req.rid in self.cancelled:
+                        req.status = "cancelled"
+                        req.finished = now
+                        queue.remove(req)
+                        continue
+                    if len(self.inflight) >= self.max_inflight:
+                        break
+                    got = self._admit(req)
+                    if got is _NEVER:
+                        req.status = "failed"
+                        req.error = "request larger than the device budget"
+                        req.finished = now
+                        queue.remove(req)
+                        continue
+                    if got is None:
+                        continue  # bypass: later requests may fit
+                    req.status = "running"
+                    req.admitted = now
+                    queue.remove(req)
+                    th = threading.Thread(target=self._worker, args=(req, t0, got), daemon=True)
+                    self.inflight[req.rid] = th
+                    self.peak_inflight = max(self.peak_inflight, len(self.inflight))
+                    th.start()
+            wait = poll
+            if nxt < len(arrivals) and not queue:
+                with self.lock:
+                    idle = not self.inflight
+                if idle:
+                    wait = max(poll, arrivals[nxt].arrival - (self.clock() - t0))
+            time.sleep(min(wait, 0.05))
+        return requests
+
+
+# ---------------------------------------------------------------------------
+# GPU executor
+# ---------------------------------------------------------------------------
+def gpu_executor(device: int = 0, digests: bool = False):
+    """(make_context, close_context, execute) running the library and the
+    eigensolve on ``cuda:device``.  With ``digests`` each request also
+    records a SHA-1 of the bytes of H and O per k-point (copies them to the
+    host: for tests on small cells)."""
+    from . import solve as _solve
+    _solve.configure()
+    import torch
+    from . import lapw
+    dev = f"cuda:{device}"
+    solve_lock = threading.Lock()
+
+    def make_context(sys_):
+        ctx = lapw.Context(device).load(sys_)
+        ctx.kp = {}      # per-k inputs, uploaded on first use and kept
+        n = sys_.ngkmax + sys_.nlotot
+        lmm = (sys_.lmaxapw + 1) ** 2
+        ctx.tstream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+        with torch.cuda.stream(ctx.tstream):
+            ctx.apwalm = torch.zeros(sys_.natmtot * lmm * sys_.apwordmax * sys_.ngkmax,
+                                     dtype=torch.complex128, device=dev)
+            ctx.H = torch.empty(n * n, dtype=torch.complex128, device=dev)
+            ctx.O = torch.empty(n * n, dtype=torch.complex128, device=dev)
+        ctx.tstream.synchronize()
+        return ctx
+
+    def execute(req: Request, ctx):
+        s = ctx.sys
+        out, dg = [], []
+        with torch.cuda.stream(ctx.tstream):
+            for ik in req.kpoints:
+                d = ctx.kp.get(ik)
+                if d is None:
+                    d = lapw.DeviceKPoint(s, ik, device=dev, with_apwalm=False)
+                    ctx.kp[ik] = d
+                ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, ctx.apwalm, stream=ctx.stream)
+                nn = d.rank * d.rank
+                sysd = lapw.EvSystem(lapw.HermitianMatrix(d.rank, ctx.H[:nn]),
+                                     lapw.HermitianMatrix(d.rank, ctx.O[:nn]))
+                ctx.hamiltonandoverlapsetup(sysd, d.ngk, ctx.apwalm, d.igpig, d.vgpc,
+                                            flags=lapw.SPLIT(req.nmod) if req.nmod else 0,
+                                            stream=ctx.stream)
+                if digests:
+                    h = hashlib.sha1(ctx.H[:nn].cpu().numpy().tobytes())
+                    h.update(ctx.O[:nn].cpu().numpy().tobytes())
+                    dg.append(h.hexdigest())
+                with solve_lock:
+                    w = _solve.geneig(ctx.H, ctx.O, d.rank, min(req.nstfv, d.rank))
+                    out.append(w.cpu().numpy())
+        if digests:
+            req.digests = dg
+        return out
+
+    def close_context(ctx):
+        ctx.tstream.synchronize()
+        ctx.close()
+        ctx.kp.clear()
+        ctx.apwalm = ctx.H = ctx.O = None
+        torch.cuda.empty_cache()
+
+    return make_context, close_context, execute
+
+
+def free_hbm(device: int = 0) -> int:
+    import torch
+    return int(torch.cuda.mem_get_info(device)[0])
+
+
+def poisson_requests(n: int, rate: float, cells: Dict[str, object], seed: int = 0,
+                     kmax: int = 16, split_fraction: float = 0.5, nmod: int = 9,
+                     nstfv: int = 40, start: int = 0) -> List[Request]:
+    """n requests with exponential inter-arrival times (``rate`` per second),
+    cells uniform over ``cells``, 1..kmax distinct k-points of the cell, split
+    precision with probability ``split_fraction``."""
+    rng = np.random.default_rng(seed)
+    names = sorted(cells)
+    t = 0.0
+    out = []
+    for rid in range(start, start + n):
+        t += rng.exponential(1.0 / rate)
+        cell = names[rng.integers(len(names))]
+        nk_cell = len(cells[cell].kpts)
+        nk = int(rng.integers(1, min(kmax, nk_cell) + 1))
+        ks = sorted(rng.choice(nk_cell, size=nk, replace=False).tolist())
+        out.append(Request(rid=rid, cell=cell, kpoints=ks,
+                           nmod=nmod if rng.random() < split_fraction else 0,
+                           nstfv=nstfv, arrival=t))
+    return out
+
+
+def summarize(reqs: List[Request], wall: float) -> dict:
+    done = [r for r in reqs if r.status == "done"]
+    lat = np.array([r.latency for r in done]) if done else np.zeros(1)
+    wait = np.array([r.admitted - r.arrival for r in done]) if done else np.zeros(1)
+    setups = sum(len(r.kpoints) for r in done)
+    return {
+        "requests": len(reqs), "done": len(done),
+        "failed": sum(r.status == "failed" for r in reqs),
+        "cancelled": sum(r.status == "cancelled" for r in reqs),
+        "setups": setups, "wall_s": wall,
+        "setups_per_s": setups / wall if wall > 0 else 0.0,
+        "requests_per_s": len(done) / wall if wall > 0 else 0.0,
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": float(lat.max())},
+        "queue_wait_s": {"p50": float(np.percentile(wait, 50)),
+                         "p99": float(np.percentile(wait, 99))},
+    }
+
+
+def main(argv=None) -> int:
+    """Serve a Poisson request stream over the 2/8/16/32/64-atom cells on one
+    GPU and print one JSON line with p50/p99 latency and setups/s."""
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=2.0, help="arrivals per second")
+    ap.add_argument("--kmax", type=int, default=16)
+    ap.add_argument("--nstfv", type=int, default=40)
+    ap.add_argument("--nmod", type=int, default=9)
+    ap.add_argument("--split-fraction", type=float, default=0.5)
+    ap.add_argument("--max-inflight", type=int, default=8)
+    ap.add_argument("--reserve-gib", type=float, default=16.0)
+    ap.add_argument("--cells", default="fcc2,sc8,ortho16,sc32,sc64")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    from . import solve as _solve
+    _solve.configure()
+    from . import synth
+    t = time.perf_counter()
+    specs = synth.serving_cells(args.kmax)
+    cells = {}
+    for name in args.cells.split(","):
+        spec, ks = specs[name]
+        cells[name] = synth.build(spec, kpoints=ks)
+    print(f"[serve] cells built in {time.perf_counter() - t:.1f} s: "
+          + ", ".join(f"{k} ngk<={c.ngkmax} nk={len(c.kpts)}" for k, c in cells.items()),
+          file=sys.stderr, flush=True)
+    make, close, execute = gpu_executor(0)
+    budget = free_hbm(0) - int(args.reserve_gib * 2**30)
+    def report(r):
+        print(f"[serve] request {r.rid} {r.cell} nk={len(r.kpoints)} nmod={r.nmod} "
+              f"{r.status} latency {r.latency:.3f} s {r.error}", file=sys.stderr, flush=True)
+
+    srv = Server(cells, execute, make, close, budget, max_inflight=args.max_inflight,
+                 on_finish=report)
+    # warm-up, untimed: one context per cell, one FP64 and one split k-point
+    warm = [Request(rid=-1 - i, cell=c, kpoints=[0], nmod=m, nstfv=args.nstfv)
+            for i, (c, m) in enumerate((c, m) for c in cells for m in (0, args.nmod))]
+    srv.run(warm)
+    bad = [r for r in warm if r.status != "done"]
+    if bad:
+        print(f"[serve] warm-up failed: {bad[0].error}", file=sys.stderr, flush=True)
+        return 1
+    print("[serve] warm-up done", file=sys.stderr, flush=True)
+    reqs = poisson_requests(args.requests, args.rate, cells, seed=args.seed, kmax=args.kmax,
+                            split_fraction=args.split_fraction, nmod=args.nmod,
+                            nstfv=args.nstfv)
+    t = time.perf_counter()
+    srv.run(reqs)
+    wall = time.perf_counter() - t
+    out = summarize(reqs, wall)
+    out.update({"workload": "Poisson stream over fcc2/sc8/ortho16/sc32/sc64 cells",
+                "rate_per_s": args.rate, "kmax": args.kmax, "nstfv": args.nstfv,
+                "split_fraction": args.split_fraction, "nmod": args.nmod,
+                "max_inflight": args.max_inflight, "budget_gib": budget / 2**30,
+                "peak_inflight": srv.peak_inflight, "contexts_created": srv.pool.created,
+                "contexts_evicted": srv.pool.evicted,
+                "peak_context_gib": srv.peak_bytes / 2**30})
+    print(json.dumps(out), flush=True)
+    srv.pool.close_all()
+    return 0 if out["failed"] == 0 else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

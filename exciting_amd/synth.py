@@ -72,8 +72,8 @@ _ROCKSALT_B = np.array([[.5, 0, 0], [0, .5, 0], [0, 0, .5], [.5, .5, .5]], float
 
 
 def config(index: int) -> CellSpec:
-    """The four setup workloads (fcc 2-atom, SC 8-atom, orthorhombic 16-atom,
-    SC 64-atom supercell)."""
+    """The setup workloads: 0 fcc 2-atom, 1 SC 8-atom, 2 orthorhombic 16-atom,
+    3 SC 64-atom supercell, 4 SC 32-atom supercell (serving mix only)."""
     if index == 0:
         return CellSpec(
             name="fcc2", avec=_fcc(10.26),
@@ -117,7 +117,36 @@ def config(index: int) -> CellSpec:
             name="sc64", avec=a * np.eye(3), positions=sp, rmt=[2.0, 2.0],
             lmaxapw=10, lmaxmat=10, apword=2, lo_l=[[0, 1, 2, 3]] * 2,
             rgkmax=8.0, gmaxvr=12.0, kpoints=_shifted_grid(4, 2, 2), seed=1003)
+    if index == 4:
+        # 32-atom 2x2x1 supercell of the 8-atom rock-salt cell, jittered like
+        # the 64-atom one; 16 k-points on a shifted 2x2x4 grid
+        rng = np.random.default_rng(1004)
+        avec = np.diag([20.52, 20.52, 10.26])
+        sp = []
+        for base in (_ROCKSALT_A, _ROCKSALT_B):
+            pts = []
+            for s in np.ndindex(2, 2, 1):
+                for p in base:
+                    pts.append((p + np.array(s, float)) / np.array([2.0, 2.0, 1.0]))
+            pts = np.array(pts)
+            pts = pts + rng.normal(0.0, 0.05, pts.shape) / np.diag(avec)
+            sp.append(pts)
+        return CellSpec(
+            name="sc32", avec=avec, positions=sp, rmt=[2.0, 2.0],
+            lmaxapw=10, lmaxmat=10, apword=2, lo_l=[[0, 1, 2, 3]] * 2,
+            rgkmax=8.0, gmaxvr=12.0, kpoints=_shifted_grid(2, 2, 4), seed=1004)
     raise ValueError(f"no workload {index}")
+
+
+def serving_cells(kmax: int = 16):
+    """The request mix's cells, 2, 8, 16, 32 and 64 atoms, built as the setup
+    workloads with at most ``kmax`` k-points each: {name: CellSpec, kpoint
+    indices}."""
+    out = {}
+    for idx in (0, 1, 2, 4, 3):
+        spec = config(idx)
+        out[spec.name] = (spec, list(range(min(kmax, len(spec.kpoints)))))
+    return out
 
 
 def tiny(seed: int = 7, lmax: int = 3, nlo_l: Sequence[int] = (0, 1),
