@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexh.so")
-SOURCES = ["api.hip", "setup_kernels.hip", "match_kernel.hip", "ozaki_kernels.hip"]
+SOURCES = ["api.hip", "setup_kernels.hip", "match_kernel.hip", "ozaki_kernels.hip", "batch.hip"]
 ARCH = os.environ.get("EXH_OFFLOAD_ARCH", "gfx950")
 
 

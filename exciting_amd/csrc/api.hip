@@ -22,6 +22,7 @@ int hip_fail(exh_context *c, hipError_t e, const char *where) {
 
 int ensure(exh_context *c, DevBuf &b, size_t bytes) {
   if (bytes <= b.bytes && b.p) return 0;
+  if (c) c->epoch++;  // device addresses change: cached batch graphs are stale
   if (b.p) {
     hipError_t e = hipFree(b.p);
     b.p = nullptr;
@@ -38,7 +39,7 @@ int ensure(exh_context *c, DevBuf &b, size_t bytes) {
   return 0;
 }
 
-static void release(DevBuf &b) {
+void release(DevBuf &b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
@@ -169,7 +170,7 @@ static int build_match_tables(exh_context *c) {
   return rc;
 }
 
-static hipEvent_t take_event(exh_context *c) {
+hipEvent_t take_event(exh_context *c) {
   if (!c->evpool.empty()) {
     hipEvent_t e = c->evpool.back();
     c->evpool.pop_back();
@@ -181,19 +182,7 @@ static hipEvent_t take_event(exh_context *c) {
 }
 
 // Bracket one launch with events when profiling is on.
-template <typename F>
-static hipError_t timed(exh_context *c, int k, hipStream_t s, F &&launch) {
-  if (!c->prof) return launch();
-  hipEvent_t a = take_event(c), b = take_event(c);
-  if (!a || !b) return hipErrorOutOfMemory;
-  hipError_t e = hipEventRecord(a, s);
-  if (e == hipSuccess) e = launch();
-  if (e == hipSuccess) e = hipEventRecord(b, s);
-  c->recs.push_back({k, a, b});
-  return e;
-}
-
-static int drain_records(exh_context *c) {
+int drain_records(exh_context *c) {
   for (auto &r : c->recs) {
     hipError_t e = hipEventSynchronize(r.b);
     float ms = 0.f;
@@ -206,6 +195,27 @@ static int drain_records(exh_context *c) {
   }
   c->recs.clear();
   return 0;
+}
+
+}  // namespace exh
+
+namespace exh {
+
+// Constants and scratch of the split-precision path for nmod moduli.
+int oz_prepare(exh_context *c, int nmod, OzLayout &L, int &bits) {
+  L = oz_layout(c->ngkmax, c->Ktot, nmod);
+  bits = oz_bits(nmod, c->Ktot);
+  if (!c->oz_consts) {
+    hipError_t e = oz_upload_constants(nmod);
+    if (e != hipSuccess) return hip_fail(c, e, "split: constants");
+    c->oz_consts = true;
+  }
+  int rc = ensure(c, c->ozqx, L.qx_bytes);
+  if (!rc) rc = ensure(c, c->ozqb, L.qb_bytes);
+  if (!rc) rc = ensure(c, c->ozr, L.r_bytes);
+  if (!rc) rc = ensure(c, c->ozex, (size_t)c->ngkmax * sizeof(int));
+  if (!rc) rc = ensure(c, c->ozeb, (size_t)c->ngkmax * sizeof(int));
+  return rc;
 }
 
 }  // namespace exh
@@ -225,6 +235,12 @@ int exh_create(int device, exh_context **out) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        ncu > 0)
+      c->ncu = ncu;
+  }
   if (e != hipSuccess) {
     int rc = (int)e;
     delete c;
@@ -238,12 +254,14 @@ void exh_destroy(exh_context *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  batch_release_all(c);
   DevBuf *bufs[] = {&c->colA, &c->katom, &c->atom_species_d, &c->apword_d, &c->omax_d,
                     &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
                     &c->aa, &c->loa, &c->lolo, &c->salo, &c->ololo_eff, &c->h1aa,
                     &c->ivg, &c->ivgig, &c->ist, &c->cf_stage, &c->vf_stage, &c->mf_stage,
                     &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex, &c->ozeb, &c->s_apwalm, &c->s_igpig,
-                    &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac};
+                    &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac,
+                    &c->bbt, &c->bat, &c->ba2t, &c->bivgk};
   for (DevBuf *b : bufs) release(*b);
   for (auto &r : c->recs) {
     (void)hipEventDestroy(r.a);
@@ -262,6 +280,7 @@ int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, 
                   int apwordmax, const int *apword, int nlomax, const int *nlorb,
                   const int *lorbl, int ngkmax) {
   EXH_CHECK_CTX(c);
+  c->epoch++;
   if (nspecies <= 0 || !natoms || !apword || !nlorb || (nlomax > 0 && !lorbl))
     return fail(c, EXH_EINVAL, "set_basis: null or empty argument");
   if (lmaxapw < 0 || lmaxmat < 0 || lmaxmat > lmaxapw)
@@ -379,6 +398,7 @@ int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, 
 int exh_set_radial(exh_context *c, double omega, const double *rmt, const int *nrmt,
                    int nrmtmax, const double *spr, const double *apwfr) {
   EXH_CHECK_CTX(c);
+  c->epoch++;
   if (!c->have_basis) return fail(c, EXH_ESTATE, "set_radial: call exh_set_basis first");
   if (!(omega > 0.0) || !rmt || !nrmt || !spr || !apwfr || nrmtmax <= 0)
     return fail(c, EXH_EINVAL, "set_radial: bad argument");
@@ -402,6 +422,7 @@ int exh_set_mt(exh_context *c, int maxaa, int maxnlo, const exh_z *aa, const exh
                const exh_z *lolo, const double *oalo, const double *ololo, const double *h1aa,
                const double *h1loa, const double *h1lolo) {
   EXH_CHECK_CTX(c);
+  c->epoch++;
   if (!c->have_basis) return fail(c, EXH_ESTATE, "set_mt: call exh_set_basis first");
   if (maxaa != c->maxaa || maxnlo != c->maxnlo)
     return fail(c, EXH_EINVAL, "set_mt: maxaa/maxnlo differ from the basis");
@@ -441,6 +462,7 @@ int exh_set_mt(exh_context *c, int maxaa, int maxnlo, const exh_z *aa, const exh
 int exh_set_gvectors(exh_context *c, int ngrtot, int ngvec, const int *ivg, const int *intgv,
                      const int *ivgig) {
   EXH_CHECK_CTX(c);
+  c->epoch++;
   if (ngrtot <= 0 || ngvec <= 0 || ngvec > ngrtot || !ivg || !intgv || !ivgig)
     return fail(c, EXH_EINVAL, "set_gvectors: bad argument");
   for (int d = 0; d < 3; d++)
@@ -464,6 +486,7 @@ int exh_set_interstitial(exh_context *c, const exh_z *cfunig, const exh_z *veffi
                          const exh_z *meffig, const exh_z *m2effig, int valence_rel_none,
                          int h1on) {
   EXH_CHECK_CTX(c);
+  c->epoch++;
   if (!c->have_gvec) return fail(c, EXH_ESTATE, "set_interstitial: call exh_set_gvectors first");
   if (!cfunig || !veffig || (!valence_rel_none && !meffig) || (h1on && !m2effig))
     return fail(c, EXH_EINVAL, "set_interstitial: null array");
@@ -538,18 +561,9 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
                        (const int4 *)c->ivgk.p, vgpc, (double2 *)h, (double2 *)o, rank, flags);
     });
   if (e == hipSuccess && nmod > 0) {
-    const OzLayout L = oz_layout(c->ngkmax, c->Ktot, nmod);
-    const int bits = oz_bits(nmod, c->Ktot);
-    if (!c->oz_consts) {
-      e = oz_upload_constants(nmod);
-      c->oz_consts = e == hipSuccess;
-    }
-    int rc2 = 0;
-    if (e == hipSuccess) rc2 = ensure(c, c->ozqx, L.qx_bytes);
-    if (!rc2) rc2 = ensure(c, c->ozqb, L.qb_bytes);
-    if (!rc2) rc2 = ensure(c, c->ozr, L.r_bytes);
-    if (!rc2) rc2 = ensure(c, c->ozex, (size_t)c->ngkmax * sizeof(int));
-    if (!rc2) rc2 = ensure(c, c->ozeb, (size_t)c->ngkmax * sizeof(int));
+    OzLayout L;
+    int bits = 0;
+    const int rc2 = oz_prepare(c, nmod, L, bits);
     if (rc2) return rc2;
     long long ldx;
     const double2 *X = k_major_operand(c, A, &ldx);

@@ -43,6 +43,30 @@ struct OzLayout {
   size_t qx_bytes = 0, qb_bytes = 0, r_bytes = 0;
 };
 
+// Per-k-point descriptor of a grouped (batched) launch, in device memory.
+// Match inputs, apwalm, the G+k maps and H/O are the caller's; x, at, bt,
+// a2t and ivgk point into the context's batch scratch.
+struct KDesc {
+  int ngp, ldh;
+  const double *gpc, *tpgpc;
+  const double2 *sfac;
+  double2 *apwalm;
+  const int *igpig;
+  const double *vgpc;
+  double2 *h, *o;
+  const double2 *x;  // K-major A: apwalm itself (equally spaced columns) or at
+  double2 *at, *bt, *a2t;
+  int4 *ivgk;
+};
+
+// Tile shapes of the grouped H/O kernel: output TMI x TMJ per workgroup
+// (4 or 8 waves of 32 x 32), K slabs of TK staged in LDS.
+struct HoShapeInfo {
+  int tmi, tmj, tk;
+};
+constexpr int kHoShapes = 5;
+extern const HoShapeInfo kHoShapeInfo[kHoShapes];
+
 struct MatchTables {
   // LU factors of the boundary-derivative matrix D per (l, ias): pmax*pmax
   // doubles (column-major, zgetf2 layout) and pmax 1-based pivots.
@@ -104,6 +128,18 @@ struct exh_context {
   double prof_ms[EXH_K_COUNT] = {0};
   int prof_n[EXH_K_COUNT] = {0};
 
+  // batched launches: plan (-1 = default), scratch cap, per-k scratch, and
+  // cached descriptor/tile uploads and graphs keyed by the call's arguments
+  int ncu = 256;
+  int batch_plan = -1;
+  size_t batch_ws_limit = size_t(8) << 30;
+  uint64_t epoch = 0;  // bumped by every state change and scratch reallocation
+  bool capturing = false;
+  exh::DevBuf bbt, bat, ba2t, bivgk;
+  struct BatchEntry;
+  std::vector<BatchEntry *> bcache;
+  uint64_t bclock = 0;
+
   // per-call scratch
   exh::DevBuf bt, a2t, at, ivgk;
   exh::DevBuf ozqx, ozqb, ozr, ozex, ozeb;
@@ -118,6 +154,24 @@ namespace exh {
 int fail(exh_context *c, int code, const std::string &msg);
 int hip_fail(exh_context *c, hipError_t e, const char *where);
 int ensure(exh_context *c, DevBuf &b, size_t bytes);
+void release(DevBuf &b);
+hipEvent_t take_event(exh_context *c);
+int drain_records(exh_context *c);
+
+// Enqueue a launch, bracketed by profiling events when profiling is on (not
+// while the stream is being captured into a graph).
+template <typename F>
+hipError_t timed(exh_context *c, int k, hipStream_t s, F &&launch) {
+  if (!c->prof || c->capturing) return launch();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, s);
+  if (e == hipSuccess) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(b, s);
+  c->recs.push_back({k, a, b});
+  return e;
+}
+void batch_release_all(exh_context *c);
 
 // Kernel launchers (setup_kernels.hip / match_kernel.hip).
 hipError_t launch_ivgk(hipStream_t s, int ngp, const int *igpig, const int *ivg,
@@ -146,6 +200,7 @@ const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, long
 int oz_bits(int nmod, int Ktot);
 hipError_t oz_upload_constants(int nmod);
 OzLayout oz_layout(int ngkmax, int Ktot, int nmod);
+int oz_prepare(exh_context *c, int nmod, OzLayout &L, int &bits);
 struct IstBox;
 hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const double2 *X,
                         long long ldx, const double2 *B, long long ldb, int nmod, int bits,
@@ -155,5 +210,26 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
 hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
                         const double *gpc, const double *tpgpc,
                         const double2 *sfacgp, double2 *apwalm);
+
+// Grouped launchers: one launch covers nk k-points described by kd (device);
+// ngpmax bounds their ngp.
+hipError_t launch_match_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                                int ngpmax);
+hipError_t launch_ivgk_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                               int ngpmax);
+hipError_t launch_pack_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                               int ngpmax, int ldb);
+hipError_t launch_bt_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                             int ngpmax, long long ldx, int ldb);
+hipError_t launch_a2t_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                              int ngpmax, int ldb);
+hipError_t launch_ho_grouped(hipStream_t s, const exh_context *c, int shape, const KDesc *kd,
+                             const int4 *tiles, int ntiles, int nwg, long long ldx, int ldb,
+                             int accumulate);
+int ho_grouped_occupancy(int shape);
+hipError_t launch_lo_border_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                                    int ngpmax, int accumulate);
+hipError_t launch_lolo_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                               int accumulate);
 
 }  // namespace exh

@@ -178,15 +178,13 @@ __device__ __forceinline__ void d_sbesseldm(int m, double x,
 }
 
 template <int LM>
-__global__ __launch_bounds__(64) void match_kernel(
-    int ngp, int ld, int pmax, const double *__restrict__ gpc,
+__device__ __forceinline__ void match_body(
+    int g, int ias, int ngp, int ld, int pmax, const double *__restrict__ gpc,
     const double *__restrict__ tpgpc, const double2 *__restrict__ sfacgp,
     double2 *__restrict__ apwalm, const int *__restrict__ atom_species,
     const double *__restrict__ rmt, const int *__restrict__ apword,
     const int *__restrict__ omaxs, const double *__restrict__ lu,
     const int *__restrict__ piv, double fpso) {
-  const int g = blockIdx.x * 64 + threadIdx.x;
-  const int ias = blockIdx.y;
   if (g >= ngp) return;
   const int is = atom_species[ias];
   const int omax = omaxs[is];
@@ -352,6 +350,30 @@ __global__ __launch_bounds__(64) void match_kernel(
   }
 }
 
+template <int LM>
+__global__ __launch_bounds__(64) void match_kernel(
+    int ngp, int ld, int pmax, const double *__restrict__ gpc,
+    const double *__restrict__ tpgpc, const double2 *__restrict__ sfacgp,
+    double2 *__restrict__ apwalm, const int *__restrict__ atom_species,
+    const double *__restrict__ rmt, const int *__restrict__ apword,
+    const int *__restrict__ omaxs, const double *__restrict__ lu,
+    const int *__restrict__ piv, double fpso) {
+  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, ngp, ld, pmax, gpc, tpgpc, sfacgp,
+                 apwalm, atom_species, rmt, apword, omaxs, lu, piv, fpso);
+}
+
+// Grouped over k-points: blockIdx.z selects the descriptor.
+template <int LM>
+__global__ __launch_bounds__(64) void match_grouped_kernel(
+    const KDesc *__restrict__ kd, int ld, int pmax, const int *__restrict__ atom_species,
+    const double *__restrict__ rmt, const int *__restrict__ apword,
+    const int *__restrict__ omaxs, const double *__restrict__ lu,
+    const int *__restrict__ piv, double fpso) {
+  const KDesc &d = kd[blockIdx.z];
+  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, ld, pmax, d.gpc, d.tpgpc,
+                 d.sfac, d.apwalm, atom_species, rmt, apword, omaxs, lu, piv, fpso);
+}
+
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
@@ -375,6 +397,32 @@ hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
     hipLaunchKernelGGL(match_kernel<L>, grid, dim3(64), 0, s, ngp, c->ngkmax,         \
                        c->apwordmax, gpc, tpgpc, sfacgp, apwalm, as,                  \
                        (const double *)c->rmt_d.p, apw, om, lu, pv, fpso);            \
+    break;
+  switch (c->lmaxapw) {
+    EXH_M(0) EXH_M(1) EXH_M(2) EXH_M(3) EXH_M(4) EXH_M(5) EXH_M(6)
+    EXH_M(7) EXH_M(8) EXH_M(9) EXH_M(10) EXH_M(11) EXH_M(12)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef EXH_M
+  return hipGetLastError();
+}
+
+hipError_t launch_match_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                                int ngpmax) {
+  if (nk <= 0 || ngpmax <= 0 || c->natmtot <= 0) return hipSuccess;
+  const double fpso = 12.566370614359172954 / sqrt(c->omega);
+  dim3 grid(cdiv(ngpmax, 64), c->natmtot, nk);
+  const int *as = (const int *)c->atom_species_d.p;
+  const int *apw = (const int *)c->apword_d.p;
+  const int *om = (const int *)c->omax_d.p;
+  const double *lu = (const double *)c->mt_match.lu.p;
+  const int *pv = (const int *)c->mt_match.piv.p;
+#define EXH_M(L)                                                                        \
+  case L:                                                                               \
+    hipLaunchKernelGGL(match_grouped_kernel<L>, grid, dim3(64), 0, s, kd, c->ngkmax,    \
+                       c->apwordmax, as, (const double *)c->rmt_d.p, apw, om, lu, pv,   \
+                       fpso);                                                           \
     break;
   switch (c->lmaxapw) {
     EXH_M(0) EXH_M(1) EXH_M(2) EXH_M(3) EXH_M(4) EXH_M(5) EXH_M(6)

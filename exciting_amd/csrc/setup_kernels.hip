@@ -32,10 +32,9 @@ constexpr int NTH = 256;  // 4 waves; wave w owns a 32x32 quadrant
 // ---------------------------------------------------------------------------
 // ivg(:, igpig(i)) for the G+k set; w = 0 flags an igpig outside 1..ngrtot.
 // ---------------------------------------------------------------------------
-__global__ void ivgk_kernel(int ngp, const int *__restrict__ igpig,
-                            const int *__restrict__ ivg, int ngrtot,
-                            int4 *__restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ivgk_body(int i, int ngp, const int *__restrict__ igpig,
+                                          const int *__restrict__ ivg, int ngrtot,
+                                          int4 *__restrict__ out) {
   if (i >= ngp) return;
   int g = igpig[i] - 1;
   int4 v = make_int4(0, 0, 0, 0);
@@ -46,6 +45,19 @@ __global__ void ivgk_kernel(int ngp, const int *__restrict__ igpig,
     v.w = 1;
   }
   out[i] = v;
+}
+
+__global__ void ivgk_kernel(int ngp, const int *__restrict__ igpig,
+                            const int *__restrict__ ivg, int ngrtot,
+                            int4 *__restrict__ out) {
+  ivgk_body(blockIdx.x * blockDim.x + threadIdx.x, ngp, igpig, ivg, ngrtot, out);
+}
+
+// Grouped over k-points: blockIdx.y selects the descriptor.
+__global__ void ivgk_grouped_kernel(const KDesc *__restrict__ kd, const int *__restrict__ ivg,
+                                    int ngrtot) {
+  const KDesc &d = kd[blockIdx.y];
+  ivgk_body(blockIdx.x * blockDim.x + threadIdx.x, d.ngp, d.igpig, ivg, ngrtot, d.ivgk);
 }
 
 // ---------------------------------------------------------------------------
@@ -91,20 +103,25 @@ __global__ void pack_kernel(int ngp, const double2 *__restrict__ apwalm,
   if (g < ngp) at[(size_t)k * ldb + g] = apwalm[colA[k] + g];
 }
 
+__global__ void pack_grouped_kernel(const KDesc *__restrict__ kd,
+                                    const long long *__restrict__ colA, int ldb) {
+  const KDesc &d = kd[blockIdx.z];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (g < d.ngp) d.at[(size_t)k * ldb + g] = d.apwalm[colA[k] + g];
+}
+
 // ---------------------------------------------------------------------------
 // Bt = (aa . A)^T per atom:  Bt[Koff+f][G] = sum_f' A^T[G][f'] aa(f, f')
 // (hamiltonsetup.f90:75-88, zm = aa * apwi).  X holds A^T K-major with
 // leading dimension ldx.  Grid (G tiles, f tiles, atoms).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NTH) void bt_kernel(
-    int ngp, const double2 *__restrict__ X, long long ldx,
+__device__ __forceinline__ void bt_body(
+    int ias, int I0, int F0, int ngp, const double2 *__restrict__ X, long long ldx,
     const int *__restrict__ Koff, const int *__restrict__ Ms,
     const double2 *__restrict__ aa, int maxaa, double2 *__restrict__ bt, int ldb) {
-  const int ias = blockIdx.z;
   const int M = Ms[ias];
-  const int F0 = blockIdx.y * TM;
-  if (F0 >= M) return;
-  const int I0 = blockIdx.x * TM;
+  if (F0 >= M || I0 >= ngp) return;
   const int koff = Koff[ias];
   const double2 *aat = aa + (size_t)ias * maxaa * maxaa;
 
@@ -190,19 +207,34 @@ __global__ __launch_bounds__(NTH) void bt_kernel(
       }
 }
 
+__global__ __launch_bounds__(NTH) void bt_kernel(
+    int ngp, const double2 *__restrict__ X, long long ldx,
+    const int *__restrict__ Koff, const int *__restrict__ Ms,
+    const double2 *__restrict__ aa, int maxaa, double2 *__restrict__ bt, int ldb) {
+  bt_body(blockIdx.z, blockIdx.x * TM, blockIdx.y * TM, ngp, X, ldx, Koff, Ms, aa, maxaa, bt,
+          ldb);
+}
+
+// Grouped: blockIdx.z = ias + natmtot * (k-point).
+__global__ __launch_bounds__(NTH) void bt_grouped_kernel(
+    const KDesc *__restrict__ kd, int natmtot, long long ldx, const int *__restrict__ Koff,
+    const int *__restrict__ Ms, const double2 *__restrict__ aa, int maxaa, int ldb) {
+  const int k = blockIdx.z / natmtot, ias = blockIdx.z - k * natmtot;
+  const KDesc &d = kd[k];
+  bt_body(ias, blockIdx.x * TM, blockIdx.y * TM, d.ngp, d.x, ldx, Koff, Ms, aa, maxaa, d.bt,
+          ldb);
+}
+
 // ---------------------------------------------------------------------------
 // A2t = ((1 + h1) . A)^T: the overlap operand when h1aa is present
 // (overlapsetup.f90:82-94).  h1aa(io1, io2, l, ias) couples the radial
 // orders of one (l, m).  One thread per (G, atom, f2).
 // ---------------------------------------------------------------------------
-__global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apwalm,
-                           const long long *__restrict__ colA,
-                           const int *__restrict__ Koff, const int *__restrict__ Ms,
-                           const int *__restrict__ kinfo, const double *__restrict__ h1aa,
-                           int pmax, int lmaxapw, double2 *__restrict__ a2t, int ldb) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  int ias = blockIdx.z;
-  int f2 = blockIdx.y;
+__device__ __forceinline__ void a2t_body(
+    int g, int ias, int f2, int ngp, const double2 *__restrict__ apwalm,
+    const long long *__restrict__ colA, const int *__restrict__ Koff,
+    const int *__restrict__ Ms, const int *__restrict__ kinfo, const double *__restrict__ h1aa,
+    int pmax, int lmaxapw, double2 *__restrict__ a2t, int ldb) {
   if (g >= ngp || f2 >= Ms[ias]) return;
   int koff = Koff[ias];
   // kinfo[k] = io | p << 4 | l << 8 | (first f of this (l,m)) << 16
@@ -221,6 +253,29 @@ __global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apw
   a2t[(size_t)(koff + f2) * ldb + g] = v;
 }
 
+__global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apwalm,
+                           const long long *__restrict__ colA,
+                           const int *__restrict__ Koff, const int *__restrict__ Ms,
+                           const int *__restrict__ kinfo, const double *__restrict__ h1aa,
+                           int pmax, int lmaxapw, double2 *__restrict__ a2t, int ldb) {
+  (void)natmtot;
+  a2t_body(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.z, blockIdx.y, ngp, apwalm, colA,
+           Koff, Ms, kinfo, h1aa, pmax, lmaxapw, a2t, ldb);
+}
+
+// Grouped: blockIdx.z = ias + natmtot * (k-point).
+__global__ void a2t_grouped_kernel(const KDesc *__restrict__ kd, int natmtot,
+                                   const long long *__restrict__ colA,
+                                   const int *__restrict__ Koff, const int *__restrict__ Ms,
+                                   const int *__restrict__ kinfo,
+                                   const double *__restrict__ h1aa, int pmax, int lmaxapw,
+                                   int ldb) {
+  const int k = blockIdx.z / natmtot, ias = blockIdx.z - k * natmtot;
+  const KDesc &d = kd[k];
+  a2t_body(blockIdx.x * blockDim.x + threadIdx.x, ias, blockIdx.y, d.ngp, d.apwalm, colA, Koff,
+           Ms, kinfo, h1aa, pmax, lmaxapw, d.a2t, ldb);
+}
+
 // ---------------------------------------------------------------------------
 // Fused upper-triangle H and O tiles.
 //   H(i,j) = sum_k conj(X[k][i]) YB[k][j]  + IST_H(i,j)   i <= j < ngp
@@ -229,39 +284,42 @@ __global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apw
 // running over every atom's (l, m, io) (hamiltonsetup.f90:90-103,
 // overlapsetup.f90:95-108).  The epilogue adds the interstitial terms
 // (hmlistln.f90, olpistln.f90) and writes lower := conj(upper)
-// (hamiltonoverlapocopy_UL.f90).  One workgroup per upper tile pair I <= J;
-// tiles are dealt to the 8 XCDs in contiguous ranges so co-resident tiles
-// share column slabs in L2.  K slabs are double-buffered in LDS (one barrier
-// per slab) and prefetched two slabs ahead into registers with clamped,
-// branch-free loads.  LDS rows are padded to 80 doubles so the four k rows
-// read by one MFMA operand fetch alternate LDS bank halves.
+// (hamiltonoverlapocopy_UL.f90).  A workgroup of 32 x 32 waves computes one
+// TMI x TMJ upper tile; tiles are dealt to the 8 XCDs in contiguous ranges
+// so co-resident tiles share column slabs in L2.  K slabs of TK are
+// double-buffered in LDS (one barrier per slab) and prefetched two slabs
+// ahead into registers with clamped, branch-free loads.  LDS rows are padded
+// by 16 doubles (80 or 144) so the four k rows read by one MFMA operand
+// fetch alternate LDS bank halves.
 // ---------------------------------------------------------------------------
-constexpr int LDSP = TM + 16;
+template <int TMI, int TMJ, int TK>
+struct HoShape {
+  static constexpr int WJ = TMJ / 32;               // waves across a tile row
+  static constexpr int NW = (TMI / 32) * WJ;        // waves per workgroup
+  static constexpr int NTH = 64 * NW;
+  static constexpr int PX = TMI + 16, PY = TMJ + 16;  // padded LDS rows
+  static constexpr int RX = NTH / TMI, RY = NTH / TMJ;  // k rows per load pass
+  static constexpr int QX = TK / RX, QY = TK / RY;      // loads per thread
+  static_assert(NTH % TMI == 0 && NTH % TMJ == 0, "tile rows must divide the workgroup");
+  static_assert(TK % RX == 0 && TK % RY == 0 && TK % 4 == 0, "K slab");
+};
 
-__global__ __launch_bounds__(NTH) void ho_kernel(
+// One TMI x TMJ upper tile (I0, J0) of H and O for one k-point.
+template <int TMI, int TMJ, int TK>
+__device__ __forceinline__ void ho_tile(
     int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
-    const double2 *__restrict__ YB, long long ldb,
-    const double2 *__restrict__ YA, long long lda,
-    const int4 *__restrict__ ivgk, const double *__restrict__ vgpc,
-    const IstEntry *__restrict__ T, int lo0, int lo1, int lo2, int n0, int n1,
-    int n2, double2 *__restrict__ H, double2 *__restrict__ O, long long ldh,
-    int accumulate) {
-  // ---- tile decode (XCD-contiguous remap, then J-major pair order) ----
-  const int bid = blockIdx.x, nb = gridDim.x;
-  const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
-  const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
-  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((J + 1) * (J + 2) / 2 <= t) J++;
-  while (J * (J + 1) / 2 > t) J--;
-  const int I = t - J * (J + 1) / 2;
-  const int I0 = I * TM, J0 = J * TM;
-
-  // [buffer][Xr, Xi, Br, Bi, Ar, Ai][k][row]
-  __shared__ double sm[2][6][TK][LDSP];
+    const double2 *__restrict__ YB, long long ldb, const double2 *__restrict__ YA,
+    long long lda, const IstBox &box, double2 *__restrict__ H, double2 *__restrict__ O,
+    long long ldh, int accumulate, int I0, int J0) {
+  using S = HoShape<TMI, TMJ, TK>;
+  // [buffer][re, im][k][row] of X; [buffer][Br, Bi, Ar, Ai][k][row] of YB, YA
+  __shared__ double sX[2][2][TK][S::PX];
+  __shared__ double sY[2][4][TK][S::PY];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wr = w >> 1, wc = w & 1, lr = lane & 15, lk = lane >> 4;
-  const int ii = tid & 63, kq = tid >> 6;
-  const int ri = min(I0 + ii, ngp - 1), rj = min(J0 + ii, ngp - 1);
+  const int wr = w / S::WJ, wc = w % S::WJ, lr = lane & 15, lk = lane >> 4;
+  const int xr_ = tid % TMI, xk = tid / TMI;
+  const int yc = tid % TMJ, yk = tid / TMJ;
+  const int ri = min(I0 + xr_, ngp - 1), rj = min(J0 + yc, ngp - 1);
   const double2 *xp = X + ri;
   const double2 *bp = YB + rj;
   const double2 *ap = YA + rj;
@@ -280,32 +338,42 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   // Rows at or past ngp only feed output rows/columns the epilogue never
   // writes, so addresses are clamped but values are not masked; the K tail
   // is zeroed by a 0/1 factor so that every load is unconditional.
-  double2 px[4], pb[4], pa[4];
+  double2 px[S::QX], pb[S::QY], pa[S::QY];
   auto load = [&](int kb) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = kb + kq + 4 * q;
+    for (int q = 0; q < S::QX; q++) {
+      const int k = kb + xk + S::RX * q;
       const double f = k < Ktot ? 1.0 : 0.0;
       const size_t kc = k < Ktot ? k : Ktot - 1;
       const double2 x = xp[kc * ldx];
+      px[q] = z2(f * x.x, f * x.y);
+    }
+#pragma unroll
+    for (int q = 0; q < S::QY; q++) {
+      const int k = kb + yk + S::RY * q;
+      const double f = k < Ktot ? 1.0 : 0.0;
+      const size_t kc = k < Ktot ? k : Ktot - 1;
       const double2 b = bp[kc * ldb];
       const double2 a = ap[kc * lda];
-      px[q] = z2(f * x.x, f * x.y);
       pb[q] = z2(f * b.x, f * b.y);
       pa[q] = z2(f * a.x, f * a.y);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int kk = kq + 4 * q;
+    for (int q = 0; q < S::QX; q++) {
+      const int kk = xk + S::RX * q;
+      sX[buf][0][kk][xr_] = px[q].x;
+      sX[buf][1][kk][xr_] = px[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < S::QY; q++) {
+      const int kk = yk + S::RY * q;
       const double2 a = pa[q];
-      sm[buf][0][kk][ii] = px[q].x;
-      sm[buf][1][kk][ii] = px[q].y;
-      sm[buf][2][kk][ii] = pb[q].x;
-      sm[buf][3][kk][ii] = pb[q].y;
-      sm[buf][4][kk][ii] = a.x;
-      sm[buf][5][kk][ii] = a.y;
+      sY[buf][0][kk][yc] = pb[q].x;
+      sY[buf][1][kk][yc] = pb[q].y;
+      sY[buf][2][kk][yc] = a.x;
+      sY[buf][3][kk][yc] = a.y;
     }
   };
   auto compute = [&](int buf) {
@@ -315,16 +383,16 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
       double xr[2], xi[2], nxi[2], br[2], bi[2], ar[2], ai[2];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
-        xr[a] = sm[buf][0][kk][wr * 32 + a * 16 + lr];
-        xi[a] = sm[buf][1][kk][wr * 32 + a * 16 + lr];
+        xr[a] = sX[buf][0][kk][wr * 32 + a * 16 + lr];
+        xi[a] = sX[buf][1][kk][wr * 32 + a * 16 + lr];
         nxi[a] = -xi[a];
       }
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        br[b] = sm[buf][2][kk][wc * 32 + b * 16 + lr];
-        bi[b] = sm[buf][3][kk][wc * 32 + b * 16 + lr];
-        ar[b] = sm[buf][4][kk][wc * 32 + b * 16 + lr];
-        ai[b] = sm[buf][5][kk][wc * 32 + b * 16 + lr];
+        br[b] = sY[buf][0][kk][wc * 32 + b * 16 + lr];
+        bi[b] = sY[buf][1][kk][wc * 32 + b * 16 + lr];
+        ar[b] = sY[buf][2][kk][wc * 32 + b * 16 + lr];
+        ai[b] = sY[buf][3][kk][wc * 32 + b * 16 + lr];
       }
       // conj(x) * y = (xr yr + xi yi) + i (xr yi - xi yr)
 #pragma unroll
@@ -349,7 +417,9 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   };
 
   // Loads past the last slab are clamped and zero-weighted, so the loop body
-  // is one straight-line block (no per-iteration conditions).
+  // is one straight-line block (no per-iteration conditions).  The closing
+  // barrier of the last slab also frees the LDS for the next tile of a
+  // persistent workgroup.
   const int nslab = (Ktot + TK - 1) / TK;
   load(0);
   store(0);
@@ -365,7 +435,6 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   // ---- epilogue: accumulate, interstitial gather, write upper + mirror ----
   // (accumulate adds the old upper value before the interstitial term, the
   // reference order: zgemm beta = 1, then hmlistln/olpistln)
-  IstBox box{ivgk, vgpc, T, lo0, lo1, lo2, n0, n1, n2};
 #pragma unroll
   for (int b = 0; b < 2; b++) {
     const int j = J0 + wc * 32 + b * 16 + lr;
@@ -392,27 +461,92 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   }
 }
 
+// Single k-point: one workgroup per upper 64 x 64 tile pair I <= J, dealt to
+// the 8 XCDs in contiguous ranges, J-major within each range.
+__global__ __launch_bounds__(NTH) void ho_kernel(
+    int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
+    const double2 *__restrict__ YB, long long ldb,
+    const double2 *__restrict__ YA, long long lda,
+    const int4 *__restrict__ ivgk, const double *__restrict__ vgpc,
+    const IstEntry *__restrict__ T, int lo0, int lo1, int lo2, int n0, int n1,
+    int n2, double2 *__restrict__ H, double2 *__restrict__ O, long long ldh,
+    int accumulate) {
+  const int bid = blockIdx.x, nb = gridDim.x;
+  const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
+  const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
+  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((J + 1) * (J + 2) / 2 <= t) J++;
+  while (J * (J + 1) / 2 > t) J--;
+  const int I = t - J * (J + 1) / 2;
+  IstBox box{ivgk, vgpc, T, lo0, lo1, lo2, n0, n1, n2};
+  ho_tile<TM, TM, TK>(ngp, Ktot, X, ldx, YB, ldb, YA, lda, box, H, O, ldh, accumulate, I * TM,
+                      J * TM);
+}
+
+// Grouped over k-points: a list of (k, I0, J0) tiles (k < 0 = padding) is
+// walked by gridDim.x workgroups, workgroup b taking entries b, b + gridDim.x,
+// ...  With gridDim.x a multiple of 8, list position t runs on XCD t % 8, so
+// the host orders the list to give each XCD contiguous tile ranges.  Every
+// tile of one shape accumulates K in the same order whatever the list order
+// or grid, so all plans of a shape give bitwise identical results.
+template <int TMI, int TMJ, int TK>
+__global__ __launch_bounds__(64 * (TMI / 32) * (TMJ / 32)) void ho_grouped_kernel(
+    const KDesc *__restrict__ kd, const int4 *__restrict__ tiles, int ntiles, int Ktot,
+    long long ldx, long long ldb, int use_a2t, const IstEntry *__restrict__ T, int lo0,
+    int lo1, int lo2, int n0, int n1, int n2, int accumulate) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int4 e = tiles[t];
+    const int k = __builtin_amdgcn_readfirstlane(e.x);
+    if (k < 0) continue;
+    const int I0 = __builtin_amdgcn_readfirstlane(e.y);
+    const int J0 = __builtin_amdgcn_readfirstlane(e.z);
+    const KDesc &d = kd[k];
+    IstBox box{d.ivgk, d.vgpc, T, lo0, lo1, lo2, n0, n1, n2};
+    const double2 *YA = use_a2t ? (const double2 *)d.a2t : d.x;
+    const long long lda = use_a2t ? ldb : ldx;
+    ho_tile<TMI, TMJ, TK>(d.ngp, Ktot, d.x, ldx, d.bt, ldb, YA, lda, box, d.h, d.o, d.ldh,
+                          accumulate, I0, J0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // APW-lo border (hamiltonsetup.f90:124-151, overlapsetup.f90:116-133).
 // One thread per (G, atom):  H(lo rows, G) = loa . A   (+ mirror),
 //                            O(G, lo cols) = sum_io conj(A_io,lm s_io) (+ mirror)
 // with s = oalo + h1loa.
 // ---------------------------------------------------------------------------
+struct LoTables {
+  const long long *colA;
+  const int *Koff, *Ms, *atom_species, *losize, *lobase;
+  const double2 *loa;
+  int maxnlo, maxaa;
+  const int *nlorb, *lorbl;
+  int nlomax;
+  const int *apword;
+  int lmaxapw, pmax, lmmax, ngkmax;
+  const double *salo;
+};
+
 template <int MAXLO>
-__global__ __launch_bounds__(64) void lo_border_kernel(
-    int ngp, const double2 *__restrict__ apwalm,
-    const long long *__restrict__ colA, const int *__restrict__ Koff,
-    const int *__restrict__ Ms, const int *__restrict__ atom_species,
-    const int *__restrict__ losize, const int *__restrict__ lobase,
-    const double2 *__restrict__ loa, int maxnlo, int maxaa,
-    const int *__restrict__ nlorb, const int *__restrict__ lorbl, int nlomax,
-    const int *__restrict__ apword, int lmaxapw, int pmax, int lmmax, int ngkmax,
-    const double *__restrict__ salo, double2 *__restrict__ H,
-    double2 *__restrict__ O, long long ldh, int accumulate) {
-  const int ias = blockIdx.y;
-  const int is = atom_species[ias];
-  const int los = losize[is];
-  const int g = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void lo_border_body(int g, int ias, int ngp,
+                                               const double2 *__restrict__ apwalm,
+                                               const LoTables &t, double2 *__restrict__ H,
+                                               double2 *__restrict__ O, long long ldh,
+                                               int accumulate) {
+  const long long *__restrict__ colA = t.colA;
+  const int *__restrict__ nlorb = t.nlorb;
+  const int *__restrict__ lorbl = t.lorbl;
+  const int *__restrict__ apword = t.apword;
+  const double *__restrict__ salo = t.salo;
+  const int maxnlo = t.maxnlo, nlomax = t.nlomax, lmaxapw = t.lmaxapw, pmax = t.pmax,
+            lmmax = t.lmmax, ngkmax = t.ngkmax;
+  const int *__restrict__ Koff = t.Koff;
+  const int *__restrict__ Ms = t.Ms;
+  const int *__restrict__ lobase = t.lobase;
+  const double2 *__restrict__ loa = t.loa;
+  const int maxaa = t.maxaa;
+  const int is = t.atom_species[ias];
+  const int los = t.losize[is];
   if (los == 0 || g >= ngp) return;
   const int r0 = ngp + lobase[ias];
   const long long *col = colA + Koff[ias];
@@ -467,19 +601,36 @@ __global__ __launch_bounds__(64) void lo_border_kernel(
   }
 }
 
+template <int MAXLO>
+__global__ __launch_bounds__(64) void lo_border_kernel(int ngp, const double2 *__restrict__ apwalm,
+                                                       LoTables t, double2 *__restrict__ H,
+                                                       double2 *__restrict__ O, long long ldh,
+                                                       int accumulate) {
+  lo_border_body<MAXLO>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, ngp, apwalm, t, H, O, ldh,
+                        accumulate);
+}
+
+// Grouped: blockIdx.z selects the k-point.
+template <int MAXLO>
+__global__ __launch_bounds__(64) void lo_border_grouped_kernel(const KDesc *__restrict__ kd,
+                                                               LoTables t, int accumulate) {
+  const KDesc &d = kd[blockIdx.z];
+  lo_border_body<MAXLO>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, d.apwalm, t, d.h,
+                        d.o, d.ldh, accumulate);
+}
+
 // ---------------------------------------------------------------------------
 // lo-lo block (hamiltonsetup.f90:153-161, overlapsetup.f90:134-151) over the
 // whole nlotot x nlotot corner: H += lolo (same atom), O += ololo + h1lolo
 // (same atom, same l, same m).  Upper threads write both triangles.
 // ---------------------------------------------------------------------------
-__global__ void lolo_kernel(int ngp, int nlotot, const LoRow *__restrict__ lorow,
-                            const int *__restrict__ lobase,
-                            const double2 *__restrict__ lolo, int maxnlo,
-                            const double *__restrict__ ololo_eff, int nlomax,
-                            double2 *__restrict__ H, double2 *__restrict__ O,
-                            long long ldh, int accumulate) {
-  const int r = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int c = blockIdx.y * 16 + (threadIdx.x >> 4);
+__device__ __forceinline__ void lolo_body(int r, int c, int ngp, int nlotot,
+                                          const LoRow *__restrict__ lorow,
+                                          const int *__restrict__ lobase,
+                                          const double2 *__restrict__ lolo, int maxnlo,
+                                          const double *__restrict__ ololo_eff, int nlomax,
+                                          double2 *__restrict__ H, double2 *__restrict__ O,
+                                          long long ldh, int accumulate) {
   if (r >= nlotot || c >= nlotot || r > c) return;
   const LoRow P = lorow[r], Q = lorow[c];
   const size_t up = (size_t)(ngp + r) + (size_t)(ngp + c) * ldh;
@@ -503,6 +654,28 @@ __global__ void lolo_kernel(int ngp, int nlotot, const LoRow *__restrict__ lorow
     H[lo] = z2(hv.x, -hv.y);
     O[lo] = z2(ov.x, -ov.y);
   }
+}
+
+__global__ void lolo_kernel(int ngp, int nlotot, const LoRow *__restrict__ lorow,
+                            const int *__restrict__ lobase,
+                            const double2 *__restrict__ lolo, int maxnlo,
+                            const double *__restrict__ ololo_eff, int nlomax,
+                            double2 *__restrict__ H, double2 *__restrict__ O,
+                            long long ldh, int accumulate) {
+  lolo_body(blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4), ngp,
+            nlotot, lorow, lobase, lolo, maxnlo, ololo_eff, nlomax, H, O, ldh, accumulate);
+}
+
+__global__ void lolo_grouped_kernel(const KDesc *__restrict__ kd, int nlotot,
+                                    const LoRow *__restrict__ lorow,
+                                    const int *__restrict__ lobase,
+                                    const double2 *__restrict__ lolo, int maxnlo,
+                                    const double *__restrict__ ololo_eff, int nlomax,
+                                    int accumulate) {
+  const KDesc &d = kd[blockIdx.z];
+  lolo_body(blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4), d.ngp,
+            nlotot, lorow, lobase, lolo, maxnlo, ololo_eff, nlomax, d.h, d.o, d.ldh,
+            accumulate);
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
@@ -616,22 +789,39 @@ hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
   return hipGetLastError();
 }
 
+static LoTables lo_tables(const exh_context *c) {
+  const int *ki = (const int *)c->katom.p;
+  const int *lo = (const int *)c->lobase_d.p;  // [lobase(natmtot), losize(nspecies), nlorb, lorbl]
+  LoTables t;
+  t.colA = (const long long *)c->colA.p;
+  t.Koff = ki;
+  t.Ms = ki + c->natmtot;
+  t.atom_species = (const int *)c->atom_species_d.p;
+  t.lobase = lo;
+  t.losize = lo + c->natmtot;
+  t.nlorb = t.losize + c->nspecies;
+  t.lorbl = t.nlorb + c->nspecies;
+  t.loa = (const double2 *)c->loa.p;
+  t.maxnlo = c->maxnlo;
+  t.maxaa = c->maxaa;
+  t.nlomax = c->nlomax;
+  t.apword = (const int *)c->apword_d.p;
+  t.lmaxapw = c->lmaxapw;
+  t.pmax = c->apwordmax;
+  t.lmmax = c->lmmax;
+  t.ngkmax = c->ngkmax;
+  t.salo = (const double *)c->salo.p;
+  return t;
+}
+
 hipError_t launch_lo_border(hipStream_t s, const exh_context *c, int ngp,
                             const double2 *apwalm, double2 *h, double2 *o,
                             int ldh, int flags) {
   if (ngp <= 0 || c->nlotot == 0) return hipSuccess;
   dim3 grid(cdiv(ngp, 64), c->natmtot);
-  const int *ki = (const int *)c->katom.p;
-  const int *lo = (const int *)c->lobase_d.p;  // [lobase(natmtot), losize(nspecies), nlorb, lorbl]
-  const int *lobase = lo, *losz = lo + c->natmtot, *nlorb = losz + c->nspecies,
-            *lorbl = nlorb + c->nspecies;
+  const LoTables t = lo_tables(c);
 #define EXH_LOB(N)                                                                     \
-  hipLaunchKernelGGL(lo_border_kernel<N>, grid, dim3(64), 0, s, ngp, apwalm,           \
-                     (const long long *)c->colA.p, ki, ki + c->natmtot,                \
-                     (const int *)c->atom_species_d.p, losz, lobase,                   \
-                     (const double2 *)c->loa.p, c->maxnlo, c->maxaa, nlorb, lorbl,     \
-                     c->nlomax, (const int *)c->apword_d.p, c->lmaxapw, c->apwordmax,  \
-                     c->lmmax, c->ngkmax, (const double *)c->salo.p, h, o,             \
+  hipLaunchKernelGGL(lo_border_kernel<N>, grid, dim3(64), 0, s, ngp, apwalm, t, h, o,  \
                      (long long)ldh, flags & EXH_ACCUMULATE)
   if (c->maxnlo <= 16) EXH_LOB(16);
   else if (c->maxnlo <= 32) EXH_LOB(32);
@@ -649,6 +839,122 @@ hipError_t launch_lolo(hipStream_t s, const exh_context *c, int ngp, double2 *h,
                      (const double2 *)c->lolo.p, c->maxnlo,
                      (const double *)c->ololo_eff.p, c->nlomax, h, o, (long long)ldh,
                      flags & EXH_ACCUMULATE);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launchers (batched k-points)
+// ---------------------------------------------------------------------------
+const HoShapeInfo kHoShapeInfo[kHoShapes] = {
+    {64, 64, 16}, {64, 64, 8}, {128, 64, 16}, {128, 64, 8}, {64, 128, 8}};
+
+hipError_t launch_ivgk_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                               int ngpmax) {
+  if (nk <= 0 || ngpmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ivgk_grouped_kernel, dim3(cdiv(ngpmax, 256), nk), dim3(256), 0, s, kd,
+                     (const int *)c->ivg.p, c->ngrtot);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                               int ngpmax, int ldb) {
+  if (nk <= 0 || ngpmax <= 0 || c->contig || c->Ktot == 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_grouped_kernel, dim3(cdiv(ngpmax, 256), c->Ktot, nk), dim3(256), 0, s,
+                     kd, (const long long *)c->colA.p, ldb);
+  return hipGetLastError();
+}
+
+hipError_t launch_bt_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                             int ngpmax, long long ldx, int ldb) {
+  if (nk <= 0 || ngpmax <= 0 || c->natmtot <= 0) return hipSuccess;
+  dim3 grid(cdiv(ngpmax, TM), cdiv(c->maxaa, TM), c->natmtot * nk);
+  hipLaunchKernelGGL(bt_grouped_kernel, grid, dim3(NTH), 0, s, kd, c->natmtot, ldx,
+                     (const int *)c->katom.p, (const int *)c->katom.p + c->natmtot,
+                     (const double2 *)c->aa.p, c->maxaa, ldb);
+  return hipGetLastError();
+}
+
+hipError_t launch_a2t_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                              int ngpmax, int ldb) {
+  if (nk <= 0 || ngpmax <= 0 || c->natmtot <= 0) return hipSuccess;
+  dim3 grid(cdiv(ngpmax, 256), c->maxaa, c->natmtot * nk);
+  hipLaunchKernelGGL(a2t_grouped_kernel, grid, dim3(256), 0, s, kd, c->natmtot,
+                     (const long long *)c->colA.p, (const int *)c->katom.p,
+                     (const int *)c->katom.p + c->natmtot,
+                     (const int *)c->katom.p + 2 * c->natmtot, (const double *)c->h1aa.p,
+                     c->apwordmax, c->lmaxapw, ldb);
+  return hipGetLastError();
+}
+
+template <int TMI, int TMJ, int TK>
+static hipError_t ho_grouped_launch(hipStream_t s, const exh_context *c, const KDesc *kd,
+                                    const int4 *tiles, int ntiles, int nwg, long long ldx,
+                                    int ldb, int accumulate) {
+  const int lo0 = c->intgv[0], lo1 = c->intgv[1], lo2 = c->intgv[2];
+  const int n0 = c->intgv[3] - lo0 + 1, n1 = c->intgv[4] - lo1 + 1, n2 = c->intgv[5] - lo2 + 1;
+  hipLaunchKernelGGL((ho_grouped_kernel<TMI, TMJ, TK>), dim3(nwg),
+                     dim3(HoShape<TMI, TMJ, TK>::NTH), 0, s, kd, tiles, ntiles, c->Ktot, ldx,
+                     (long long)ldb, c->have_h1aa ? 1 : 0, (const IstEntry *)c->ist.p, lo0,
+                     lo1, lo2, n0, n1, n2, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_ho_grouped(hipStream_t s, const exh_context *c, int shape, const KDesc *kd,
+                             const int4 *tiles, int ntiles, int nwg, long long ldx, int ldb,
+                             int accumulate) {
+  if (ntiles <= 0 || nwg <= 0) return hipSuccess;
+  switch (shape) {
+    case 0: return ho_grouped_launch<64, 64, 16>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
+    case 1: return ho_grouped_launch<64, 64, 8>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
+    case 2: return ho_grouped_launch<128, 64, 16>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
+    case 3: return ho_grouped_launch<128, 64, 8>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
+    case 4: return ho_grouped_launch<64, 128, 8>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Workgroups of one shape that fit on a CU at once (LDS, registers, waves).
+int ho_grouped_occupancy(int shape) {
+  int n = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (shape) {
+#define EXH_OCC(I, J, K)                                                                    \
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ho_grouped_kernel<I, J, K>,           \
+                                                   HoShape<I, J, K>::NTH, 0);               \
+  break;
+    case 0: EXH_OCC(64, 64, 16)
+    case 1: EXH_OCC(64, 64, 8)
+    case 2: EXH_OCC(128, 64, 16)
+    case 3: EXH_OCC(128, 64, 8)
+    case 4: EXH_OCC(64, 128, 8)
+#undef EXH_OCC
+    default: break;
+  }
+  return e == hipSuccess && n > 0 ? n : 1;
+}
+
+hipError_t launch_lo_border_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                                    int ngpmax, int accumulate) {
+  if (nk <= 0 || ngpmax <= 0 || c->nlotot == 0) return hipSuccess;
+  dim3 grid(cdiv(ngpmax, 64), c->natmtot, nk);
+  const LoTables t = lo_tables(c);
+  if (c->maxnlo <= 16)
+    hipLaunchKernelGGL(lo_border_grouped_kernel<16>, grid, dim3(64), 0, s, kd, t, accumulate);
+  else if (c->maxnlo <= 32)
+    hipLaunchKernelGGL(lo_border_grouped_kernel<32>, grid, dim3(64), 0, s, kd, t, accumulate);
+  else
+    hipLaunchKernelGGL(lo_border_grouped_kernel<64>, grid, dim3(64), 0, s, kd, t, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_lolo_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+                               int accumulate) {
+  if (nk <= 0 || c->nlotot == 0) return hipSuccess;
+  dim3 grid(cdiv(c->nlotot, 16), cdiv(c->nlotot, 16), nk);
+  hipLaunchKernelGGL(lolo_grouped_kernel, grid, dim3(256), 0, s, kd, c->nlotot,
+                     (const LoRow *)c->lorow.p, (const int *)c->lobase_d.p,
+                     (const double2 *)c->lolo.p, c->maxnlo, (const double *)c->ololo_eff.p,
+                     c->nlomax, accumulate);
   return hipGetLastError();
 }
 

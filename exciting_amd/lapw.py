@@ -76,8 +76,21 @@ def lib():
     L.exh_profile_enable.argtypes = [_p, _i]
     L.exh_profile_read.argtypes = [_p, _i, C.POINTER(_d), C.POINTER(_i)]
     L.exh_profile_reset.argtypes = [_p]
+    L.exh_setup_batch_device.argtypes = [_p, _p, _i, _p, _i]
+    L.exh_batch_plan_count.argtypes = []
+    L.exh_batch_plan_info.argtypes = [_i] + [C.POINTER(_i)] * 5
+    L.exh_set_batch_plan.argtypes = [_p, _i]
+    L.exh_get_batch_plan.argtypes = [_p]
+    L.exh_tune_batch.argtypes = [_p, _p, _i, _p, _i, C.POINTER(_i)]
+    L.exh_set_batch_workspace.argtypes = [_p, C.c_size_t]
     _lib = L
     return L
+
+
+class KPoint(C.Structure):
+    """exh_kpoint: one k-point of a batch (device pointers)."""
+    _fields_ = [("ngp", _i), ("rank", _i), ("gpc", _p), ("tpgpc", _p), ("sfacgp", _p),
+                ("apwalm", _p), ("igpig", _p), ("vgpc", _p), ("h", _p), ("o", _p)]
 
 
 EXPORTS = (
@@ -85,11 +98,31 @@ EXPORTS = (
     "exh_set_radial", "exh_set_mt", "exh_set_gvectors", "exh_set_interstitial",
     "exh_match", "exh_match_device", "exh_hamiltonandoverlapsetup",
     "exh_hamiltonandoverlapsetup_ex", "exh_setup_device", "exh_profile_enable",
-    "exh_profile_read", "exh_profile_reset",
+    "exh_profile_read", "exh_profile_reset", "exh_setup_batch_device",
+    "exh_batch_plan_count", "exh_batch_plan_info", "exh_set_batch_plan",
+    "exh_get_batch_plan", "exh_tune_batch", "exh_set_batch_workspace",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
-           "oz_gemm", "oz_combine")
+           "oz_gemm", "oz_combine", "ho_grouped", "batch_graph")
+
+MATCH = 2   # batch: run match first
+GRAPH = 4   # batch: capture in a HIP graph, replay on identical calls
+
+
+def plan_count() -> int:
+    return lib().exh_batch_plan_count()
+
+
+def plan_info(plan: int) -> dict:
+    """Tile rows, columns, K slab, persistence and XCD ordering of a batch
+    plan of the grouped H/O kernel."""
+    v = [_i() for _ in range(5)]
+    rc = lib().exh_batch_plan_info(plan, *[C.byref(x) for x in v])
+    if rc != 0:
+        raise ExhError(f"no batch plan {plan}")
+    return dict(zip(("tile_rows", "tile_cols", "k_slab", "persistent", "xcd_order"),
+                    (x.value for x in v)))
 
 
 def SPLIT(nmod: int) -> int:
@@ -294,6 +327,48 @@ class Context:
                                                          _ptr(o), flags),
                     "exh_hamiltonandoverlapsetup")
         return system
+
+    # ---- batched k-points -------------------------------------------------
+    @staticmethod
+    def _kpoint_array(kpts):
+        arr = (KPoint * len(kpts))(*kpts)
+        return arr
+
+    def setup_batch(self, kpts, flags: int = ACCUMULATE, stream=None):
+        """Grouped match (with MATCH) + H/O setup over a list of KPoint
+        structures (device pointers; see ``kpoint_struct``)."""
+        arr = self._kpoint_array(kpts)
+        self._check(lib().exh_setup_batch_device(self._h, stream, len(kpts), arr, flags),
+                    "exh_setup_batch_device")
+        return arr
+
+    def tune_batch(self, kpts, flags: int = 0, stream=None) -> int:
+        """Time every batch plan on these k-points and keep the fastest.  H
+        and O are scratch; on return they hold the non-accumulated result."""
+        arr = self._kpoint_array(kpts)
+        best = _i(-1)
+        self._check(lib().exh_tune_batch(self._h, stream, len(kpts), arr, flags,
+                                         C.byref(best)), "exh_tune_batch")
+        return best.value
+
+    @property
+    def batch_plan(self) -> int:
+        return lib().exh_get_batch_plan(self._h)
+
+    @batch_plan.setter
+    def batch_plan(self, plan: int):
+        self._check(lib().exh_set_batch_plan(self._h, plan), "exh_set_batch_plan")
+
+    def set_batch_workspace(self, nbytes: int):
+        self._check(lib().exh_set_batch_workspace(self._h, int(nbytes)),
+                    "exh_set_batch_workspace")
+
+
+def kpoint_struct(d: "DeviceKPoint", h, o, apwalm=None) -> KPoint:
+    """exh_kpoint for a DeviceKPoint and its H, O tensors."""
+    a = d.apwalm if apwalm is None else apwalm
+    return KPoint(d.ngk, d.rank, _ptr(d.gpc), _ptr(d.tpgpc), _ptr(d.sfacgp), _ptr(a),
+                  _ptr(d.igpig), _ptr(d.vgpc), _ptr(h), _ptr(o))
 
 
 class DeviceKPoint:

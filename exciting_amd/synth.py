@@ -151,8 +151,10 @@ def serving_cells(kmax: int = 16):
 
 def tiny(seed: int = 7, lmax: int = 3, nlo_l: Sequence[int] = (0, 1),
          rgkmax: float = 4.0, natoms: Sequence[int] = (1, 1), apword: int = 2,
-         lmaxmat: Optional[int] = None) -> CellSpec:
-    """A small cell for brute-force checks (few dozen G+k vectors)."""
+         lmaxmat: Optional[int] = None, nk: int = 1) -> CellSpec:
+    """A small cell for brute-force checks (few dozen G+k vectors); with
+    nk > 1 the extra k-points are seeded random points of the zone, so the
+    G+k sets have uneven sizes."""
     rng = np.random.default_rng(seed)
     avec = np.array([[7.0, 0.3, 0.1], [0.2, 7.5, 0.4], [0.0, 0.5, 8.0]]).T
     pos = _random_positions(rng, avec, sum(natoms), 3.8)
@@ -164,7 +166,9 @@ def tiny(seed: int = 7, lmax: int = 3, nlo_l: Sequence[int] = (0, 1),
         name=f"tiny{seed}", avec=avec, positions=sp, rmt=[1.8] * len(natoms),
         lmaxapw=lmax, lmaxmat=lmax if lmaxmat is None else lmaxmat,
         apword=apword, lo_l=[list(nlo_l)] * len(natoms), rgkmax=rgkmax,
-        gmaxvr=2.0 * rgkmax / 1.8 + 1.0, kpoints=np.array([[0.13, -0.21, 0.37]]),
+        gmaxvr=2.0 * rgkmax / 1.8 + 1.0,
+        kpoints=np.concatenate([np.array([[0.13, -0.21, 0.37]]),
+                                rng.uniform(-0.5, 0.5, (nk - 1, 3))]),
         nrmt=120, seed=seed)
 
 

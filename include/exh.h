@@ -29,6 +29,7 @@
 #define EXH_H
 
 #include <complex.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -150,6 +151,66 @@ int exh_setup_device(exh_context *ctx, void *stream, int ngp, int rank,
                      const exh_z *apwalm, const int *igpig, const double *vgpc,
                      exh_z *h, exh_z *o, int flags);
 
+/* ---- batched k-points ---------------------------------------------------- */
+/* No reference counterpart as one call: the reference runs match and
+ * hamiltonandoverlapsetup once per k-point inside the k loop of the SCF cycle
+ * (scf_cycle.f90:256-296 -> seceqn.f90:57 match, seceqnfv.f90:76-77 setup).
+ * A batch takes the whole list of k-points of one structure (uneven ngp) and
+ * runs every stage as one grouped launch over all of them; the H/O
+ * contraction is one persistent launch over the upper tiles of every
+ * k-point.  Per k-point the result is bitwise identical to
+ * exh_match_device + exh_setup_device with the same flags.  Device pointers
+ * throughout; rank = ngp + nlotot is the order and leading dimension of h and
+ * o.  gpc, tpgpc and sfacgp are read only with EXH_MATCH. */
+typedef struct exh_kpoint {
+  int ngp;
+  int rank;
+  const double *gpc;     /* gpc(ngkmax)                     (EXH_MATCH) */
+  const double *tpgpc;   /* tpgpc(2,ngkmax)                 (EXH_MATCH) */
+  const exh_z *sfacgp;   /* sfacgp(ngkmax,natmtot)          (EXH_MATCH) */
+  exh_z *apwalm;         /* apwalm(ngkmax,apwordmax,lmmaxapw,natmtot)   */
+  const int *igpig;      /* igpig(ngkmax), 1-based                       */
+  const double *vgpc;    /* vgpc(3,ngkmax)                               */
+  exh_z *h;              /* h(rank,rank)                                 */
+  exh_z *o;              /* o(rank,rank)                                 */
+} exh_kpoint;
+enum {
+  /* Batch: compute apwalm with match first (grouped), as seceqn does before
+   * the setup. */
+  EXH_MATCH = 2,
+  /* Batch: capture the launches in a HIP graph on the first call and replay
+   * it on later calls with identical arguments (same pointers, sizes, flags,
+   * plan and context state).  Tables changed through exh_set_* are read by
+   * the replay; a set_* call that needs larger device buffers, or a new plan,
+   * rebuilds the graph. */
+  EXH_GRAPH = 4
+};
+/* Flags: EXH_ACCUMULATE, EXH_SPLIT(n), EXH_MATCH, EXH_GRAPH.  The per-k
+ * scratch of one grouped launch is bounded by the batch workspace limit;
+ * larger batches run as consecutive groups. */
+int exh_setup_batch_device(exh_context *ctx, void *stream, int nk, const exh_kpoint *kpts,
+                           int flags);
+/* Plans of the grouped H/O kernel: tile shape (rows x columns x K slab),
+ * persistent grid (one workgroup per resident slot on every CU, walking the
+ * tile list) or one workgroup per tile, and tile order (k-major, or split
+ * into 8 contiguous ranges, one per XCD).  All plans give bitwise identical
+ * results. */
+int exh_batch_plan_count(void);
+int exh_batch_plan_info(int plan, int *tile_rows, int *tile_cols, int *k_slab,
+                        int *persistent, int *xcd_order);
+/* Plan used by exh_setup_batch_device (-1 = the default, plan 0 tuned for
+ * the 64-atom workload). */
+int exh_set_batch_plan(exh_context *ctx, int plan);
+int exh_get_batch_plan(exh_context *ctx);
+/* Autotuner: times every plan on (the first few of) these k-points, with the
+ * same flags minus EXH_ACCUMULATE and EXH_GRAPH, keeps the fastest as the
+ * context's batch plan and returns it in *plan.  h and o are used as scratch:
+ * on return they hold the non-accumulated result. */
+int exh_tune_batch(exh_context *ctx, void *stream, int nk, const exh_kpoint *kpts, int flags,
+                   int *plan);
+/* Upper bound on the batch's per-k-point scratch in bytes (default 8 GiB,
+ * at least one k-point's worth is always used). */
+int exh_set_batch_workspace(exh_context *ctx, size_t bytes);
 /* ---- instrumentation ----------------------------------------------------- */
 /* No reference counterpart (the reference accumulates wall-clock timers such
  * as timemat, hamiltonsetup.f90:190).  When enabled, every hot-path kernel
@@ -168,7 +229,9 @@ enum {
   EXH_K_OZRES = 8,    /* split: column exponents + int8 residues        */
   EXH_K_OZGEMM = 9,   /* split: modular int8 GEMMs (MFMA)               */
   EXH_K_OZCOMB = 10,  /* split: Garner/CRT + interstitial + mirror      */
-  EXH_K_COUNT = 11
+  EXH_K_HOGRP = 11,   /* batch: grouped persistent H/O contraction      */
+  EXH_K_GRAPH = 12,   /* batch: one replay of a captured batch graph    */
+  EXH_K_COUNT = 13
 };
 int exh_profile_enable(exh_context *ctx, int on);
 int exh_profile_read(exh_context *ctx, int kernel, double *total_ms, int *launches);

@@ -18,6 +18,16 @@ def system(name: str):
         return synth.add_relativity(synth.build(synth.tiny(seed=11)), seed=3, iora=True)
     if name == "tiny_zora":
         return synth.add_relativity(synth.build(synth.tiny(seed=12)), seed=4, iora=False)
+    if name == "tiny_multi":
+        return synth.build(synth.tiny(seed=13, nk=5))
+    if name == "tiny_lmaxmat_multi":
+        return synth.build(synth.tiny(seed=14, lmax=4, lmaxmat=2, natoms=(2, 1), nk=3))
+    if name == "tiny_rel_multi":
+        return synth.add_relativity(synth.build(synth.tiny(seed=15, nk=3)), seed=6, iora=True)
+    if name == "sc8_4k":
+        return synth.build(synth.config(1), kpoints=[0, 3, 5, 6])
+    if name == "ortho16_2k":
+        return synth.build(synth.config(2), kpoints=[0, 77])
     if name == "fcc2":
         return synth.build(synth.config(0))
     if name == "sc8_k0":

@@ -35,3 +35,21 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(lapw, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(lapw.ExhError, match="not built"):
         lapw.lib()
+
+
+def test_batch_plan_table():
+    """Every plan names one of the compiled tile shapes; plans enumerate
+    shape x {tile grid, persistent} x {k-major, XCD order} exactly once."""
+    n = lapw.plan_count()
+    seen = set()
+    for p in range(n):
+        info = lapw.plan_info(p)
+        assert info["tile_rows"] in (64, 128) and info["tile_cols"] in (64, 128)
+        assert info["k_slab"] in (8, 16)
+        assert info["persistent"] in (0, 1) and info["xcd_order"] in (0, 1)
+        seen.add(tuple(info.values()))
+    assert len(seen) == n
+    with pytest.raises(lapw.ExhError):
+        lapw.plan_info(n)
+    with pytest.raises(lapw.ExhError):
+        lapw.plan_info(-1)
