@@ -3,9 +3,21 @@
 
 A step is one pass of the hot path (match + Hamiltonian/overlap setup) over
 every k-point of the workload, with all per-k inputs already resident in HBM.
-The default workload is the 64-atom simple-cubic supercell (a = 20.52 bohr,
+The headline workload is the 64-atom simple-cubic supercell (a = 20.52 bohr,
 lmaxapw = 10, APW order 2 + one local orbital per l <= 3, rgkmax = 8, so
-ngk ~ 9340 and n ~ 10364) with 16 k-points, computed in FP64.
+ngk ~ 9340 and n ~ 10364) with 16 k-points.  It is run three ways:
+
+  * fp64_loop: the single-k entry points called once per k-point (the
+    reference's calling pattern);
+  * fp64: one batched call over the 16 k-points (grouped launches, the
+    persistent H/O kernel with the plan picked by the autotuner);
+  * split: the same batch with the muffin-tin products in int8 Ozaki
+    scheme II (the headline value).
+
+Then, unless switched off: the 16-atom orthorhombic cell with 128 uneven
+k-points as one batch, a bounded sample of the serving loop (Poisson
+requests over the 2..64-atom cells, setup + eigensolve, p50/p99 latency),
+and the oracle on one k-point of the headline workload as the CPU baseline.
 
 Prints one JSON line (see the keys below).  Progress goes to stderr.
 """
@@ -45,17 +57,15 @@ def flops_ho(natm, M, G):
     return natm * 8.0 * G * (G + 1) * M
 
 
-PROFILE = os.path.join(ROOT, "profiles", "ho_kernel_sc64_fp64.md")
-
-
-def measured_traffic(config_index: int):
-    """HBM-side bytes per ho_kernel launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, each scaled by a calibration copy).
-    Only the 64-atom workload was counted; other workloads report null."""
-    if config_index != 3 or not os.path.exists(PROFILE):
+def measured_traffic(profile: str, per: str = "launch"):
+    """HBM-side bytes of one kernel from a committed rocprofv3 --pmc summary
+    under profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by a calibration
+    copy): the value of its ``traffic_bytes_per_<per>`` row, or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
         return None
     import re
-    m = re.search(r"\|\s*traffic_bytes_per_launch\s*\|\s*([0-9.eE+-]+)", open(PROFILE).read())
+    m = re.search(r"\|\s*traffic_bytes_per_" + per + r"\s*\|\s*([0-9.eE+-]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -70,38 +80,29 @@ def oz_bits(nmod, K):
     return int(min(30, math.floor((lm - 1.0 - math.log2(6.0 * K)) * 0.5)))
 
 
-def gen_eigvals(H, O, n, nst):
-    """Lowest nst eigenvalues of H x = e O x from the column-major device
-    buffers (the tensors hold H^T): LAPACK ZHEGVX semantics ('I', upper
-    triangles) on the host, as solvewithlapack does (modfvsystem.f90:691)."""
-    import scipy.linalg as sl
-    Hm = H[:n * n].view(n, n).transpose(0, 1).cpu().numpy()
-    Om = O[:n * n].view(n, n).transpose(0, 1).cpu().numpy()
-    return sl.eigh(Hm, Om, lower=False, eigvals_only=True, subset_by_index=[0, nst - 1],
-                   driver="gvx"), "host LAPACK"
-
-
-def accuracy_check(ctx, d, apwalm, one_setup, nmod, nst):
-    """Split vs FP64 on one k-point: relative Frobenius errors of H and O and
-    the largest difference among the lowest nst generalized eigenvalues."""
+def accuracy(ctx, kp, dk, H, O, nmod, nst):
+    """Split vs FP64 on one k-point of a batch: relative Frobenius errors of H
+    and O and the largest difference among the lowest nst generalized
+    eigenvalues, both solved in FP64 on the GPU (Cholesky reduction +
+    Hermitian eigensolve, the ZHEGVX problem of solvewithlapack)."""
     import torch
-    from exciting_amd import lapw
-    n = d.rank
-    H64 = torch.empty(n * n, dtype=torch.complex128, device="cuda")
-    O64 = torch.empty_like(H64)
-    Hs = torch.empty_like(H64)
-    Os = torch.empty_like(H64)
-    one_setup(d, 0, H64, O64)
-    one_setup(d, lapw.SPLIT(nmod), Hs, Os)
+    from exciting_amd import lapw, solve
+    n = dk.rank
+    ctx.setup_batch([kp], flags=lapw.MATCH)
     torch.cuda.synchronize()
-    rh = float(torch.linalg.norm(Hs - H64) / torch.linalg.norm(H64))
-    ro = float(torch.linalg.norm(Os - O64) / torch.linalg.norm(O64))
+    H64, O64 = H.clone(), O.clone()
+    ctx.setup_batch([kp], flags=lapw.MATCH | lapw.SPLIT(nmod))
+    torch.cuda.synchronize()
+    rh = float(torch.linalg.norm(H - H64) / torch.linalg.norm(H64))
+    ro = float(torch.linalg.norm(O - O64) / torch.linalg.norm(O64))
     t0 = time.perf_counter()
-    w64, where = gen_eigvals(H64, O64, n, nst)
-    ws, _ = gen_eigvals(Hs, Os, n, nst)
+    w64 = solve.geneig(H64, O64, n, nst).cpu().numpy()
+    ws = solve.geneig(H, O, n, nst).cpu().numpy()
     de = float(np.abs(ws - w64).max())
+    del H64, O64
     return {"kpoint": 0, "n": n, "nstates": nst, "rel_frob_H": rh, "rel_frob_O": ro,
-            "max_abs_dE_Ha": de, "within_1uHa": bool(de <= 1e-6), "solver": where,
+            "max_abs_dE_Ha": de, "within_1uHa": bool(de <= 1e-6),
+            "solver": "GPU FP64: Cholesky of O, two triangular solves, Hermitian eigensolve",
             "eig_seconds": time.perf_counter() - t0}
 
 
@@ -135,18 +136,192 @@ def cpu_baseline(s, ik: int):
     }
 
 
+class Workload:
+    """One workload resident on the GPU: library context with the cell's
+    tables, per-k inputs, and per-k apwalm, H and O (a batch writes every
+    k-point's own matrices)."""
+
+    def __init__(self, index, nkpts=0, workspace_gib=0.0):
+        import torch
+        from exciting_amd import lapw, synth
+        self.spec = synth.config(index)
+        ks = list(range(len(self.spec.kpoints)))
+        if nkpts:
+            ks = ks[:nkpts]
+        log(f"building workload {self.spec.name} with {len(ks)} k-points")
+        self.s = synth.build(self.spec, kpoints=ks)
+        self.nk = len(self.s.kpts)
+        self.ngks = [k.ngk for k in self.s.kpts]
+        log(f"  ngk min/mean/max {min(self.ngks)}/{np.mean(self.ngks):.0f}/{max(self.ngks)} "
+            f"nlotot {self.s.nlotot}")
+        self.ctx = lapw.Context(0).load(self.s)
+        if workspace_gib:
+            self.ctx.set_batch_workspace(int(workspace_gib * 2**30))
+        self.dks = [lapw.DeviceKPoint(self.s, ik) for ik in range(self.nk)]
+        self.H = [torch.zeros(d.rank * d.rank, dtype=torch.complex128, device="cuda")
+                  for d in self.dks]
+        self.O = [torch.zeros_like(h) for h in self.H]
+        self.kp = [lapw.kpoint_struct(d, h, o) for d, h, o in zip(self.dks, self.H, self.O)]
+        torch.cuda.synchronize()
+        self.stream = torch.cuda.ExternalStream(self.ctx.stream)
+        M = self.s.maxaa
+        self.nlo_atom = int(self.s.losize.max()) if self.s.losize.size else 0
+        self.f_setup = [flops(self.s.natmtot, M, g, self.nlo_atom) for g in self.ngks]
+        self.f_ho = [flops_ho(self.s.natmtot, M, g) for g in self.ngks]
+        self.K = self.s.natmtot * M
+
+    def free(self):
+        import torch
+        self.ctx.close()
+        self.H = self.O = self.kp = self.dks = None
+        torch.cuda.empty_cache()
+
+    def one_setup(self, ik, flags):
+        from exciting_amd import lapw
+        d = self.dks[ik]
+        self.ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, d.apwalm)
+        sysd = lapw.EvSystem(lapw.HermitianMatrix(d.rank, self.H[ik]),
+                             lapw.HermitianMatrix(d.rank, self.O[ik]))
+        self.ctx.hamiltonandoverlapsetup(sysd, d.ngk, d.apwalm, d.igpig, d.vgpc, flags=flags)
+
+    def run(self, flags, label, steps, warmup, batch=True, profile=True):
+        """Timed steps; per step either one batched call over every k-point or
+        the single-k calls in a loop.  Returns host wall time, per-setup
+        latencies (ms, HIP events on the library's stream) and the per-kernel
+        device times (HIP events around every launch) when profiled."""
+        import torch
+        from exciting_amd import lapw
+
+        def step(lat):
+            if batch:
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(self.stream)
+                self.ctx.setup_batch(self.kp, flags=flags | lapw.MATCH)
+                b.record(self.stream)
+                torch.cuda.synchronize()
+                if lat is not None:   # every setup of a batch completes with it
+                    lat += [a.elapsed_time(b)] * self.nk
+                return
+            evs = []
+            for ik in range(self.nk):
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(self.stream)
+                self.one_setup(ik, flags)
+                b.record(self.stream)
+                evs.append((a, b))
+            torch.cuda.synchronize()
+            if lat is not None:
+                lat += [a.elapsed_time(b) for a, b in evs]
+
+        for w in range(warmup):
+            step(None)
+            log(f"{label}: warmup step {w + 1}/{warmup} done")
+        if profile:
+            self.ctx.profile(True)
+            self.ctx.profile_reset()
+        lat = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step(lat)
+            log(f"{label}: step {i + 1}/{steps} done")
+        t1 = time.perf_counter()
+        prof = self.ctx.profile_read() if profile else {}
+        if profile:
+            self.ctx.profile(False)
+        el = t1 - t0
+        f = steps * sum(self.f_setup)
+        return {"value": steps * self.nk / el, "ms_per_step": 1e3 * el / steps,
+                "gflops": f / el / 1e9, "frac_fp64_peak": f / el / 1e12 / FP64_PEAK_TFLOPS,
+                "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                               "p99": float(np.percentile(lat, 99))},
+                "kernels_ms_per_setup": {k: v[0] / (steps * self.nk)
+                                         for k, v in prof.items() if v[1]},
+                "_prof": prof, "_steps": steps}
+
+    def ho_roofline(self, r, kernel, profile_md):
+        ms, n = r["_prof"][kernel]
+        achieved = r["_steps"] * sum(self.f_ho) / (ms * 1e-3) / 1e12
+        return {"kernel": f"{kernel} (fused upper-triangle H/O contraction + interstitial)",
+                "launches": n, "avg_launch_ms": ms / n,
+                "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": measured_traffic(profile_md)}
+
+    def oz_roofline(self, r, nmod, profile_md):
+        ms, n = r["_prof"]["oz_gemm"]
+        K = self.K
+        ops = r["_steps"] * sum(6 * nmod * g * (g + 1) / 2 * K * 2.0 for g in self.ngks)
+        achieved = ops / (ms * 1e-3) / 1e12
+        return {"kernel": f"ozgemm_kernel (int8 MFMA modular GEMMs, {nmod} moduli x 6 "
+                          f"products, upper triangle)",
+                "launches": n, "avg_launch_ms": ms / n,
+                "bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "frac": achieved / INT8_PEAK_TOPS, "traffic": measured_traffic(profile_md)}
+
+
+def clean(r):
+    return {k: v for k, v in r.items() if not k.startswith("_")}
+
+
+def serving_sample(args):
+    """Bounded sample of the serving loop: Poisson requests over the 2, 8,
+    16, 32 and 64-atom cells (1..kmax k-points each, FP64 or split), each
+    match + setup + GPU eigensolve of the lowest states per k-point."""
+    from exciting_amd import serve, synth
+    specs = synth.serving_cells(args.serve_kmax)
+    cells = {name: synth.build(spec, kpoints=ks) for name, (spec, ks) in specs.items()}
+    make, close, execute = serve.gpu_executor(0)
+    budget = serve.free_hbm(0) - (16 << 30)
+    srv = serve.Server(cells, execute, make, close, budget, max_inflight=8,
+                       on_finish=lambda r: log(f"serve: request {r.rid} {r.cell} "
+                                               f"nk={len(r.kpoints)} nmod={r.nmod} {r.status} "
+                                               f"{r.latency:.2f} s"))
+    warm = [serve.Request(rid=-1 - i, cell=c, kpoints=[0], nmod=m, nstfv=args.serve_nstfv)
+            for i, (c, m) in enumerate((c, m) for c in cells for m in (0, args.moduli))]
+    srv.run(warm)
+    reqs = serve.poisson_requests(args.serve_requests, args.serve_rate, cells, seed=7,
+                                  kmax=args.serve_kmax, nmod=args.moduli,
+                                  nstfv=args.serve_nstfv)
+    t0 = time.perf_counter()
+    srv.run(reqs)
+    out = serve.summarize(reqs, time.perf_counter() - t0)
+    out.update({"sample": (f"{args.serve_requests} Poisson requests at {args.serve_rate}/s over "
+                           f"fcc2/sc8/ortho16/sc32/sc64, 1..{args.serve_kmax} k-points each, "
+                           f"half split ({args.moduli} moduli), {args.serve_nstfv} states"),
+                "peak_inflight": srv.peak_inflight, "contexts": srv.pool.created})
+    srv.pool.close_all()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=3, help="workload index (0..3)")
+    ap.add_argument("--config", type=int, default=3, help="headline workload index (0..3)")
     ap.add_argument("--kpoints", type=int, default=0, help="use the first N k-points (0 = all)")
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
     ap.add_argument("--moduli", type=int, default=9,
                     help="split precision: number of int8 moduli (0 = FP64 only)")
     ap.add_argument("--nstates", type=int, default=400,
                     help="lowest eigenvalues compared between split and FP64 (one k-point)")
+    ap.add_argument("--tune", choices=("on", "off"), default="on",
+                    help="pick the batch plan with the autotuner before timing")
+    ap.add_argument("--loop", choices=("on", "off"), default="on",
+                    help="also time the single-k entry points in a loop")
+    ap.add_argument("--workspace-gib", type=float, default=48.0,
+                    help="batch scratch cap (48 GiB holds the 16 headline k-points in one group)")
+    ap.add_argument("--uneven", choices=("on", "off"), default="on",
+                    help="also run the 16-atom cell with 128 uneven k-points as one batch")
+    ap.add_argument("--uneven-moduli", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests in the serving sample (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-kmax", type=int, default=4)
+    ap.add_argument("--serve-nstfv", type=int, default=40)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -159,30 +334,13 @@ def main():
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    from exciting_amd import solve
+    solve.configure()          # before the first hipBLAS handle
     import torch
-    from exciting_amd import lapw, synth
+    from exciting_amd import lapw
 
-    spec = synth.config(args.config)
-    ks = list(range(len(spec.kpoints)))
-    if args.kpoints:
-        ks = ks[:args.kpoints]
-    log(f"building workload {spec.name} with {len(ks)} k-points")
-    s = synth.build(spec, kpoints=ks)
-    nk = len(s.kpts)
-    ngks = [k.ngk for k in s.kpts]
-    log(f"ngk min/mean/max {min(ngks)}/{np.mean(ngks):.0f}/{max(ngks)} nlotot {s.nlotot}")
-
-    ctx = lapw.Context(0).load(s)
-    dks = [lapw.DeviceKPoint(s, ik) for ik in range(nk)]
-    # one apwalm and one H/O pair reused by every k-point of a step
-    apwalm = dks[0].apwalm
-    for d in dks[1:]:
-        d.apwalm = apwalm
-    nmax = max(d.rank for d in dks)
-    Hbuf = torch.empty(nmax * nmax, dtype=torch.complex128, device="cuda")
-    Obuf = torch.empty(nmax * nmax, dtype=torch.complex128, device="cuda")
-    torch.cuda.synchronize()
-    stream = torch.cuda.ExternalStream(ctx.stream)
+    W = Workload(args.config, args.kpoints, args.workspace_gib)
+    s, nk = W.s, W.nk
     if args.copy_calibration_mib:
         n = args.copy_calibration_mib * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -192,85 +350,33 @@ def main():
         del src, dst
         log(f"calibration copy of {args.copy_calibration_mib} MiB done")
 
-    def one_setup(d, flags, H=None, O=None):
-        ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, apwalm)
-        Hb = Hbuf if H is None else H
-        Ob = Obuf if O is None else O
-        sysd = lapw.EvSystem(lapw.HermitianMatrix(d.rank, Hb[:d.rank * d.rank]),
-                             lapw.HermitianMatrix(d.rank, Ob[:d.rank * d.rank]))
-        ctx.hamiltonandoverlapsetup(sysd, d.ngk, apwalm, d.igpig, d.vgpc, flags=flags)
-
-    def run(flags, label):
-        for w in range(args.warmup):
-            for d in dks:
-                one_setup(d, flags)
-            torch.cuda.synchronize()
-            log(f"{label}: warmup step {w + 1}/{args.warmup} done")
-        ctx.profile(True)
-        ctx.profile_reset()
-        lat = []
-        torch.cuda.synchronize()
+    plan = W.ctx.batch_plan
+    if args.tune == "on":
         t0 = time.perf_counter()
-        for step in range(args.steps):
-            evs = []
-            for d in dks:
-                a = torch.cuda.Event(enable_timing=True)
-                b = torch.cuda.Event(enable_timing=True)
-                a.record(stream)
-                one_setup(d, flags)
-                b.record(stream)
-                evs.append((a, b))
-            torch.cuda.synchronize()
-            lat += [a.elapsed_time(b) for a, b in evs]
-            log(f"{label}: step {step + 1}/{args.steps} done")
-        t1 = time.perf_counter()
-        prof = ctx.profile_read()
-        ctx.profile(False)
-        return {"elapsed": t1 - t0, "lat": lat, "prof": prof}
-
-    M = s.maxaa
-    nlo_atom = int(s.losize.max()) if s.losize.size else 0
-    f_setup = [flops(s.natmtot, M, g, nlo_atom) for g in ngks]
-    f_ho = [flops_ho(s.natmtot, M, g) for g in ngks]
-
-    def summary(r):
-        el = r["elapsed"]
-        return {"value": args.steps * nk / el, "ms_per_step": 1e3 * el / args.steps,
-                "gflops": args.steps * sum(f_setup) / el / 1e9,
-                "frac_fp64_peak": args.steps * sum(f_setup) / el / 1e12 / FP64_PEAK_TFLOPS,
-                "latency_ms": {"p50": float(np.percentile(r["lat"], 50)),
-                               "p99": float(np.percentile(r["lat"], 99))},
-                "kernels_ms_per_setup": {k: (v[0] / v[1] if v[1] else 0.0)
-                                         for k, v in r["prof"].items() if v[1]}}
-
-    r64 = run(0, "fp64")
-    fp64 = summary(r64)
-    ho_ms, ho_n = r64["prof"]["ho"]
-    achieved_ho = (args.steps * sum(f_ho) / ho_n) / (ho_ms / ho_n * 1e-3) / 1e12
-    fp64["roofline"] = {"kernel": "ho_kernel (fused upper-triangle H/O contraction + interstitial)",
-                        "bound": "mfma", "achieved": achieved_ho, "peak": FP64_PEAK_TFLOPS,
-                        "unit": "TFLOP/s", "frac": achieved_ho / FP64_PEAK_TFLOPS,
-                        "traffic": measured_traffic(args.config)}
+        plan = W.ctx.tune_batch(W.kp, flags=lapw.MATCH)
+        log(f"autotuner: plan {plan} {lapw.plan_info(plan)} in {time.perf_counter() - t0:.1f} s")
+    fp64_loop = None
+    if args.loop == "on":
+        fp64_loop = W.run(0, "fp64 loop", args.steps, args.warmup, batch=False)
+    fp64 = W.run(0, "fp64 batch", args.steps, args.warmup)
+    fp64["roofline"] = W.ho_roofline(fp64, "ho_grouped", "ho_grouped_sc64_fp64.md")
+    fp64["plan"] = dict(lapw.plan_info(plan), id=plan)
+    if fp64_loop:
+        fp64_loop["roofline"] = W.ho_roofline(fp64_loop, "ho", "ho_kernel_sc64_fp64.md")
 
     head = fp64
     split = None
     nmod = args.moduli
     if nmod:
-        rsp = run(lapw.SPLIT(nmod), f"split({nmod})")
-        split = summary(rsp)
-        K = s.natmtot * M
-        bits = oz_bits(nmod, K)
-        gm_ms, gm_n = rsp["prof"]["oz_gemm"]
-        i8_ops = args.steps * sum(6 * nmod * g * (g + 1) / 2 * K * 2.0 for g in ngks)
-        achieved_i8 = (i8_ops / gm_n) / (gm_ms / gm_n * 1e-3) / 1e12
-        split["roofline"] = {"kernel": f"ozgemm_kernel (int8 MFMA modular GEMMs, {nmod} moduli x "
-                                       f"6 products, upper triangle)",
-                             "bound": "mfma", "achieved": achieved_i8, "peak": INT8_PEAK_TOPS,
-                             "unit": "TOP/s", "frac": achieved_i8 / INT8_PEAK_TOPS,
-                             "traffic": None}
+        split = W.run(lapw.SPLIT(nmod), f"split({nmod}) batch", args.steps, args.warmup)
+        split["roofline"] = W.oz_roofline(split, nmod, f"ozgemm_sc64_split{nmod}.md")
+        g = W.run(lapw.SPLIT(nmod) | lapw.GRAPH, f"split({nmod}) graph", args.steps,
+                  args.warmup, profile=False)
+        split["graph"] = {"value": g["value"], "ms_per_step": g["ms_per_step"]}
         split["moduli"] = nmod
-        split["bits"] = bits
-        split["accuracy"] = accuracy_check(ctx, dks[0], apwalm, one_setup, nmod, args.nstates)
+        split["bits"] = oz_bits(nmod, W.K)
+        split["accuracy"] = accuracy(W.ctx, W.kp[0], W.dks[0], W.H[0], W.O[0], nmod,
+                                     args.nstates)
         log(f"split accuracy {split['accuracy']}")
         head = split
 
@@ -288,31 +394,68 @@ def main():
         "dtype": f"i8x{nmod}+f64" if nmod else "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"{spec.name}: {s.natmtot}-atom cell, lmaxapw={s.lmaxapw}, APW order "
-                         f"{s.apwordmax} + {nlo_atom} LO functions per atom, rgkmax="
-                         f"{spec.rgkmax}, {nk} k-points, match + H/O setup"),
+            "workload": (f"{W.spec.name}: {s.natmtot}-atom cell, lmaxapw={s.lmaxapw}, APW order "
+                         f"{s.apwordmax} + {W.nlo_atom} LO functions per atom, rgkmax="
+                         f"{W.spec.rgkmax}, {nk} k-points, match + H/O setup as one batch"),
             "kpoints": nk,
-            "ngk_mean": float(np.mean(ngks)),
-            "n_mean": float(np.mean(ngks)) + s.nlotot,
+            "ngk_mean": float(np.mean(W.ngks)),
+            "n_mean": float(np.mean(W.ngks)) + s.nlotot,
             "natoms": s.natmtot,
             "precision": (f"split: muffin-tin products by int8 Ozaki scheme II with {nmod} "
-                          f"moduli (operands rounded to {oz_bits(nmod, s.natmtot * M)} bits per "
-                          f"G column); match, LO blocks, interstitial FP64") if nmod else "fp64",
+                          f"moduli (operands rounded to {oz_bits(nmod, W.K)} bits per G "
+                          f"column); match, LO blocks, interstitial FP64") if nmod else "fp64",
         },
         "gflops": head["gflops"],
         "latency_ms": head["latency_ms"],
         "kernels_ms_per_setup": head["kernels_ms_per_setup"],
         "roofline": head["roofline"],
-        "fp64": fp64,
+        "fp64": clean(fp64),
     }
+    if fp64_loop:
+        out["fp64_loop"] = clean(fp64_loop)
     if split:
+        out["split"] = {k: split[k] for k in ("value", "ms_per_step", "graph", "moduli", "bits")}
         out["speedup_vs_fp64"] = split["value"] / fp64["value"]
+        if fp64_loop:
+            out["speedup_vs_fp64_loop"] = split["value"] / fp64_loop["value"]
         out["accuracy"] = split["accuracy"]
+    W.free()
+
+    if args.uneven == "on":
+        U = Workload(2, 0, 0.0)
+        up = U.ctx.tune_batch(U.kp, flags=lapw.MATCH) if args.tune == "on" else U.ctx.batch_plan
+        log(f"uneven: plan {up} {lapw.plan_info(up)}")
+        ub = U.run(0, "uneven fp64 batch", args.steps, args.warmup)
+        ug = U.run(lapw.GRAPH, "uneven fp64 graph", args.steps, args.warmup, profile=False)
+        ul = U.run(0, "uneven fp64 loop", args.steps, args.warmup, batch=False)
+        um = args.uneven_moduli
+        us = U.run(lapw.SPLIT(um), f"uneven split({um}) batch", args.steps, args.warmup)
+        uacc = accuracy(U.ctx, U.kp[0], U.dks[0], U.H[0], U.O[0], um, args.nstates)
+        out["uneven"] = {
+            "workload": (f"{U.spec.name}: {U.s.natmtot}-atom orthorhombic cell, {U.nk} "
+                         f"k-points, ngk {min(U.ngks)}..{max(U.ngks)}, one batch per step"),
+            "plan": dict(lapw.plan_info(up), id=up),
+            "fp64_batch": {k: ub[k] for k in ("value", "ms_per_step", "gflops", "frac_fp64_peak")},
+            "fp64_graph": {k: ug[k] for k in ("value", "ms_per_step")},
+            "fp64_loop": {k: ul[k] for k in ("value", "ms_per_step", "gflops", "frac_fp64_peak")},
+            "batch_vs_loop": ub["value"] / ul["value"],
+            "ho_roofline": U.ho_roofline(ub, "ho_grouped", "ho_grouped_ortho16_fp64.md"),
+            "split_batch": {k: us[k] for k in ("value", "ms_per_step")},
+            "split_moduli": um,
+            "split_vs_fp64_batch": us["value"] / ub["value"],
+            "split_accuracy": uacc,
+        }
+        log(f"uneven: {out['uneven']}")
+        U.free()
+
+    if args.serve_requests:
+        out["serving"] = serving_sample(args)
+        log(f"serving: {out['serving']}")
+
     if args.cpu_baseline == "on":
         log("cpu baseline: oracle on one k-point")
         out["cpu_baseline"] = cpu_baseline(s, 0)
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -201,6 +201,27 @@ int drain_records(exh_context *c) {
 
 namespace exh {
 
+hipError_t mark_done(exh_context *c, hipStream_t s) {
+  if (c->capturing) return hipSuccess;
+  if (!c->done_ev) {
+    hipError_t e = hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      c->done_ev = nullptr;
+      return e;
+    }
+  }
+  hipError_t e = hipEventRecord(c->done_ev, s);
+  if (e == hipSuccess) c->done_pending = true;
+  return e;
+}
+
+int wait_done(exh_context *c) {
+  if (!c->done_pending) return 0;
+  hipError_t e = hipEventSynchronize(c->done_ev);
+  c->done_pending = false;
+  return e == hipSuccess ? 0 : hip_fail(c, e, "waiting for queued work");
+}
+
 // Constants and scratch of the split-precision path for nmod moduli.
 int oz_prepare(exh_context *c, int nmod, OzLayout &L, int &bits) {
   L = oz_layout(c->ngkmax, c->Ktot, nmod);
@@ -255,6 +276,7 @@ void exh_destroy(exh_context *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   batch_release_all(c);
+  if (c->done_ev) (void)hipEventDestroy(c->done_ev);
   DevBuf *bufs[] = {&c->colA, &c->katom, &c->atom_species_d, &c->apword_d, &c->omax_d,
                     &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
                     &c->aa, &c->loa, &c->lolo, &c->salo, &c->ololo_eff, &c->h1aa,
@@ -280,7 +302,7 @@ int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, 
                   int apwordmax, const int *apword, int nlomax, const int *nlorb,
                   const int *lorbl, int ngkmax) {
   EXH_CHECK_CTX(c);
-  c->epoch++;
+  if (int rcw = wait_done(c)) return rcw;
   if (nspecies <= 0 || !natoms || !apword || !nlorb || (nlomax > 0 && !lorbl))
     return fail(c, EXH_EINVAL, "set_basis: null or empty argument");
   if (lmaxapw < 0 || lmaxmat < 0 || lmaxmat > lmaxapw)
@@ -398,7 +420,7 @@ int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, 
 int exh_set_radial(exh_context *c, double omega, const double *rmt, const int *nrmt,
                    int nrmtmax, const double *spr, const double *apwfr) {
   EXH_CHECK_CTX(c);
-  c->epoch++;
+  if (int rcw = wait_done(c)) return rcw;
   if (!c->have_basis) return fail(c, EXH_ESTATE, "set_radial: call exh_set_basis first");
   if (!(omega > 0.0) || !rmt || !nrmt || !spr || !apwfr || nrmtmax <= 0)
     return fail(c, EXH_EINVAL, "set_radial: bad argument");
@@ -422,7 +444,7 @@ int exh_set_mt(exh_context *c, int maxaa, int maxnlo, const exh_z *aa, const exh
                const exh_z *lolo, const double *oalo, const double *ololo, const double *h1aa,
                const double *h1loa, const double *h1lolo) {
   EXH_CHECK_CTX(c);
-  c->epoch++;
+  if (int rcw = wait_done(c)) return rcw;
   if (!c->have_basis) return fail(c, EXH_ESTATE, "set_mt: call exh_set_basis first");
   if (maxaa != c->maxaa || maxnlo != c->maxnlo)
     return fail(c, EXH_EINVAL, "set_mt: maxaa/maxnlo differ from the basis");
@@ -462,7 +484,7 @@ int exh_set_mt(exh_context *c, int maxaa, int maxnlo, const exh_z *aa, const exh
 int exh_set_gvectors(exh_context *c, int ngrtot, int ngvec, const int *ivg, const int *intgv,
                      const int *ivgig) {
   EXH_CHECK_CTX(c);
-  c->epoch++;
+  if (int rcw = wait_done(c)) return rcw;
   if (ngrtot <= 0 || ngvec <= 0 || ngvec > ngrtot || !ivg || !intgv || !ivgig)
     return fail(c, EXH_EINVAL, "set_gvectors: bad argument");
   for (int d = 0; d < 3; d++)
@@ -486,7 +508,7 @@ int exh_set_interstitial(exh_context *c, const exh_z *cfunig, const exh_z *veffi
                          const exh_z *meffig, const exh_z *m2effig, int valence_rel_none,
                          int h1on) {
   EXH_CHECK_CTX(c);
-  c->epoch++;
+  if (int rcw = wait_done(c)) return rcw;
   if (!c->have_gvec) return fail(c, EXH_ESTATE, "set_interstitial: call exh_set_gvectors first");
   if (!cfunig || !veffig || (!valence_rel_none && !meffig) || (h1on && !m2effig))
     return fail(c, EXH_EINVAL, "set_interstitial: null array");
@@ -518,6 +540,7 @@ int exh_match_device(exh_context *c, void *stream, int ngp, const double *gpc,
   hipError_t e = timed(c, EXH_K_MATCH, s, [&] {
     return launch_match(s, c, ngp, gpc, tpgpc, (const double2 *)sfacgp, (double2 *)apwalm);
   });
+  if (e == hipSuccess) e = mark_done(c, s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "match kernel");
 }
 
@@ -585,6 +608,7 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
     });
   if (e == hipSuccess)
     e = timed(c, EXH_K_LOLO, s, [&] { return launch_lolo(s, c, ngp, (double2 *)h, (double2 *)o, rank, flags); });
+  if (e == hipSuccess) e = mark_done(c, s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "setup kernels");
 }
 

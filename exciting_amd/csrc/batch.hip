@@ -81,6 +81,33 @@ void build_tiles(const exh_kpoint *kp, int k0, int nk, int tmi, int tmj, bool xc
     }
 }
 
+// Everything a recorded batch bakes in besides the k-point list: device
+// addresses of the tables and scratch, and every scalar passed to a kernel
+// or deciding which kernels run.  Tables rewritten in place by exh_set_* keep
+// the signature, so a captured graph stays valid across SCF iterations.
+uint64_t state_signature(const exh_context *c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+  };
+  const DevBuf *bufs[] = {&c->colA, &c->katom, &c->atom_species_d, &c->apword_d, &c->omax_d,
+                          &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu,
+                          &c->mt_match.piv, &c->aa, &c->loa, &c->lolo, &c->salo,
+                          &c->ololo_eff, &c->h1aa, &c->ivg, &c->ist, &c->bbt, &c->bat,
+                          &c->ba2t, &c->bivgk, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex,
+                          &c->ozeb};
+  for (const DevBuf *b : bufs) mix(&b->p, sizeof b->p);
+  const int sc[] = {c->nspecies, c->natmtot, c->lmaxapw, c->lmaxmat, c->apwordmax, c->nlomax,
+                    c->ngkmax, c->maxaa, c->maxnlo, c->nlotot, c->lmmax, c->Ktot,
+                    c->ngrtot, c->ngvec, c->intgv[0], c->intgv[1], c->intgv[2], c->intgv[3],
+                    c->intgv[4], c->intgv[5], c->contig ? 1 : 0, c->have_h1aa ? 1 : 0,
+                    c->uniform_apword ? 1 : 0, c->ncu};
+  mix(sc, sizeof sc);
+  mix(&c->omega, sizeof c->omega);
+  return h;
+}
+
 int validate(exh_context *c, int nk, const exh_kpoint *kp, int flags) {
   if (!c->have_basis || !c->have_mt || !c->have_gvec || !c->have_ist)
     return fail(c, EXH_ESTATE, "batch: basis, muffin-tin tables and interstitial data must be set");
@@ -312,13 +339,16 @@ int exh_setup_batch_device(exh_context *c, void *stream, int nk, const exh_kpoin
     if (!rc && nmod) rc = oz_prepare(c, nmod, L, bits);
     if (rc) return rc;
   }
-  // key: arguments, plan, group size and the context epoch
-  std::vector<unsigned char> key(sizeof(int) * 4 + sizeof(uint64_t) + nk * sizeof(exh_kpoint));
+  // key: arguments, plan, group size, the context's reallocation epoch and
+  // the signature of what the kernels take from the context
+  std::vector<unsigned char> key(sizeof(int) * 4 + 2 * sizeof(uint64_t) +
+                                 nk * sizeof(exh_kpoint));
   {
     const int hdr[4] = {nk, flags & ~EXH_GRAPH, plan, nkg};
+    const uint64_t sig[2] = {c->epoch, state_signature(c)};
     std::memcpy(key.data(), hdr, sizeof hdr);
-    std::memcpy(key.data() + sizeof hdr, &c->epoch, sizeof(uint64_t));
-    std::memcpy(key.data() + sizeof hdr + sizeof(uint64_t), kp, nk * sizeof(exh_kpoint));
+    std::memcpy(key.data() + sizeof hdr, sig, sizeof sig);
+    std::memcpy(key.data() + sizeof hdr + sizeof sig, kp, nk * sizeof(exh_kpoint));
   }
   exh_context::BatchEntry *E = find_entry(c, key);
   if (!E) {
@@ -351,11 +381,13 @@ int exh_setup_batch_device(exh_context *c, void *stream, int nk, const exh_kpoin
       E->graph = g;
     }
     e = timed(c, EXH_K_GRAPH, s, [&] { return hipGraphLaunch(E->exec, s); });
+    if (e == hipSuccess) e = mark_done(c, s);
     return e == hipSuccess ? 0 : hip_fail(c, e, "batch: graph launch");
   }
   const bool was = c->capturing;
   c->capturing = was || outer_capture;
   e = enqueue(c, s, *E, flags, plan, ldb, L, bits);
+  if (e == hipSuccess) e = mark_done(c, s);
   c->capturing = was;
   return e == hipSuccess ? 0 : hip_fail(c, e, "batch kernels");
 }

@@ -135,6 +135,10 @@ struct exh_context {
   size_t batch_ws_limit = size_t(8) << 30;
   uint64_t epoch = 0;  // bumped by every state change and scratch reallocation
   bool capturing = false;
+  // completion of the last hot-path work queued on any stream; the setters
+  // wait for it before rewriting tables that work may still read
+  hipEvent_t done_ev = nullptr;
+  bool done_pending = false;
   exh::DevBuf bbt, bat, ba2t, bivgk;
   struct BatchEntry;
   std::vector<BatchEntry *> bcache;
@@ -172,6 +176,8 @@ hipError_t timed(exh_context *c, int k, hipStream_t s, F &&launch) {
   return e;
 }
 void batch_release_all(exh_context *c);
+hipError_t mark_done(exh_context *c, hipStream_t s);
+int wait_done(exh_context *c);
 
 // Kernel launchers (setup_kernels.hip / match_kernel.hip).
 hipError_t launch_ivgk(hipStream_t s, int ngp, const int *igpig, const int *ivg,

@@ -124,95 +124,18 @@ __global__ void residue_kernel(int ngp, int Ktot, int Gpad, const double2 *__res
 }
 
 // ---------------------------------------------------------------------------
-// Upper-triangular int8 GEMM, one (product, modulus) per workgroup:
-//   C[i][j] = sum_k A[k][i] Bm[k][j]  (int32, exact), stored mod p as bytes.
-// 128x128 tiles, 4 waves of 64x64 (2x2 blocks of v_mfma_i32_32x32x32_i8).
+// Upper-triangular int8 GEMMs, one (product pair, modulus, tile) per
+// workgroup.  The H product P_q and the O product Q_q (q = 0, 1, 2) share the
+// row operand (xr, xi or xr+xi), as the FP64 kernel shares conj(A) between H
+// and O, so a workgroup streams three operand panels for two products:
+//   C_H[i][j] = sum_k A[k][i] BH[k][j],  C_O[i][j] = sum_k A[k][i] BO[k][j]
+// (int32, exact), each stored mod p as bytes.  128x128 tiles, 4 waves of
+// 64x64 (2x2 blocks of v_mfma_i32_32x32x32_i8 per product).
 // Operand lane layout: row/column = lane%32, 16 k bytes from k block
 // 2*ks + lane/32; result D[8*(r/4) + 4*(lane/32) + r%4][lane%32].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void ozgemm_kernel(int npairs, int nslab, int Gpad, int nmod,
-                                                     const int8_t *__restrict__ QX,
-                                                     const int8_t *__restrict__ QB,
-                                                     size_t qstride, uint8_t *__restrict__ R) {
-  // XCD-contiguous remap over the whole grid, then (product*nmod+m, tile)
-  const int bid = blockIdx.x, nb = gridDim.x;
-  const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
-  const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
-  const int pm = t / npairs, tile = t - pm * npairs;
-  const int prod = pm / nmod, m = pm - prod * nmod;
-  int J = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
-  while ((J + 1) * (J + 2) / 2 <= tile) J++;
-  while (J * (J + 1) / 2 > tile) J--;
-  const int I = tile - J * (J + 1) / 2;
-  const int I0 = I * OZ_T, J0 = J * OZ_T;
-
-  // operands: rows from X arrays {xr, xi, xs}; columns from B {br, bi, bd}
-  // (H products 0-2) or X {xr, xi, xd} (O products 3-5)
-  const int ra = prod % 3;
-  const int8_t *Ab = QX + (size_t)(m * 4 + ra) * qstride;
-  const int8_t *Bb = prod < 3 ? QB + (size_t)(m * 3 + ra) * qstride
-                              : QX + (size_t)(m * 4 + (ra == 2 ? 3 : ra)) * qstride;
-
-  __shared__ v4i sA[2][4][OZ_T], sB[2][4][OZ_T];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wr = w >> 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
-
-  v16i acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = v16i{0};
-
-  // global -> register prefetch: 2 chunks of 16 B per operand per thread
-  v4i pa[2], pb[2];
-  auto load = [&](int s) {
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int c = tid + 256 * q, kk = c >> 7, row = c & 127;
-      const size_t kblk = (size_t)s * 4 + kk;
-      pa[q] = *(const v4i *)(Ab + (kblk * Gpad + I0 + row) * 16);
-      pb[q] = *(const v4i *)(Bb + (kblk * Gpad + J0 + row) * 16);
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int c = tid + 256 * q, kk = c >> 7, row = c & 127;
-      sA[buf][kk][row] = pa[q];
-      sB[buf][kk][row] = pb[q];
-    }
-  };
-  load(0);
-  store(0);
-  __syncthreads();
-  if (nslab > 1) load(1);
-  for (int s = 0; s < nslab; s++) {
-    const int buf = s & 1;
-#pragma unroll
-    for (int ks = 0; ks < 2; ks++) {
-      const int kk = 2 * ks + lh;
-      v4i av[2], bv[2];
-#pragma unroll
-      for (int a = 0; a < 2; a++) av[a] = sA[buf][kk][wr * 64 + a * 32 + l32];
-#pragma unroll
-      for (int b = 0; b < 2; b++) bv[b] = sB[buf][kk][wc * 64 + b * 32 + l32];
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
-    }
-    if (s + 1 < nslab) {
-      store(buf ^ 1);
-      if (s + 2 < nslab) load(s + 2);
-    }
-    __syncthreads();
-  }
-
-  // residues mod p_m, four consecutive rows packed per 32-bit store
-  const int p = c_p[m];
-  const float rp = c_rp[m];
-  uint8_t *out = R + ((size_t)pm * npairs + tile) * (OZ_T * OZ_T);
+__device__ __forceinline__ void store_residues(const v16i (&acc)[2][2], int p, float rp, int wr,
+                                               int wc, int lh, int l32, uint8_t *out) {
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -233,6 +156,104 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(int npairs, int nslab, int 
         const int jl = wc * 64 + b * 32 + l32;
         *(unsigned *)(out + il + OZ_T * jl) = packed;
       }
+}
+
+__global__ __launch_bounds__(256) void ozgemm_kernel(int npairs, int nslab, int Gpad, int nmod,
+                                                     const int8_t *__restrict__ QX,
+                                                     const int8_t *__restrict__ QB,
+                                                     size_t qstride, uint8_t *__restrict__ R) {
+  // XCD-contiguous remap over the whole grid, then (pair*nmod+m, tile)
+  const int bid = blockIdx.x, nb = gridDim.x;
+  const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
+  const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
+  const int qm = t / npairs, tile = t - qm * npairs;
+  const int q = qm / nmod, m = qm - q * nmod;
+  int J = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+  while ((J + 1) * (J + 2) / 2 <= tile) J++;
+  while (J * (J + 1) / 2 > tile) J--;
+  const int I = tile - J * (J + 1) / 2;
+  const int I0 = I * OZ_T, J0 = J * OZ_T;
+
+  // rows: X array q of {xr, xi, xs}; H columns: B array q of {br, bi, bd};
+  // O columns: X array {xr, xi, xd}[q]
+  const int8_t *Ab = QX + (size_t)(m * 4 + q) * qstride;
+  const int8_t *Hb = QB + (size_t)(m * 3 + q) * qstride;
+  const int8_t *Ob = QX + (size_t)(m * 4 + (q == 2 ? 3 : q)) * qstride;
+
+  __shared__ v4i sA[2][4][OZ_T], sH[2][4][OZ_T], sO[2][4][OZ_T];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
+
+  v16i ah[2][2], ao[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      ah[a][b] = v16i{0};
+      ao[a][b] = v16i{0};
+    }
+
+  // global -> register prefetch: 2 chunks of 16 B per operand per thread
+  v4i pa[2], ph[2], po[2];
+  auto load = [&](int s) {
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int c = tid + 256 * u, kk = c >> 7, row = c & 127;
+      const size_t kblk = (size_t)s * 4 + kk;
+      pa[u] = *(const v4i *)(Ab + (kblk * Gpad + I0 + row) * 16);
+      ph[u] = *(const v4i *)(Hb + (kblk * Gpad + J0 + row) * 16);
+      po[u] = *(const v4i *)(Ob + (kblk * Gpad + J0 + row) * 16);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int c = tid + 256 * u, kk = c >> 7, row = c & 127;
+      sA[buf][kk][row] = pa[u];
+      sH[buf][kk][row] = ph[u];
+      sO[buf][kk][row] = po[u];
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  if (nslab > 1) load(1);
+  for (int s = 0; s < nslab; s++) {
+    const int buf = s & 1;
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      const int kk = 2 * ks + lh;
+      v4i av[2], hv[2], ov[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) av[a] = sA[buf][kk][wr * 64 + a * 32 + l32];
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        hv[b] = sH[buf][kk][wc * 64 + b * 32 + l32];
+        ov[b] = sO[buf][kk][wc * 64 + b * 32 + l32];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          ah[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], hv[b], ah[a][b], 0, 0, 0);
+          ao[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], ov[b], ao[a][b], 0, 0, 0);
+        }
+    }
+    if (s + 1 < nslab) {
+      store(buf ^ 1);
+      if (s + 2 < nslab) load(s + 2);
+    }
+    __syncthreads();
+  }
+
+  // residues mod p_m, four consecutive rows packed per 32-bit store; H
+  // product q and O product 3 + q keep the per-product layout of R
+  const int p = c_p[m];
+  const float rp = c_rp[m];
+  store_residues(ah, p, rp, wr, wc, lh, l32,
+                 R + ((size_t)(q * nmod + m) * npairs + tile) * (OZ_T * OZ_T));
+  store_residues(ao, p, rp, wr, wc, lh, l32,
+                 R + ((size_t)((3 + q) * nmod + m) * npairs + tile) * (OZ_T * OZ_T));
 }
 
 // Garner with symmetric digits: the integer x in (-M/2, M/2) with residues r_k.
@@ -411,7 +432,7 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
     if (e != hipSuccess) return e;
   }
   if (stage_mask & 2) {
-    const long long nblk = npairs * 6 * nmod;
+    const long long nblk = npairs * 3 * nmod;
     hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)npairs, nslab,
                        L.Gpad, nmod, QX, QB, L.qstride, R);
     hipError_t e = hipGetLastError();

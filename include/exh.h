@@ -23,7 +23,10 @@
  *     `stream` (a hipStream_t, NULL = the context's stream) without host
  *     synchronisation.  The others take host pointers and return when the
  *     result is back in host memory.
- *   - A context is used by one host thread at a time.
+ *   - A context is used by one host thread at a time.  Work queued by the
+ *     _device entry points may run on any stream; an exh_set_* call first
+ *     waits for the context's last queued work, so tables are never rewritten
+ *     under a running kernel.
  */
 #ifndef EXH_H
 #define EXH_H
@@ -179,10 +182,11 @@ enum {
    * the setup. */
   EXH_MATCH = 2,
   /* Batch: capture the launches in a HIP graph on the first call and replay
-   * it on later calls with identical arguments (same pointers, sizes, flags,
-   * plan and context state).  Tables changed through exh_set_* are read by
-   * the replay; a set_* call that needs larger device buffers, or a new plan,
-   * rebuilds the graph. */
+   * it on later calls with identical arguments (same pointers, sizes, flags
+   * and plan).  exh_set_* calls that rewrite tables in place (same sizes, as
+   * in every SCF iteration) keep the graph, and replays read the new values;
+   * a change of any size, address or flag the kernels take from the context
+   * makes the next call capture a new graph. */
   EXH_GRAPH = 4
 };
 /* Flags: EXH_ACCUMULATE, EXH_SPLIT(n), EXH_MATCH, EXH_GRAPH.  The per-k
@@ -198,8 +202,8 @@ int exh_setup_batch_device(exh_context *ctx, void *stream, int nk, const exh_kpo
 int exh_batch_plan_count(void);
 int exh_batch_plan_info(int plan, int *tile_rows, int *tile_cols, int *k_slab,
                         int *persistent, int *xcd_order);
-/* Plan used by exh_setup_batch_device (-1 = the default, plan 0 tuned for
- * the 64-atom workload). */
+/* Plan used by exh_setup_batch_device (-1 = the default: 64x64 tiles with K
+ * slabs of 16, one workgroup per tile, XCD order). */
 int exh_set_batch_plan(exh_context *ctx, int plan);
 int exh_get_batch_plan(exh_context *ctx);
 /* Autotuner: times every plan on (the first few of) these k-points, with the

@@ -171,10 +171,21 @@ def test_graph_replays_match_eager(cases):
     c.ctx.set_mt(s.maxaa, s.maxnlo, aa2, s.loa, s.lolo, s.oalo, s.ololo, s.h1aa, s.h1loa,
                  s.h1lolo)
     try:
+        # tables rewritten in place: the replay reads the new values
         new_eager = c.batch(0)
         got = c.batch(lapw.GRAPH, bufs=bufs)
-        same(got, new_eager, "after set_mt")
+        same(got, new_eager, "after set_mt in place")
         assert not np.array_equal(got[0][1], eager[0][1])
+        # an h1aa table switches the a2t stage on: the graph must be recaptured
+        rng = np.random.default_rng(2)
+        h1aa = 0.01 * rng.standard_normal((s.apwordmax, s.apwordmax, s.lmaxapw + 1, s.natmtot))
+        h1loa = 0.01 * rng.standard_normal((s.apwordmax, s.nlomax, s.natmtot))
+        h1lolo = np.zeros((s.nlomax, s.nlomax, s.natmtot))
+        c.ctx.set_mt(s.maxaa, s.maxnlo, s.aa, s.loa, s.lolo, s.oalo, s.ololo, h1aa, h1loa,
+                     h1lolo)
+        rel_eager = c.batch(0)
+        same(c.batch(lapw.GRAPH, bufs=bufs), rel_eager, "after h1aa switched on")
+        assert not np.array_equal(rel_eager[0][2], eager[0][2])
     finally:
         c.ctx.set_mt(s.maxaa, s.maxnlo, s.aa, s.loa, s.lolo, s.oalo, s.ololo, s.h1aa,
                      s.h1loa, s.h1lolo)
