@@ -283,7 +283,10 @@ __device__ __forceinline__ int modp(int x, int k) {
 
 // ---------------------------------------------------------------------------
 // Residues -> FP64 H and O upper elements (+ interstitial, + mirror).
-// One workgroup per tile; thread handles elements tid, tid+256, ...
+// One workgroup per 128x128 tile, walked in 32x32 sub-blocks: each thread
+// reconstructs four elements of a sub-block and writes them to the upper
+// triangle (coalesced along i); the sub-block is staged in LDS and written
+// back transposed and conjugated to the lower triangle (coalesced along j).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ozcombine_kernel(
     int ngp, int npairs, int nmod, const uint8_t *__restrict__ R, const int *__restrict__ ex,
@@ -296,38 +299,61 @@ __global__ __launch_bounds__(256) void ozcombine_kernel(
   const int I = tile - J * (J + 1) / 2;
   const int I0 = I * OZ_T, J0 = J * OZ_T;
   const size_t TT = (size_t)OZ_T * OZ_T;
-  for (int e = threadIdx.x; e < OZ_T * OZ_T; e += 256) {
-    const int il = e & (OZ_T - 1), jl = e >> 7;
-    const int i = I0 + il, j = J0 + jl;
-    if (i >= ngp || j >= ngp || i > j) continue;
-    int hre[kOzMaxMod], him[kOzMaxMod], ore[kOzMaxMod], oim[kOzMaxMod];
-    for (int k = 0; k < nmod; k++) {
-      int r[6];
+  __shared__ double2 sH[32][33], sO[32][33];
+  const int tid = threadIdx.x, lo = tid & 31, hi8 = tid >> 5;
+  for (int sb = 0; sb < 16; sb++) {
+    const int bi = sb & 3, bj = sb >> 2;
+    if (I == J && bi > bj) continue;  // wholly below the diagonal (uniform)
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+      const int il = bi * 32 + lo, jl = bj * 32 + hi8 + 8 * r;
+      const int i = I0 + il, j = J0 + jl;
+      if (i >= ngp || j >= ngp || i > j) continue;
+      const int e = il + OZ_T * jl;
+      int hre[kOzMaxMod], him[kOzMaxMod], ore[kOzMaxMod], oim[kOzMaxMod];
+      for (int k = 0; k < nmod; k++) {
+        int rr[6];
 #pragma unroll
-      for (int pr = 0; pr < 6; pr++)
-        r[pr] = R[((size_t)(pr * nmod + k) * npairs + tile) * TT + e];
-      hre[k] = modp(r[0] + r[1], k);
-      him[k] = modp(r[0] - r[1] - r[2], k);
-      ore[k] = modp(r[3] + r[4], k);
-      oim[k] = modp(r[3] - r[4] - r[5], k);
-    }
-    const int si = ex[i] == INT_MIN ? 0 : bits - ex[i];
-    const int sjb = eb[j] == INT_MIN ? 0 : bits - eb[j];
-    const int sjx = ex[j] == INT_MIN ? 0 : bits - ex[j];
-    double hr = ldexp(garner(hre, nmod), -(si + sjb));
-    double hi = ldexp(garner(him, nmod), -(si + sjb));
-    double orr = ldexp(garner(ore, nmod), -(si + sjx));
-    double oi = ldexp(garner(oim, nmod), -(si + sjx));
-    if (accumulate) {
+        for (int pr = 0; pr < 6; pr++)
+          rr[pr] = R[((size_t)(pr * nmod + k) * npairs + tile) * TT + e];
+        hre[k] = modp(rr[0] + rr[1], k);
+        him[k] = modp(rr[0] - rr[1] - rr[2], k);
+        ore[k] = modp(rr[3] + rr[4], k);
+        oim[k] = modp(rr[3] - rr[4] - rr[5], k);
+      }
+      const int si = ex[i] == INT_MIN ? 0 : bits - ex[i];
+      const int sjb = eb[j] == INT_MIN ? 0 : bits - eb[j];
+      const int sjx = ex[j] == INT_MIN ? 0 : bits - ex[j];
+      double hr = ldexp(garner(hre, nmod), -(si + sjb));
+      double hv = ldexp(garner(him, nmod), -(si + sjb));
+      double orr = ldexp(garner(ore, nmod), -(si + sjx));
+      double oi = ldexp(garner(oim, nmod), -(si + sjx));
       const size_t up = (size_t)i + (size_t)j * ldh;
-      const double2 h0 = H[up], o0 = O[up];
-      hr += h0.x;
-      hi += h0.y;
-      orr += o0.x;
-      oi += o0.y;
+      if (accumulate) {
+        const double2 h0 = H[up], o0 = O[up];
+        hr += h0.x;
+        hv += h0.y;
+        orr += o0.x;
+        oi += o0.y;
+      }
+      ist_add(box, i, j, hr, hv, orr, oi);
+      H[up] = z2(hr, hv);
+      O[up] = z2(orr, oi);
+      sH[jl - bj * 32][lo] = z2(hr, -hv);
+      sO[jl - bj * 32][lo] = z2(orr, -oi);
     }
-    ist_add(box, i, j, hr, hi, orr, oi);
-    store_upper(H, O, ldh, i, j, hr, hi, orr, oi, 0);
+    __syncthreads();
+    // lower triangle: element (j, i) = conj(upper (i, j)), i < j
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+      const int jl = bj * 32 + lo, il = bi * 32 + hi8 + 8 * r;
+      const int i = I0 + il, j = J0 + jl;
+      if (i >= ngp || j >= ngp || i >= j) continue;
+      const size_t lw = (size_t)j + (size_t)i * ldh;
+      H[lw] = sH[lo][hi8 + 8 * r];
+      O[lw] = sO[lo][hi8 + 8 * r];
+    }
+    __syncthreads();
   }
 }
 

@@ -18,11 +18,22 @@ module exh_lapw
   private
 
   integer(c_int), parameter, public :: EXH_ACCUMULATE = 1
+  ! EXH_MATCH of include/exh.h (Fortran names are case-blind: exh_match is the
+  ! function): batch flag that runs match first
+  integer(c_int), parameter, public :: EXH_BATCH_MATCH = 2
+  integer(c_int), parameter, public :: EXH_GRAPH = 4   ! batch: capture / replay a HIP graph
+
+  ! One k-point of a batch (exh_kpoint in include/exh.h); device pointers.
+  type, bind(C), public :: exh_kpoint
+    integer(c_int) :: ngp, rank
+    type(c_ptr) :: gpc, tpgpc, sfacgp, apwalm, igpig, vgpc, h, o
+  end type
 
   public :: exh_create, exh_destroy, exh_set_basis, exh_set_radial, exh_set_mt, &
             exh_set_gvectors, exh_set_interstitial, exh_match, &
             exh_hamiltonandoverlapsetup, exh_hamiltonandoverlapsetup_ex, &
-            exh_error_message
+            exh_error_message, exh_split, exh_setup_batch_device, exh_tune_batch, &
+            exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -118,6 +129,42 @@ module exh_lapw
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s
     end function
+
+    ! Batched k-points of one structure (scf_cycle.f90:256-296 k loop).
+    integer(c_int) function exh_setup_batch_device(ctx, stream, nk, kpts, flags) &
+        bind(C, name="exh_setup_batch_device")
+      import :: c_int, c_ptr, exh_kpoint
+      type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: nk, flags
+      type(exh_kpoint), intent(in) :: kpts(*)
+    end function
+
+    integer(c_int) function exh_tune_batch(ctx, stream, nk, kpts, flags, plan) &
+        bind(C, name="exh_tune_batch")
+      import :: c_int, c_ptr, exh_kpoint
+      type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: nk, flags
+      type(exh_kpoint), intent(in) :: kpts(*)
+      integer(c_int), intent(out) :: plan
+    end function
+
+    integer(c_int) function exh_set_batch_plan(ctx, plan) bind(C, name="exh_set_batch_plan")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: plan
+    end function
+
+    integer(c_int) function exh_get_batch_plan(ctx) bind(C, name="exh_get_batch_plan")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function exh_set_batch_workspace(ctx, nbytes) &
+        bind(C, name="exh_set_batch_workspace")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_size_t), value :: nbytes
+    end function
   end interface
 
 contains
@@ -136,6 +183,12 @@ contains
     do i = 1, n
       msg(i:i) = chars(i)
     end do
+  end function
+
+  ! EXH_SPLIT(n) of include/exh.h: split-precision flag with n int8 moduli.
+  pure integer(c_int) function exh_split(n)
+    integer, intent(in) :: n
+    exh_split = ishft(int(n, c_int), 8)
   end function
 
 end module exh_lapw
