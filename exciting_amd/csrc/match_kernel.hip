@@ -10,7 +10,7 @@
 // Writes apwalm(G, io, lm, atom): consecutive threads write consecutive G,
 // so every store is a coalesced 1 KB row segment.  The kernel is bound by
 // those writes (16 B x G x p x (lmax+1)^2 x atoms).
-#include "exh_internal.h"
+#include "exh_device.h"
 
 namespace exh {
 namespace {
@@ -177,11 +177,10 @@ __device__ __forceinline__ void d_sbesseldm(int m, double x,
   }
 }
 
-template <int LM>
+template <int LM, typename PD, typename PS, typename PA>
 __device__ __forceinline__ void match_body(
-    int g, int ias, int ngp, int ld, int pmax, const double *__restrict__ gpc,
-    const double *__restrict__ tpgpc, const double2 *__restrict__ sfacgp,
-    double2 *__restrict__ apwalm, const int *__restrict__ atom_species,
+    int g, int ias, int ngp, int ld, int pmax, PD __restrict__ gpc, PD __restrict__ tpgpc,
+    PS __restrict__ sfacgp, PA __restrict__ apwalm, const int *__restrict__ atom_species,
     const double *__restrict__ rmt, const int *__restrict__ apword,
     const int *__restrict__ omaxs, const double *__restrict__ lu,
     const int *__restrict__ piv, double fpso) {
@@ -227,8 +226,8 @@ __device__ __forceinline__ void match_body(
     zr[m] = cos(t1);
     zi[m] = sin(t1);
   }
-  const double2 S = sfacgp[g + (size_t)ld * ias];
-  double2 *out = apwalm + g + (size_t)ld * pmax * LMM * ias;
+  const double2 S = exh::ld(sfacgp + g + (size_t)ld * ias);
+  const PA out = apwalm + g + (size_t)ld * pmax * LMM * ias;
 
 #pragma unroll
   for (int l = 0; l <= LM; l++) {
@@ -344,7 +343,7 @@ __device__ __forceinline__ void match_body(
           double2 v;
           v.x = br[io];
           v.y = bi[io];
-          out[(size_t)ld * (io + (size_t)pmax * lm)] = v;
+          st(out + (size_t)ld * (io + (size_t)pmax * lm), v);
         }
     }
   }
@@ -370,8 +369,9 @@ __global__ __launch_bounds__(64) void match_grouped_kernel(
     const int *__restrict__ omaxs, const double *__restrict__ lu,
     const int *__restrict__ piv, double fpso) {
   const KDesc &d = kd[blockIdx.z];
-  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, ld, pmax, d.gpc, d.tpgpc,
-                 d.sfac, d.apwalm, atom_species, rmt, apword, omaxs, lu, piv, fpso);
+  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, ld, pmax, gptr(d.gpc),
+                 gptr(d.tpgpc), gptr(d.sfac), gptr(d.apwalm), atom_species, rmt, apword, omaxs,
+                 lu, piv, fpso);
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }

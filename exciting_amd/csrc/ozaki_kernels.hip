@@ -257,12 +257,15 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(int npairs, int nslab, int 
 }
 
 // Garner with symmetric digits: the integer x in (-M/2, M/2) with residues r_k.
-__device__ __forceinline__ double garner(const int *r, int nmod) {
-  int v[kOzMaxMod];
+template <int NM>
+__device__ __forceinline__ double garner(const int *r) {
+  int v[NM];
   double x = 0.0;
-  for (int k = 0; k < nmod; k++) {
+#pragma unroll
+  for (int k = 0; k < NM; k++) {
     const int p = c_p[k];
     int acc = r[k] * c_w[k];
+#pragma unroll
     for (int j = 0; j < k; j++) acc -= v[j] * c_c[j][k];
     int q = (int)floorf((float)acc * c_rp[k]);
     int d = acc - q * p;
@@ -271,7 +274,8 @@ __device__ __forceinline__ double garner(const int *r, int nmod) {
     if (d > (p >> 1)) d -= p;
     v[k] = d;
   }
-  for (int k = nmod - 1; k >= 0; k--) x = fma(x, (double)c_p[k], (double)v[k]);
+#pragma unroll
+  for (int k = NM - 1; k >= 0; k--) x = fma(x, (double)c_p[k], (double)v[k]);
   return x;
 }
 
@@ -288,8 +292,9 @@ __device__ __forceinline__ int modp(int x, int k) {
 // triangle (coalesced along i); the sub-block is staged in LDS and written
 // back transposed and conjugated to the lower triangle (coalesced along j).
 // ---------------------------------------------------------------------------
+template <int NM>
 __global__ __launch_bounds__(256) void ozcombine_kernel(
-    int ngp, int npairs, int nmod, const uint8_t *__restrict__ R, const int *__restrict__ ex,
+    int ngp, int npairs, const uint8_t *__restrict__ R, const int *__restrict__ ex,
     const int *__restrict__ eb, int bits, IstBox box, double2 *__restrict__ H,
     double2 *__restrict__ O, long long ldh, int accumulate) {
   const int tile = blockIdx.x;
@@ -310,12 +315,13 @@ __global__ __launch_bounds__(256) void ozcombine_kernel(
       const int i = I0 + il, j = J0 + jl;
       if (i >= ngp || j >= ngp || i > j) continue;
       const int e = il + OZ_T * jl;
-      int hre[kOzMaxMod], him[kOzMaxMod], ore[kOzMaxMod], oim[kOzMaxMod];
-      for (int k = 0; k < nmod; k++) {
+      int hre[NM], him[NM], ore[NM], oim[NM];
+#pragma unroll
+      for (int k = 0; k < NM; k++) {
         int rr[6];
 #pragma unroll
         for (int pr = 0; pr < 6; pr++)
-          rr[pr] = R[((size_t)(pr * nmod + k) * npairs + tile) * TT + e];
+          rr[pr] = R[((size_t)(pr * NM + k) * npairs + tile) * TT + e];
         hre[k] = modp(rr[0] + rr[1], k);
         him[k] = modp(rr[0] - rr[1] - rr[2], k);
         ore[k] = modp(rr[3] + rr[4], k);
@@ -324,10 +330,10 @@ __global__ __launch_bounds__(256) void ozcombine_kernel(
       const int si = ex[i] == INT_MIN ? 0 : bits - ex[i];
       const int sjb = eb[j] == INT_MIN ? 0 : bits - eb[j];
       const int sjx = ex[j] == INT_MIN ? 0 : bits - ex[j];
-      double hr = ldexp(garner(hre, nmod), -(si + sjb));
-      double hv = ldexp(garner(him, nmod), -(si + sjb));
-      double orr = ldexp(garner(ore, nmod), -(si + sjx));
-      double oi = ldexp(garner(oim, nmod), -(si + sjx));
+      double hr = ldexp(garner<NM>(hre), -(si + sjb));
+      double hv = ldexp(garner<NM>(him), -(si + sjb));
+      double orr = ldexp(garner<NM>(ore), -(si + sjx));
+      double oi = ldexp(garner<NM>(oim), -(si + sjx));
       const size_t up = (size_t)i + (size_t)j * ldh;
       if (accumulate) {
         const double2 h0 = H[up], o0 = O[up];
@@ -465,8 +471,17 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
     if (e != hipSuccess) return e;
   }
   if (stage_mask & 4) {
-    hipLaunchKernelGGL(ozcombine_kernel, dim3((unsigned)npairs), dim3(256), 0, s, ngp,
-                       (int)npairs, nmod, R, ex, eb, bits, box, h, o, (long long)ldh, accumulate);
+    switch (nmod) {
+#define EXH_OZC(N)                                                                          \
+  case N:                                                                                   \
+    hipLaunchKernelGGL(ozcombine_kernel<N>, dim3((unsigned)npairs), dim3(256), 0, s, ngp,     \
+                       (int)npairs, R, ex, eb, bits, box, h, o, (long long)ldh, accumulate); \
+    break;
+      EXH_OZC(6) EXH_OZC(7) EXH_OZC(8) EXH_OZC(9) EXH_OZC(10) EXH_OZC(11) EXH_OZC(12)
+#undef EXH_OZC
+      default:
+        return hipErrorInvalidValue;
+    }
   }
   return hipGetLastError();
 }

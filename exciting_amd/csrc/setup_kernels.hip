@@ -32,9 +32,10 @@ constexpr int NTH = 256;  // 4 waves; wave w owns a 32x32 quadrant
 // ---------------------------------------------------------------------------
 // ivg(:, igpig(i)) for the G+k set; w = 0 flags an igpig outside 1..ngrtot.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void ivgk_body(int i, int ngp, const int *__restrict__ igpig,
+template <typename PI, typename PO>
+__device__ __forceinline__ void ivgk_body(int i, int ngp, PI __restrict__ igpig,
                                           const int *__restrict__ ivg, int ngrtot,
-                                          int4 *__restrict__ out) {
+                                          PO __restrict__ out) {
   if (i >= ngp) return;
   int g = igpig[i] - 1;
   int4 v = make_int4(0, 0, 0, 0);
@@ -44,7 +45,7 @@ __device__ __forceinline__ void ivgk_body(int i, int ngp, const int *__restrict_
     v.z = ivg[3 * g + 2];
     v.w = 1;
   }
-  out[i] = v;
+  st(out + i, v);
 }
 
 __global__ void ivgk_kernel(int ngp, const int *__restrict__ igpig,
@@ -57,7 +58,8 @@ __global__ void ivgk_kernel(int ngp, const int *__restrict__ igpig,
 __global__ void ivgk_grouped_kernel(const KDesc *__restrict__ kd, const int *__restrict__ ivg,
                                     int ngrtot) {
   const KDesc &d = kd[blockIdx.y];
-  ivgk_body(blockIdx.x * blockDim.x + threadIdx.x, d.ngp, d.igpig, ivg, ngrtot, d.ivgk);
+  ivgk_body(blockIdx.x * blockDim.x + threadIdx.x, d.ngp, gptr(d.igpig), ivg, ngrtot,
+            gptr(d.ivgk));
 }
 
 // ---------------------------------------------------------------------------
@@ -108,7 +110,7 @@ __global__ void pack_grouped_kernel(const KDesc *__restrict__ kd,
   const KDesc &d = kd[blockIdx.z];
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
-  if (g < d.ngp) d.at[(size_t)k * ldb + g] = d.apwalm[colA[k] + g];
+  if (g < d.ngp) st(gptr(d.at) + (size_t)k * ldb + g, ld(gptr((const double2 *)d.apwalm) + colA[k] + g));
 }
 
 // ---------------------------------------------------------------------------
@@ -116,10 +118,11 @@ __global__ void pack_grouped_kernel(const KDesc *__restrict__ kd,
 // (hamiltonsetup.f90:75-88, zm = aa * apwi).  X holds A^T K-major with
 // leading dimension ldx.  Grid (G tiles, f tiles, atoms).
 // ---------------------------------------------------------------------------
+template <typename PX, typename PB>
 __device__ __forceinline__ void bt_body(
-    int ias, int I0, int F0, int ngp, const double2 *__restrict__ X, long long ldx,
+    int ias, int I0, int F0, int ngp, PX __restrict__ X, long long ldx,
     const int *__restrict__ Koff, const int *__restrict__ Ms,
-    const double2 *__restrict__ aa, int maxaa, double2 *__restrict__ bt, int ldb) {
+    const double2 *__restrict__ aa, int maxaa, PB __restrict__ bt, int ldb) {
   const int M = Ms[ias];
   if (F0 >= M || I0 >= ngp) return;
   const int koff = Koff[ias];
@@ -130,7 +133,7 @@ __device__ __forceinline__ void bt_body(
   const int wr = w >> 1, wc = w & 1, lr = lane & 15, lk = lane >> 4;
   const int ii = tid & 63, kq = tid >> 6;
   const bool vi = I0 + ii < ngp, vf = F0 + ii < M;
-  const double2 *xp = X + (size_t)koff * ldx + (vi ? I0 + ii : ngp - 1);
+  const PX xp = X + (size_t)koff * ldx + (vi ? I0 + ii : ngp - 1);
   const double2 *yp = aat + (vf ? F0 + ii : M - 1);
 
   dx4 cre[2][2], cim[2][2];
@@ -142,17 +145,20 @@ __device__ __forceinline__ void bt_body(
       cim[a][b] = dx4{0, 0, 0, 0};
     }
 
+  // the validity factors are applied at the LDS store, after the slab's
+  // MFMAs, so the prefetch loads stay in flight meanwhile
   double2 px[4], py[4];
+  double fxv[4], fyv[4];
   auto load = [&](int kb) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int k = kb + kq + 4 * q;
       const bool kv = k < M;
       const int kc = kv ? k : M - 1;
-      const double2 x = xp[(size_t)kc * ldx];
-      const double2 y = yp[(size_t)maxaa * kc];
-      px[q] = (kv && vi) ? x : z2(0, 0);
-      py[q] = (kv && vf) ? y : z2(0, 0);
+      px[q] = ld(xp + (size_t)kc * ldx);
+      py[q] = yp[(size_t)maxaa * kc];
+      fxv[q] = (kv && vi) ? 1.0 : 0.0;
+      fyv[q] = (kv && vf) ? 1.0 : 0.0;
     }
   };
   load(0);
@@ -160,10 +166,10 @@ __device__ __forceinline__ void bt_body(
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int kk = kq + 4 * q;
-      sXr[kk][ii] = px[q].x;
-      sXi[kk][ii] = px[q].y;
-      sYr[kk][ii] = py[q].x;
-      sYi[kk][ii] = py[q].y;
+      sXr[kk][ii] = fxv[q] * px[q].x;
+      sXi[kk][ii] = fxv[q] * px[q].y;
+      sYr[kk][ii] = fyv[q] * py[q].x;
+      sYi[kk][ii] = fyv[q] * py[q].y;
     }
     __syncthreads();
     if (kb + TK < M) load(kb + TK);
@@ -203,7 +209,7 @@ __device__ __forceinline__ void bt_body(
         int i = I0 + wr * 32 + a * 16 + lk + 4 * r;
         int f = F0 + wc * 32 + b * 16 + lr;
         if (i < ngp && f < M)
-          bt[(size_t)(koff + f) * ldb + i] = z2(cre[a][b][r], cim[a][b][r]);
+          st(bt + (size_t)(koff + f) * ldb + i, z2(cre[a][b][r], cim[a][b][r]));
       }
 }
 
@@ -221,8 +227,8 @@ __global__ __launch_bounds__(NTH) void bt_grouped_kernel(
     const int *__restrict__ Ms, const double2 *__restrict__ aa, int maxaa, int ldb) {
   const int k = blockIdx.z / natmtot, ias = blockIdx.z - k * natmtot;
   const KDesc &d = kd[k];
-  bt_body(ias, blockIdx.x * TM, blockIdx.y * TM, d.ngp, d.x, ldx, Koff, Ms, aa, maxaa, d.bt,
-          ldb);
+  bt_body(ias, blockIdx.x * TM, blockIdx.y * TM, d.ngp, gptr(d.x), ldx, Koff, Ms, aa, maxaa,
+          gptr(d.bt), ldb);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,11 +236,12 @@ __global__ __launch_bounds__(NTH) void bt_grouped_kernel(
 // (overlapsetup.f90:82-94).  h1aa(io1, io2, l, ias) couples the radial
 // orders of one (l, m).  One thread per (G, atom, f2).
 // ---------------------------------------------------------------------------
+template <typename PA, typename PT>
 __device__ __forceinline__ void a2t_body(
-    int g, int ias, int f2, int ngp, const double2 *__restrict__ apwalm,
+    int g, int ias, int f2, int ngp, PA __restrict__ apwalm,
     const long long *__restrict__ colA, const int *__restrict__ Koff,
     const int *__restrict__ Ms, const int *__restrict__ kinfo, const double *__restrict__ h1aa,
-    int pmax, int lmaxapw, double2 *__restrict__ a2t, int ldb) {
+    int pmax, int lmaxapw, PT __restrict__ a2t, int ldb) {
   if (g >= ngp || f2 >= Ms[ias]) return;
   int koff = Koff[ias];
   // kinfo[k] = io | p << 4 | l << 8 | (first f of this (l,m)) << 16
@@ -243,14 +250,14 @@ __device__ __forceinline__ void a2t_body(
   double2 v = z2(0, 0);
   for (int io1 = 0; io1 < p; io1++) {
     double hv = h1aa[io1 + pmax * (io2 + (size_t)pmax * (l + (size_t)(lmaxapw + 1) * ias))];
-    double2 a = apwalm[colA[koff + fb + io1] + g];
+    double2 a = ld(apwalm + colA[koff + fb + io1] + g);
     v.x += hv * a.x;
     v.y += hv * a.y;
   }
-  double2 a = apwalm[colA[koff + f2] + g];
+  double2 a = ld(apwalm + colA[koff + f2] + g);
   v.x += a.x;
   v.y += a.y;
-  a2t[(size_t)(koff + f2) * ldb + g] = v;
+  st(a2t + (size_t)(koff + f2) * ldb + g, v);
 }
 
 __global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apwalm,
@@ -272,8 +279,9 @@ __global__ void a2t_grouped_kernel(const KDesc *__restrict__ kd, int natmtot,
                                    int ldb) {
   const int k = blockIdx.z / natmtot, ias = blockIdx.z - k * natmtot;
   const KDesc &d = kd[k];
-  a2t_body(blockIdx.x * blockDim.x + threadIdx.x, ias, blockIdx.y, d.ngp, d.apwalm, colA, Koff,
-           Ms, kinfo, h1aa, pmax, lmaxapw, d.a2t, ldb);
+  a2t_body(blockIdx.x * blockDim.x + threadIdx.x, ias, blockIdx.y, d.ngp,
+           gptr((const double2 *)d.apwalm), colA, Koff, Ms, kinfo, h1aa, pmax, lmaxapw,
+           gptr(d.a2t), ldb);
 }
 
 // ---------------------------------------------------------------------------
@@ -305,12 +313,11 @@ struct HoShape {
 };
 
 // One TMI x TMJ upper tile (I0, J0) of H and O for one k-point.
-template <int TMI, int TMJ, int TK>
+template <int TMI, int TMJ, int TK, typename PX, typename PH, typename B>
 __device__ __forceinline__ void ho_tile(
-    int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
-    const double2 *__restrict__ YB, long long ldb, const double2 *__restrict__ YA,
-    long long lda, const IstBox &box, double2 *__restrict__ H, double2 *__restrict__ O,
-    long long ldh, int accumulate, int I0, int J0) {
+    int ngp, int Ktot, PX __restrict__ X, long long ldx, PX __restrict__ YB, long long ldb,
+    PX __restrict__ YA, long long lda, const B &box, PH __restrict__ H,
+    PH __restrict__ O, long long ldh, int accumulate, int I0, int J0) {
   using S = HoShape<TMI, TMJ, TK>;
   // [buffer][re, im][k][row] of X; [buffer][Br, Bi, Ar, Ai][k][row] of YB, YA
   __shared__ double sX[2][2][TK][S::PX];
@@ -320,9 +327,9 @@ __device__ __forceinline__ void ho_tile(
   const int xr_ = tid % TMI, xk = tid / TMI;
   const int yc = tid % TMJ, yk = tid / TMJ;
   const int ri = min(I0 + xr_, ngp - 1), rj = min(J0 + yc, ngp - 1);
-  const double2 *xp = X + ri;
-  const double2 *bp = YB + rj;
-  const double2 *ap = YA + rj;
+  const PX xp = X + ri;
+  const PX bp = YB + rj;
+  const PX ap = YA + rj;
 
   dx4 hre[2][2], him[2][2], ore[2][2], oim[2][2];
 #pragma unroll
@@ -337,43 +344,43 @@ __device__ __forceinline__ void ho_tile(
 
   // Rows at or past ngp only feed output rows/columns the epilogue never
   // writes, so addresses are clamped but values are not masked; the K tail
-  // is zeroed by a 0/1 factor so that every load is unconditional.
+  // is zeroed by a 0/1 factor so that every load is unconditional.  The
+  // factor is applied when the slab is stored to LDS, one slab after the
+  // load was issued, so the loads stay in flight across a whole slab of
+  // MFMAs instead of being waited for right away.
   double2 px[S::QX], pb[S::QY], pa[S::QY];
+  double fx[S::QX], fy[S::QY];
   auto load = [&](int kb) {
 #pragma unroll
     for (int q = 0; q < S::QX; q++) {
       const int k = kb + xk + S::RX * q;
-      const double f = k < Ktot ? 1.0 : 0.0;
+      fx[q] = k < Ktot ? 1.0 : 0.0;
       const size_t kc = k < Ktot ? k : Ktot - 1;
-      const double2 x = xp[kc * ldx];
-      px[q] = z2(f * x.x, f * x.y);
+      px[q] = ld(xp + kc * ldx);
     }
 #pragma unroll
     for (int q = 0; q < S::QY; q++) {
       const int k = kb + yk + S::RY * q;
-      const double f = k < Ktot ? 1.0 : 0.0;
+      fy[q] = k < Ktot ? 1.0 : 0.0;
       const size_t kc = k < Ktot ? k : Ktot - 1;
-      const double2 b = bp[kc * ldb];
-      const double2 a = ap[kc * lda];
-      pb[q] = z2(f * b.x, f * b.y);
-      pa[q] = z2(f * a.x, f * a.y);
+      pb[q] = ld(bp + kc * ldb);
+      pa[q] = ld(ap + kc * lda);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < S::QX; q++) {
       const int kk = xk + S::RX * q;
-      sX[buf][0][kk][xr_] = px[q].x;
-      sX[buf][1][kk][xr_] = px[q].y;
+      sX[buf][0][kk][xr_] = fx[q] * px[q].x;
+      sX[buf][1][kk][xr_] = fx[q] * px[q].y;
     }
 #pragma unroll
     for (int q = 0; q < S::QY; q++) {
       const int kk = yk + S::RY * q;
-      const double2 a = pa[q];
-      sY[buf][0][kk][yc] = pb[q].x;
-      sY[buf][1][kk][yc] = pb[q].y;
-      sY[buf][2][kk][yc] = a.x;
-      sY[buf][3][kk][yc] = a.y;
+      sY[buf][0][kk][yc] = fy[q] * pb[q].x;
+      sY[buf][1][kk][yc] = fy[q] * pb[q].y;
+      sY[buf][2][kk][yc] = fy[q] * pa[q].x;
+      sY[buf][3][kk][yc] = fy[q] * pa[q].y;
     }
   };
   auto compute = [&](int buf) {
@@ -427,6 +434,9 @@ __device__ __forceinline__ void ho_tile(
   load(TK);
   for (int sl = 0; sl < nslab; sl++) {
     compute(sl & 1);
+    // keep the scaling and LDS store of the prefetched slab after this
+    // slab's MFMAs, so its loads are waited for only here
+    __builtin_amdgcn_sched_barrier(0);
     store((sl + 1) & 1);
     load((sl + 2) * TK);
     __syncthreads();
@@ -449,7 +459,7 @@ __device__ __forceinline__ void ho_tile(
         double orr = ore[a][b][r], oi = oim[a][b][r];
         if (accumulate) {
           const size_t up = (size_t)i + (size_t)j * ldh;
-          const double2 h0 = H[up], o0 = O[up];
+          const double2 h0 = ld(H + up), o0 = ld(O + up);
           hr += h0.x;
           hi += h0.y;
           orr += o0.x;
@@ -501,11 +511,14 @@ __global__ __launch_bounds__(64 * (TMI / 32) * (TMJ / 32)) void ho_grouped_kerne
     const int I0 = __builtin_amdgcn_readfirstlane(e.y);
     const int J0 = __builtin_amdgcn_readfirstlane(e.z);
     const KDesc &d = kd[k];
-    IstBox box{d.ivgk, d.vgpc, T, lo0, lo1, lo2, n0, n1, n2};
-    const double2 *YA = use_a2t ? (const double2 *)d.a2t : d.x;
+    const IstBoxT<const EXH_G int4 *, const EXH_G double *> box{
+        gptr((const int4 *)d.ivgk), gptr(d.vgpc), T, lo0, lo1, lo2, n0, n1, n2};
+    const EXH_G double2 *X = gptr(d.x);
+    const EXH_G double2 *YB = gptr((const double2 *)d.bt);
+    const EXH_G double2 *YA = use_a2t ? gptr((const double2 *)d.a2t) : X;
     const long long lda = use_a2t ? ldb : ldx;
-    ho_tile<TMI, TMJ, TK>(d.ngp, Ktot, d.x, ldx, d.bt, ldb, YA, lda, box, d.h, d.o, d.ldh,
-                          accumulate, I0, J0);
+    ho_tile<TMI, TMJ, TK>(d.ngp, Ktot, X, ldx, YB, ldb, YA, lda, box, gptr(d.h), gptr(d.o),
+                          d.ldh, accumulate, I0, J0);
   }
 }
 
@@ -527,11 +540,10 @@ struct LoTables {
   const double *salo;
 };
 
-template <int MAXLO>
-__device__ __forceinline__ void lo_border_body(int g, int ias, int ngp,
-                                               const double2 *__restrict__ apwalm,
-                                               const LoTables &t, double2 *__restrict__ H,
-                                               double2 *__restrict__ O, long long ldh,
+template <int MAXLO, typename PA, typename PH>
+__device__ __forceinline__ void lo_border_body(int g, int ias, int ngp, PA __restrict__ apwalm,
+                                               const LoTables &t, PH __restrict__ H,
+                                               PH __restrict__ O, long long ldh,
                                                int accumulate) {
   const long long *__restrict__ colA = t.colA;
   const int *__restrict__ nlorb = t.nlorb;
@@ -557,7 +569,7 @@ __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp,
 #pragma unroll
   for (int r = 0; r < MAXLO; r++) acc[r] = z2(0, 0);
   for (int f = 0; f < M; f++) {
-    const double2 a = apwalm[col[f] + g];
+    const double2 a = ld(apwalm + col[f] + g);
     const double2 *Lf = L + (size_t)maxnlo * f;
 #pragma unroll
     for (int r = 0; r < MAXLO; r++)
@@ -573,12 +585,12 @@ __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp,
       const size_t e = (size_t)(r0 + r) + (size_t)g * ldh;
       double2 v = acc[r];
       if (accumulate) {
-        double2 h0 = H[e];
+        double2 h0 = ld(H + e);
         v.x += h0.x;
         v.y += h0.y;
       }
-      H[e] = v;
-      H[(size_t)g + (size_t)(r0 + r) * ldh] = z2(v.x, -v.y);
+      st(H + e, v);
+      st(H + (size_t)g + (size_t)(r0 + r) * ldh, z2(v.x, -v.y));
     }
 
   int c = r0;
@@ -588,15 +600,16 @@ __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp,
     for (int mi = 0; mi <= 2 * l; mi++, c++) {
       const int lm = l * l + mi;
       const size_t e = (size_t)g + (size_t)c * ldh;
-      double2 v = accumulate ? O[e] : z2(0, 0);
+      double2 v = accumulate ? ld(O + e) : z2(0, 0);
       for (int io = 0; io < p; io++) {
-        const double2 a = apwalm[g + (size_t)ngkmax * (io + (size_t)pmax * (lm + (size_t)lmmax * ias))];
+        const double2 a =
+            ld(apwalm + g + (size_t)ngkmax * (io + (size_t)pmax * (lm + (size_t)lmmax * ias)));
         const double s = salo[io + pmax * (ilo + (size_t)nlomax * ias)];
         v.x += a.x * s;
         v.y += -(a.y * s);
       }
-      O[e] = v;
-      O[(size_t)c + (size_t)g * ldh] = z2(v.x, -v.y);
+      st(O + e, v);
+      st(O + (size_t)c + (size_t)g * ldh, z2(v.x, -v.y));
     }
   }
 }
@@ -615,8 +628,9 @@ template <int MAXLO>
 __global__ __launch_bounds__(64) void lo_border_grouped_kernel(const KDesc *__restrict__ kd,
                                                                LoTables t, int accumulate) {
   const KDesc &d = kd[blockIdx.z];
-  lo_border_body<MAXLO>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, d.apwalm, t, d.h,
-                        d.o, d.ldh, accumulate);
+  lo_border_body<MAXLO>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp,
+                        gptr((const double2 *)d.apwalm), t, gptr(d.h), gptr(d.o), d.ldh,
+                        accumulate);
 }
 
 // ---------------------------------------------------------------------------
@@ -624,20 +638,21 @@ __global__ __launch_bounds__(64) void lo_border_grouped_kernel(const KDesc *__re
 // whole nlotot x nlotot corner: H += lolo (same atom), O += ololo + h1lolo
 // (same atom, same l, same m).  Upper threads write both triangles.
 // ---------------------------------------------------------------------------
+template <typename PH>
 __device__ __forceinline__ void lolo_body(int r, int c, int ngp, int nlotot,
                                           const LoRow *__restrict__ lorow,
                                           const int *__restrict__ lobase,
                                           const double2 *__restrict__ lolo, int maxnlo,
                                           const double *__restrict__ ololo_eff, int nlomax,
-                                          double2 *__restrict__ H, double2 *__restrict__ O,
-                                          long long ldh, int accumulate) {
+                                          PH __restrict__ H, PH __restrict__ O, long long ldh,
+                                          int accumulate) {
   if (r >= nlotot || c >= nlotot || r > c) return;
   const LoRow P = lorow[r], Q = lorow[c];
   const size_t up = (size_t)(ngp + r) + (size_t)(ngp + c) * ldh;
   double2 hv = z2(0, 0), ov = z2(0, 0);
   if (accumulate) {
-    hv = H[up];
-    ov = O[up];
+    hv = ld(H + up);
+    ov = ld(O + up);
   }
   if (P.ias == Q.ias) {
     const int b = lobase[P.ias];
@@ -647,12 +662,12 @@ __device__ __forceinline__ void lolo_body(int r, int c, int ngp, int nlotot,
     if (P.l == Q.l && P.mi == Q.mi)
       ov.x += ololo_eff[P.ilo + nlomax * (Q.ilo + (size_t)nlomax * P.ias)];
   }
-  H[up] = hv;
-  O[up] = ov;
+  st(H + up, hv);
+  st(O + up, ov);
   if (r != c) {
     const size_t lo = (size_t)(ngp + c) + (size_t)(ngp + r) * ldh;
-    H[lo] = z2(hv.x, -hv.y);
-    O[lo] = z2(ov.x, -ov.y);
+    st(H + lo, z2(hv.x, -hv.y));
+    st(O + lo, z2(ov.x, -ov.y));
   }
 }
 
@@ -674,8 +689,8 @@ __global__ void lolo_grouped_kernel(const KDesc *__restrict__ kd, int nlotot,
                                     int accumulate) {
   const KDesc &d = kd[blockIdx.z];
   lolo_body(blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4), d.ngp,
-            nlotot, lorow, lobase, lolo, maxnlo, ololo_eff, nlomax, d.h, d.o, d.ldh,
-            accumulate);
+            nlotot, lorow, lobase, lolo, maxnlo, ololo_eff, nlomax, gptr(d.h), gptr(d.o),
+            d.ldh, accumulate);
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }

@@ -200,6 +200,7 @@ class Context:
         self.ngkmax = 0
 
     def close(self):
+        self._tstream = None
         if getattr(self, "_h", None):
             lib().exh_destroy(self._h)
             self._h = None
@@ -218,6 +219,24 @@ class Context:
     @property
     def stream(self) -> int:
         return lib().exh_stream(self._h)
+
+    def _device_call(self, stream, launch):
+        """Run launch(stream handle) for a _device entry point.  With an
+        explicit stream the caller orders the work.  Without one the work
+        runs on the context's stream, ordered after everything already queued
+        on PyTorch's current stream (which produced the inputs) and before
+        anything queued there afterwards (which reads the outputs)."""
+        if stream is not None:
+            return launch(stream)
+        import torch
+        if getattr(self, "_tstream", None) is None:
+            self._tstream = torch.cuda.ExternalStream(self.stream,
+                                                      device=f"cuda:{self.device}")
+        cur = torch.cuda.current_stream(self.device)
+        self._tstream.wait_stream(cur)
+        rc = launch(self.stream)
+        cur.wait_stream(self._tstream)
+        return rc
 
     # ---- instrumentation ------------------------------------------------
     def profile(self, on: bool = True):
@@ -292,8 +311,8 @@ class Context:
     def match(self, ngp, gpc, tpgpc, sfacgp, apwalm, stream=None):
         """match.f90:13-194: fills apwalm (NumPy Fortran array or CUDA tensor)."""
         if _is_torch(apwalm):
-            rc = lib().exh_match_device(self._h, stream, ngp, _ptr(gpc), _ptr(tpgpc),
-                                        _ptr(sfacgp), _ptr(apwalm))
+            rc = self._device_call(stream, lambda st: lib().exh_match_device(
+                self._h, st, ngp, _ptr(gpc), _ptr(tpgpc), _ptr(sfacgp), _ptr(apwalm)))
             self._check(rc, "exh_match_device")
             return apwalm
         if not (isinstance(apwalm, np.ndarray) and apwalm.flags.f_contiguous
@@ -312,8 +331,9 @@ class Context:
         h, o = system.hamilton.za, system.overlap.za
         rank = system.hamilton.rank
         if _is_torch(h):
-            rc = lib().exh_setup_device(self._h, stream, ngp, rank, _ptr(apwalm), _ptr(igpig),
-                                        _ptr(vgpc), _ptr(h), _ptr(o), flags)
+            rc = self._device_call(stream, lambda st: lib().exh_setup_device(
+                self._h, st, ngp, rank, _ptr(apwalm), _ptr(igpig), _ptr(vgpc), _ptr(h), _ptr(o),
+                flags))
             self._check(rc, "exh_setup_device")
             return system
         for m in (h, o):
@@ -338,8 +358,8 @@ class Context:
         """Grouped match (with MATCH) + H/O setup over a list of KPoint
         structures (device pointers; see ``kpoint_struct``)."""
         arr = self._kpoint_array(kpts)
-        self._check(lib().exh_setup_batch_device(self._h, stream, len(kpts), arr, flags),
-                    "exh_setup_batch_device")
+        self._check(self._device_call(stream, lambda st: lib().exh_setup_batch_device(
+            self._h, st, len(kpts), arr, flags)), "exh_setup_batch_device")
         return arr
 
     def tune_batch(self, kpts, flags: int = 0, stream=None) -> int:
@@ -347,8 +367,8 @@ class Context:
         and O are scratch; on return they hold the non-accumulated result."""
         arr = self._kpoint_array(kpts)
         best = _i(-1)
-        self._check(lib().exh_tune_batch(self._h, stream, len(kpts), arr, flags,
-                                         C.byref(best)), "exh_tune_batch")
+        self._check(self._device_call(stream, lambda st: lib().exh_tune_batch(
+            self._h, st, len(kpts), arr, flags, C.byref(best))), "exh_tune_batch")
         return best.value
 
     @property
