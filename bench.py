@@ -80,7 +80,20 @@ def oz_bits(nmod, K):
     return int(min(30, math.floor((lm - 1.0 - math.log2(6.0 * K)) * 0.5)))
 
 
-def accuracy(ctx, kp, dk, H, O, nmod, nst):
+def pick_moduli(W, arg, nst):
+    """Modulus count from --moduli: an integer, or 'auto' (the precision
+    tuner on k-point 0, untimed).  Returns (nmod, tuning record or None)."""
+    if str(arg) != "auto":
+        return int(arg), None
+    from exciting_amd import precision
+    t0 = time.perf_counter()
+    res = precision.tune_moduli(W.ctx, W.kp[0], W.dks[0].rank, W.H[0], W.O[0], nst or 400)
+    res["seconds"] = time.perf_counter() - t0
+    log(f"moduli tuner: {res}")
+    return (res["nmod"] or 12), res
+
+
+def accuracy(ctx, kp, dk, H, O, nmod, nst, kpoint=0):
     """Split vs FP64 on one k-point of a batch: relative Frobenius errors of H
     and O and the largest difference among the lowest nst generalized
     eigenvalues, both solved in FP64 on the GPU (Cholesky reduction +
@@ -100,7 +113,7 @@ def accuracy(ctx, kp, dk, H, O, nmod, nst):
     ws = solve.geneig(H, O, n, nst).cpu().numpy()
     de = float(np.abs(ws - w64).max())
     del H64, O64
-    return {"kpoint": 0, "n": n, "nstates": nst, "rel_frob_H": rh, "rel_frob_O": ro,
+    return {"kpoint": kpoint, "n": n, "nstates": nst, "rel_frob_H": rh, "rel_frob_O": ro,
             "max_abs_dE_Ha": de, "within_1uHa": bool(de <= 1e-6),
             "solver": "GPU FP64: Cholesky of O, two triangular solves, Hermitian eigensolve",
             "eig_seconds": time.perf_counter() - t0}
@@ -280,17 +293,17 @@ def serving_sample(args):
                                                f"nk={len(r.kpoints)} nmod={r.nmod} {r.status} "
                                                f"{r.latency:.2f} s"))
     warm = [serve.Request(rid=-1 - i, cell=c, kpoints=[0], nmod=m, nstfv=args.serve_nstfv)
-            for i, (c, m) in enumerate((c, m) for c in cells for m in (0, args.moduli))]
+            for i, (c, m) in enumerate((c, m) for c in cells for m in (0, args.serve_moduli))]
     srv.run(warm)
     reqs = serve.poisson_requests(args.serve_requests, args.serve_rate, cells, seed=7,
-                                  kmax=args.serve_kmax, nmod=args.moduli,
+                                  kmax=args.serve_kmax, nmod=args.serve_moduli,
                                   nstfv=args.serve_nstfv)
     t0 = time.perf_counter()
     srv.run(reqs)
     out = serve.summarize(reqs, time.perf_counter() - t0)
     out.update({"sample": (f"{args.serve_requests} Poisson requests at {args.serve_rate}/s over "
                            f"fcc2/sc8/ortho16/sc32/sc64, 1..{args.serve_kmax} k-points each, "
-                           f"half split ({args.moduli} moduli), {args.serve_nstfv} states"),
+                           f"half split ({args.serve_moduli} moduli), {args.serve_nstfv} states"),
                 "peak_inflight": srv.peak_inflight, "contexts": srv.pool.created})
     srv.pool.close_all()
     return out
@@ -304,8 +317,10 @@ def main():
     ap.add_argument("--config", type=int, default=3, help="headline workload index (0..3)")
     ap.add_argument("--kpoints", type=int, default=0, help="use the first N k-points (0 = all)")
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
-    ap.add_argument("--moduli", type=int, default=9,
-                    help="split precision: number of int8 moduli (0 = FP64 only)")
+    ap.add_argument("--moduli", default="auto",
+                    help="split precision: number of int8 moduli, 0 = FP64 only, or 'auto': "
+                         "the fewest moduli keeping the lowest eigenvalues of k-point 0 within "
+                         "1/3 uHa of FP64 (checked again on the last k-point)")
     ap.add_argument("--nstates", type=int, default=400,
                     help="lowest eigenvalues compared between split and FP64 (one k-point; "
                          "0 = skip the check)")
@@ -319,12 +334,15 @@ def main():
                     help="batch scratch cap (48 GiB holds the 16 headline k-points in one group)")
     ap.add_argument("--uneven", choices=("on", "off"), default="on",
                     help="also run the 16-atom cell with 128 uneven k-points as one batch")
-    ap.add_argument("--uneven-moduli", type=int, default=8)
+    ap.add_argument("--uneven-moduli", default="auto")
     ap.add_argument("--serve-requests", type=int, default=16,
                     help="requests in the serving sample (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=1.0)
     ap.add_argument("--serve-kmax", type=int, default=4)
     ap.add_argument("--serve-nstfv", type=int, default=40)
+    ap.add_argument("--serve-moduli", type=int, default=9,
+                    help="moduli of the split requests (9 keeps every cell of the mix within "
+                         "1 uHa)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -371,7 +389,7 @@ def main():
 
     head = fp64
     split = None
-    nmod = args.moduli
+    nmod, mod_tuning = pick_moduli(W, args.moduli, args.nstates)
     if nmod:
         split = W.run(lapw.SPLIT(nmod), f"split({nmod}) batch", args.steps, args.warmup)
         split["roofline"] = W.oz_roofline(split, nmod, f"ozgemm_sc64_split{nmod}.md")
@@ -380,9 +398,11 @@ def main():
         split["graph"] = {"value": g["value"], "ms_per_step": g["ms_per_step"]}
         split["moduli"] = nmod
         split["bits"] = oz_bits(nmod, W.K)
+        split["moduli_tuning"] = mod_tuning
         if args.nstates:
-            split["accuracy"] = accuracy(W.ctx, W.kp[0], W.dks[0], W.H[0], W.O[0], nmod,
-                                         args.nstates)
+            kc = nk - 1
+            split["accuracy"] = accuracy(W.ctx, W.kp[kc], W.dks[kc], W.H[kc], W.O[kc], nmod,
+                                         args.nstates, kpoint=kc)
             log(f"split accuracy {split['accuracy']}")
         head = split
 
@@ -420,7 +440,8 @@ def main():
     if fp64_loop:
         out["fp64_loop"] = clean(fp64_loop)
     if split:
-        out["split"] = {k: split[k] for k in ("value", "ms_per_step", "graph", "moduli", "bits")}
+        out["split"] = {k: split[k] for k in ("value", "ms_per_step", "graph", "moduli", "bits",
+                                              "moduli_tuning")}
         out["speedup_vs_fp64"] = split["value"] / fp64["value"]
         if fp64_loop:
             out["speedup_vs_fp64_loop"] = split["value"] / fp64_loop["value"]
@@ -435,9 +456,11 @@ def main():
         ub = U.run(0, "uneven fp64 batch", args.steps, args.warmup)
         ug = U.run(lapw.GRAPH, "uneven fp64 graph", args.steps, args.warmup, profile=False)
         ul = U.run(0, "uneven fp64 loop", args.steps, args.warmup, batch=False)
-        um = args.uneven_moduli
+        um, um_tuning = pick_moduli(U, args.uneven_moduli, args.nstates)
         us = U.run(lapw.SPLIT(um), f"uneven split({um}) batch", args.steps, args.warmup)
-        uacc = accuracy(U.ctx, U.kp[0], U.dks[0], U.H[0], U.O[0], um, args.nstates)
+        kc = U.nk - 1
+        uacc = accuracy(U.ctx, U.kp[kc], U.dks[kc], U.H[kc], U.O[kc], um, args.nstates or 400,
+                        kpoint=kc)
         out["uneven"] = {
             "workload": (f"{U.spec.name}: {U.s.natmtot}-atom orthorhombic cell, {U.nk} "
                          f"k-points, ngk {min(U.ngks)}..{max(U.ngks)}, one batch per step"),
@@ -449,6 +472,7 @@ def main():
             "ho_roofline": U.ho_roofline(ub, "ho_grouped", "ho_grouped_ortho16_fp64.md"),
             "split_batch": {k: us[k] for k in ("value", "ms_per_step")},
             "split_moduli": um,
+            "split_moduli_tuning": um_tuning,
             "split_vs_fp64_batch": us["value"] / ub["value"],
             "split_accuracy": uacc,
         }

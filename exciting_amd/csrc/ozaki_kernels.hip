@@ -158,20 +158,32 @@ __device__ __forceinline__ void store_residues(const v16i (&acc)[2][2], int p, f
       }
 }
 
-__global__ __launch_bounds__(256) void ozgemm_kernel(int npairs, int nslab, int Gpad, int nmod,
+constexpr int OZ_BLK = 8;  // tile-block edge of the GEMM walk order
+
+__global__ __launch_bounds__(256) void ozgemm_kernel(int nt, int nslab, int Gpad, int nmod,
                                                      const int8_t *__restrict__ QX,
                                                      const int8_t *__restrict__ QB,
                                                      size_t qstride, uint8_t *__restrict__ R) {
-  // XCD-contiguous remap over the whole grid, then (pair*nmod+m, tile)
+  // XCD-contiguous remap over the whole grid, then (pair*nmod+m, block,
+  // tile in block).  Each GEMM's upper triangle is walked in OZ_BLK x OZ_BLK
+  // blocks of tiles, so the ~64 workgroups resident on one XCD share
+  // OZ_BLK row panels and OZ_BLK column panels in its L2.
   const int bid = blockIdx.x, nb = gridDim.x;
   const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
   const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
-  const int qm = t / npairs, tile = t - qm * npairs;
+  const int nbl = (nt + OZ_BLK - 1) / OZ_BLK;
+  const int per = nbl * (nbl + 1) / 2 * (OZ_BLK * OZ_BLK);
+  const int qm = t / per, o = t - qm * per;
   const int q = qm / nmod, m = qm - q * nmod;
-  int J = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
-  while ((J + 1) * (J + 2) / 2 <= tile) J++;
-  while (J * (J + 1) / 2 > tile) J--;
-  const int I = tile - J * (J + 1) / 2;
+  const int blk = o / (OZ_BLK * OZ_BLK), l = o - blk * (OZ_BLK * OZ_BLK);
+  int BJ = (int)((sqrt(8.0 * (double)blk + 1.0) - 1.0) * 0.5);
+  while ((BJ + 1) * (BJ + 2) / 2 <= blk) BJ++;
+  while (BJ * (BJ + 1) / 2 > blk) BJ--;
+  const int BI = blk - BJ * (BJ + 1) / 2;
+  const int I = BI * OZ_BLK + l % OZ_BLK, J = BJ * OZ_BLK + l / OZ_BLK;
+  if (I > J || J >= nt) return;  // whole workgroup: padding of the block walk
+  const long long npairs = (long long)nt * (nt + 1) / 2;
+  const long long tile = (long long)J * (J + 1) / 2 + I;
   const int I0 = I * OZ_T, J0 = J * OZ_T;
 
   // rows: X array q of {xr, xi, xs}; H columns: B array q of {br, bi, bd};
@@ -193,7 +205,8 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(int npairs, int nslab, int 
       ao[a][b] = v16i{0};
     }
 
-  // global -> register prefetch: 2 chunks of 16 B per operand per thread
+  // global -> register prefetch one slab ahead: the loads of slab s + 2 are
+  // issued after slab s + 1 is stored and waited for a whole slab later
   v4i pa[2], ph[2], po[2];
   auto load = [&](int s) {
 #pragma unroll
@@ -464,8 +477,9 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
     if (e != hipSuccess) return e;
   }
   if (stage_mask & 2) {
-    const long long nblk = npairs * 3 * nmod;
-    hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)npairs, nslab,
+    const long long nbl = (nt + OZ_BLK - 1) / OZ_BLK;
+    const long long nblk = nbl * (nbl + 1) / 2 * (OZ_BLK * OZ_BLK) * 3 * nmod;
+    hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)nt, nslab,
                        L.Gpad, nmod, QX, QB, L.qstride, R);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
