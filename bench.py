@@ -57,7 +57,7 @@ def flops_ho(natm, M, G):
     return natm * 8.0 * G * (G + 1) * M
 
 
-def measured_traffic(profile: str, per: str = "launch"):
+def measured_traffic(profile: str, per: str = "kpoint"):
     """HBM-side bytes of one kernel from a committed rocprofv3 --pmc summary
     under profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by a calibration
     copy): the value of its ``traffic_bytes_per_<per>`` row, or None."""
@@ -257,11 +257,13 @@ class Workload:
     def ho_roofline(self, r, kernel, profile_md):
         ms, n = r["_prof"][kernel]
         achieved = r["_steps"] * sum(self.f_ho) / (ms * 1e-3) / 1e12
+        kpl = r["_steps"] * self.nk / n           # k-points per launch
+        per_k = measured_traffic(profile_md) if profile_md else None
         return {"kernel": f"{kernel} (fused upper-triangle H/O contraction + interstitial)",
-                "launches": n, "avg_launch_ms": ms / n,
+                "launches": n, "kpoints_per_launch": kpl, "avg_launch_ms": ms / n,
                 "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": measured_traffic(profile_md)}
+                "traffic": per_k * kpl if per_k else None}
 
     def oz_roofline(self, r, nmod, profile_md):
         ms, n = r["_prof"]["oz_gemm"]
@@ -382,17 +384,17 @@ def main():
     if args.loop == "on":
         fp64_loop = W.run(0, "fp64 loop", args.steps, args.warmup, batch=False)
     fp64 = W.run(0, "fp64 batch", args.steps, args.warmup)
-    fp64["roofline"] = W.ho_roofline(fp64, "ho_grouped", "ho_grouped_sc64_fp64.md")
+    fp64["roofline"] = W.ho_roofline(fp64, "ho_grouped", f"ho_grouped_{W.spec.name}_fp64.md")
     fp64["plan"] = dict(lapw.plan_info(plan), id=plan)
     if fp64_loop:
-        fp64_loop["roofline"] = W.ho_roofline(fp64_loop, "ho", "ho_kernel_sc64_fp64.md")
+        fp64_loop["roofline"] = W.ho_roofline(fp64_loop, "ho", None)
 
     head = fp64
     split = None
     nmod, mod_tuning = pick_moduli(W, args.moduli, args.nstates)
     if nmod:
         split = W.run(lapw.SPLIT(nmod), f"split({nmod}) batch", args.steps, args.warmup)
-        split["roofline"] = W.oz_roofline(split, nmod, f"ozgemm_sc64_split{nmod}.md")
+        split["roofline"] = W.oz_roofline(split, nmod, f"ozgemm_{W.spec.name}_split{nmod}.md")
         g = W.run(lapw.SPLIT(nmod) | lapw.GRAPH, f"split({nmod}) graph", args.steps,
                   args.warmup, profile=False)
         split["graph"] = {"value": g["value"], "ms_per_step": g["ms_per_step"]}
@@ -469,7 +471,7 @@ def main():
             "fp64_graph": {k: ug[k] for k in ("value", "ms_per_step")},
             "fp64_loop": {k: ul[k] for k in ("value", "ms_per_step", "gflops", "frac_fp64_peak")},
             "batch_vs_loop": ub["value"] / ul["value"],
-            "ho_roofline": U.ho_roofline(ub, "ho_grouped", "ho_grouped_ortho16_fp64.md"),
+            "ho_roofline": U.ho_roofline(ub, "ho_grouped", None),
             "split_batch": {k: us[k] for k in ("value", "ms_per_step")},
             "split_moduli": um,
             "split_moduli_tuning": um_tuning,
