@@ -158,43 +158,45 @@ __device__ __forceinline__ void store_residues(const v16i (&acc)[2][2], int p, f
       }
 }
 
-constexpr int OZ_BLK = 8;  // tile-block edge of the GEMM walk order
+// One workgroup computes the product pair (H_q, O_q) of one modulus for two
+// vertically stacked 128x128 tiles (I = 2*IP and 2*IP+1, one column block
+// J): a 256-row panel of the row operand against 128-column panels of the H
+// and O column operands, which the two halves share.  8 waves in two halves
+// of 2x2 waves, each wave 64x64 per product.  Tiles walk J-major: the pair
+// rows IP = 0..J/2 of column block J are consecutive, so co-resident
+// workgroups share the column panels in L2.
+constexpr int OZ_R = 2 * OZ_T;  // rows of the row panel per workgroup
 
-__global__ __launch_bounds__(256) void ozgemm_kernel(int nt, int nslab, int Gpad, int nmod,
+__global__ __launch_bounds__(512) void ozgemm_kernel(int nt, int nslab, int Gpad, int nmod,
                                                      const int8_t *__restrict__ QX,
                                                      const int8_t *__restrict__ QB,
                                                      size_t qstride, uint8_t *__restrict__ R) {
-  // XCD-contiguous remap over the whole grid, then (pair*nmod+m, block,
-  // tile in block).  Each GEMM's upper triangle is walked in OZ_BLK x OZ_BLK
-  // blocks of tiles, so the ~64 workgroups resident on one XCD share
-  // OZ_BLK row panels and OZ_BLK column panels in its L2.
+  // XCD-contiguous remap over the whole grid, then (pair*nmod+m, tile pair)
   const int bid = blockIdx.x, nb = gridDim.x;
   const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
   const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
-  const int nbl = (nt + OZ_BLK - 1) / OZ_BLK;
-  const int per = nbl * (nbl + 1) / 2 * (OZ_BLK * OZ_BLK);
+  // tile pairs per GEMM: column block J has J/2 + 1 of them; blocks 2c and
+  // 2c+1 together start at c(c+1)
+  const int ncb = (nt + 1) / 2;
+  const int per = ncb * (ncb + 1) - ((nt & 1) ? ncb : 0);
   const int qm = t / per, o = t - qm * per;
   const int q = qm / nmod, m = qm - q * nmod;
-  const int blk = o / (OZ_BLK * OZ_BLK), l = o - blk * (OZ_BLK * OZ_BLK);
-  int BJ = (int)((sqrt(8.0 * (double)blk + 1.0) - 1.0) * 0.5);
-  while ((BJ + 1) * (BJ + 2) / 2 <= blk) BJ++;
-  while (BJ * (BJ + 1) / 2 > blk) BJ--;
-  const int BI = blk - BJ * (BJ + 1) / 2;
-  const int I = BI * OZ_BLK + l % OZ_BLK, J = BJ * OZ_BLK + l / OZ_BLK;
-  if (I > J || J >= nt) return;  // whole workgroup: padding of the block walk
+  int c = (int)((sqrt(4.0 * (double)o + 1.0) - 1.0) * 0.5);
+  while ((c + 1) * (c + 2) <= o) c++;
+  while (c * (c + 1) > o) c--;
+  const int rr = o - c * (c + 1);
+  const int J = rr <= c ? 2 * c : 2 * c + 1;
+  const int IP = rr <= c ? rr : rr - (c + 1);
   const long long npairs = (long long)nt * (nt + 1) / 2;
-  const long long tile = (long long)J * (J + 1) / 2 + I;
-  const int I0 = I * OZ_T, J0 = J * OZ_T;
+  const int I0 = IP * OZ_R, J0 = J * OZ_T;
 
-  // rows: X array q of {xr, xi, xs}; H columns: B array q of {br, bi, bd};
-  // O columns: X array {xr, xi, xd}[q]
   const int8_t *Ab = QX + (size_t)(m * 4 + q) * qstride;
   const int8_t *Hb = QB + (size_t)(m * 3 + q) * qstride;
   const int8_t *Ob = QX + (size_t)(m * 4 + (q == 2 ? 3 : q)) * qstride;
 
-  __shared__ v4i sA[2][4][OZ_T], sH[2][4][OZ_T], sO[2][4][OZ_T];
+  __shared__ v4i sA[2][4][OZ_R], sH[2][4][OZ_T], sO[2][4][OZ_T];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wr = w >> 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
+  const int half = w >> 2, wr = (w >> 1) & 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
 
   v16i ah[2][2], ao[2][2];
 #pragma unroll
@@ -205,27 +207,28 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(int nt, int nslab, int Gpad
       ao[a][b] = v16i{0};
     }
 
-  // global -> register prefetch one slab ahead: the loads of slab s + 2 are
-  // issued after slab s + 1 is stored and waited for a whole slab later
-  v4i pa[2], ph[2], po[2];
+  // global -> register prefetch one slab ahead (A: 2 chunks of 16 B per
+  // thread, H and O: 1 each); stored to LDS after the next slab's MFMAs
+  v4i pa[2], ph, po;
   auto load = [&](int s) {
 #pragma unroll
     for (int u = 0; u < 2; u++) {
-      const int c = tid + 256 * u, kk = c >> 7, row = c & 127;
-      const size_t kblk = (size_t)s * 4 + kk;
-      pa[u] = *(const v4i *)(Ab + (kblk * Gpad + I0 + row) * 16);
-      ph[u] = *(const v4i *)(Hb + (kblk * Gpad + J0 + row) * 16);
-      po[u] = *(const v4i *)(Ob + (kblk * Gpad + J0 + row) * 16);
+      const int cc = tid + 512 * u, kk = cc >> 8, row = cc & 255;
+      pa[u] = *(const v4i *)(Ab + (((size_t)s * 4 + kk) * Gpad + I0 + row) * 16);
     }
+    const int kk = tid >> 7, col = tid & 127;
+    ph = *(const v4i *)(Hb + (((size_t)s * 4 + kk) * Gpad + J0 + col) * 16);
+    po = *(const v4i *)(Ob + (((size_t)s * 4 + kk) * Gpad + J0 + col) * 16);
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int u = 0; u < 2; u++) {
-      const int c = tid + 256 * u, kk = c >> 7, row = c & 127;
+      const int cc = tid + 512 * u, kk = cc >> 8, row = cc & 255;
       sA[buf][kk][row] = pa[u];
-      sH[buf][kk][row] = ph[u];
-      sO[buf][kk][row] = po[u];
     }
+    const int kk = tid >> 7, col = tid & 127;
+    sH[buf][kk][col] = ph;
+    sO[buf][kk][col] = po;
   };
   load(0);
   store(0);
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(int nt, int nslab, int Gpad
       const int kk = 2 * ks + lh;
       v4i av[2], hv[2], ov[2];
 #pragma unroll
-      for (int a = 0; a < 2; a++) av[a] = sA[buf][kk][wr * 64 + a * 32 + l32];
+      for (int a = 0; a < 2; a++) av[a] = sA[buf][kk][half * OZ_T + wr * 64 + a * 32 + l32];
 #pragma unroll
       for (int b = 0; b < 2; b++) {
         hv[b] = sH[buf][kk][wc * 64 + b * 32 + l32];
@@ -259,8 +262,12 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(int nt, int nslab, int Gpad
     __syncthreads();
   }
 
-  // residues mod p_m, four consecutive rows packed per 32-bit store; H
-  // product q and O product 3 + q keep the per-product layout of R
+  // residues mod p_m of this half's 128x128 tile (when it is an upper tile
+  // of the matrix), four consecutive rows packed per 32-bit store; H product
+  // q and O product 3 + q keep the per-product layout of R
+  const int I = 2 * IP + half;
+  if (I > J) return;  // lower half of a diagonal pair: below the diagonal
+  const long long tile = (long long)J * (J + 1) / 2 + I;
   const int p = c_p[m];
   const float rp = c_rp[m];
   store_residues(ah, p, rp, wr, wc, lh, l32,
@@ -441,7 +448,8 @@ hipError_t oz_upload_constants(int nmod) {
 
 OzLayout oz_layout(int ngkmax, int Ktot, int nmod) {
   OzLayout L;
-  L.Gpad = (ngkmax + OZ_T - 1) / OZ_T * OZ_T;
+  // rows padded to the 256-row panel of the GEMM workgroups
+  L.Gpad = (ngkmax + 2 * OZ_T - 1) / (2 * OZ_T) * (2 * OZ_T);
   L.Kpad = (Ktot + OZ_KS - 1) / OZ_KS * OZ_KS;
   L.qstride = (size_t)L.Kpad * L.Gpad;
   const long long nt = L.Gpad / OZ_T;
@@ -477,10 +485,10 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
     if (e != hipSuccess) return e;
   }
   if (stage_mask & 2) {
-    const long long nbl = (nt + OZ_BLK - 1) / OZ_BLK;
-    const long long nblk = nbl * (nbl + 1) / 2 * (OZ_BLK * OZ_BLK) * 3 * nmod;
-    hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (int)nt, nslab,
-                       L.Gpad, nmod, QX, QB, L.qstride, R);
+    const long long ncb = (nt + 1) / 2;
+    const long long per = ncb * (ncb + 1) - ((nt & 1) ? ncb : 0);
+    hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)(per * 3 * nmod)), dim3(512), 0, s, (int)nt,
+                       nslab, L.Gpad, nmod, QX, QB, L.qstride, R);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
