@@ -288,7 +288,7 @@ def serving_sample(args):
     from exciting_amd import serve, synth
     specs = synth.serving_cells(args.serve_kmax)
     cells = {name: synth.build(spec, kpoints=ks) for name, (spec, ks) in specs.items()}
-    make, close, execute = serve.gpu_executor(0)
+    make, close, execute = serve.gpu_executor(0, max_solves=args.serve_solves)
     budget = serve.free_hbm(0) - (16 << 30)
     srv = serve.Server(cells, execute, make, close, budget, max_inflight=8,
                        on_finish=lambda r: log(f"serve: request {r.rid} {r.cell} "
@@ -305,7 +305,8 @@ def serving_sample(args):
     out = serve.summarize(reqs, time.perf_counter() - t0)
     out.update({"sample": (f"{args.serve_requests} Poisson requests at {args.serve_rate}/s over "
                            f"fcc2/sc8/ortho16/sc32/sc64, 1..{args.serve_kmax} k-points each, "
-                           f"half split ({args.serve_moduli} moduli), {args.serve_nstfv} states"),
+                           f"half split ({args.serve_moduli} moduli), {args.serve_nstfv} states, "
+                           f"up to {args.serve_solves} eigensolves at once"),
                 "peak_inflight": srv.peak_inflight, "contexts": srv.pool.created})
     srv.pool.close_all()
     return out
@@ -342,6 +343,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1.0)
     ap.add_argument("--serve-kmax", type=int, default=4)
     ap.add_argument("--serve-nstfv", type=int, default=40)
+    ap.add_argument("--serve-solves", type=int, default=4,
+                    help="eigensolves the serving sample runs at once")
     ap.add_argument("--serve-moduli", type=int, default=9,
                     help="moduli of the split requests (9 keeps every cell of the mix within "
                          "1 uHa)")

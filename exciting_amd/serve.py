@@ -24,9 +24,10 @@ Scheduling:
   * Each admitted request runs on its own context (own HIP stream and
     scratch) in its own host thread, so requests that arrive while others are
     on the GPU start at once and their kernels overlap on the device.  The
-    eigensolves (PyTorch's ROCm linear algebra, not re-entrant across host
-    threads) are serialized; they overlap with other requests' match and
-    setup kernels.
+    eigensolves (PyTorch's ROCm solver, whose tridiagonal reduction is a long
+    chain of small kernels) also run concurrently, on the requests' own
+    streams, up to a bound; PyTorch keeps its solver and BLAS handles per
+    thread and stream.
   * Results do not depend on what else runs: every setup kernel has a fixed
     reduction order and owns its output tiles (no atomics on H or O), and
     the eigensolve sees bitwise identical inputs.
@@ -264,17 +265,18 @@ class Server:
 # ---------------------------------------------------------------------------
 # GPU executor
 # ---------------------------------------------------------------------------
-def gpu_executor(device: int = 0, digests: bool = False):
+def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
     """(make_context, close_context, execute) running the library and the
-    eigensolve on ``cuda:device``.  With ``digests`` each request also
-    records a SHA-1 of the bytes of H and O per k-point (copies them to the
-    host: for tests on small cells)."""
+    eigensolve on ``cuda:device``.  At most ``max_solves`` eigensolves run at
+    once.  With ``digests`` each request also records a SHA-1 of the bytes of
+    H and O per k-point (copies them to the host: for tests on small
+    cells)."""
     from . import solve as _solve
     _solve.configure()
     import torch
     from . import lapw
     dev = f"cuda:{device}"
-    solve_lock = threading.Lock()
+    solve_slots = threading.BoundedSemaphore(max(1, max_solves))
 
     def make_context(sys_):
         ctx = lapw.Context(device).load(sys_)
@@ -310,7 +312,7 @@ def gpu_executor(device: int = 0, digests: bool = False):
                     h = hashlib.sha1(ctx.H[:nn].cpu().numpy().tobytes())
                     h.update(ctx.O[:nn].cpu().numpy().tobytes())
                     dg.append(h.hexdigest())
-                with solve_lock:
+                with solve_slots:
                     w = _solve.geneig(ctx.H, ctx.O, d.rank, min(req.nstfv, d.rank))
                     out.append(w.cpu().numpy())
         if digests:
@@ -384,6 +386,8 @@ def main(argv=None) -> int:
     ap.add_argument("--nmod", type=int, default=9)
     ap.add_argument("--split-fraction", type=float, default=0.5)
     ap.add_argument("--max-inflight", type=int, default=8)
+    ap.add_argument("--max-solves", type=int, default=4,
+                    help="eigensolves allowed to run at once")
     ap.add_argument("--reserve-gib", type=float, default=16.0)
     ap.add_argument("--cells", default="fcc2,sc8,ortho16,sc32,sc64")
     ap.add_argument("--seed", type=int, default=0)
@@ -400,7 +404,7 @@ def main(argv=None) -> int:
     print(f"[serve] cells built in {time.perf_counter() - t:.1f} s: "
           + ", ".join(f"{k} ngk<={c.ngkmax} nk={len(c.kpts)}" for k, c in cells.items()),
           file=sys.stderr, flush=True)
-    make, close, execute = gpu_executor(0)
+    make, close, execute = gpu_executor(0, max_solves=args.max_solves)
     budget = free_hbm(0) - int(args.reserve_gib * 2**30)
     def report(r):
         print(f"[serve] request {r.rid} {r.cell} nk={len(r.kpoints)} nmod={r.nmod} "
@@ -427,7 +431,8 @@ def main(argv=None) -> int:
     out.update({"workload": "Poisson stream over fcc2/sc8/ortho16/sc32/sc64 cells",
                 "rate_per_s": args.rate, "kmax": args.kmax, "nstfv": args.nstfv,
                 "split_fraction": args.split_fraction, "nmod": args.nmod,
-                "max_inflight": args.max_inflight, "budget_gib": budget / 2**30,
+                "max_inflight": args.max_inflight, "max_solves": args.max_solves,
+                "budget_gib": budget / 2**30,
                 "peak_inflight": srv.peak_inflight, "contexts_created": srv.pool.created,
                 "contexts_evicted": srv.pool.evicted,
                 "peak_context_gib": srv.peak_bytes / 2**30})
