@@ -52,3 +52,39 @@ def geneig(H, O, n: int, nstfv: int, vectors: bool = False):
     # eigenvectors of the transposed problem are the conjugates of the
     # original ones
     return w[:nstfv], X.conj()
+
+
+def solvewithlapack(H, O, n: int, nstfv: int, nmatmax: int = 0):
+    """modfvsystem.f90:585-720 on the GPU: the lowest nstfv eigenpairs of
+    H x = e O x with ZHEGVX(1, 'V', 'I', 'U') semantics.
+
+    Returns (evalfv, evecfv): evalfv a float64 tensor (nstfv,), ascending;
+    evecfv a complex128 tensor of shape (nstfv, nmatmax) whose memory is the
+    Fortran array evecfv(nmatmax, nstfv) (column ist is state ist, rows n+1..
+    nmatmax zero).  Normalisation x^H O x = 1 as ZHEGVX's; the phase of each
+    vector is the solver's own (the reference's consumers only use
+    phase-invariant products).  Raises RuntimeError where ZHEGVX returns
+    INFO > n (O not positive definite; the reference stops)."""
+    import torch
+    nmatmax = max(nmatmax, n)
+    w, X = geneig(H, O, n, nstfv, vectors=True)
+    evec = torch.zeros((nstfv, nmatmax), dtype=torch.complex128, device=X.device)
+    evec[:, :n] = X.transpose(0, 1)
+    return w, evec
+
+
+def normalise_large_components(evecfv, O0, n: int):
+    """seceqnfv.f90:93-148 (ValenceRelativity 'iora*'): after the solve, each
+    state is divided by sqrt(|x^H O0 x|), where O0 is the overlap set up
+    again without the h1 terms (h1aa = h1loa = h1lolo = 0, h1on = false).
+    evecfv as returned by solvewithlapack (updated in place and returned);
+    O0 the column-major n x n overlap (flat or (n, n) device tensor)."""
+    import torch
+    O0t = O0.reshape(-1)[:n * n].view(n, n)       # row-major view = O0^T
+    X = evecfv[:, :n]                             # (nstfv, n): row ist = state ist
+    # zm = O0 x for every state, zm2(ist, ist) = x^H O0 x
+    Z = torch.matmul(X, O0t.contiguous())         # rows: (O0 x)^T
+    d = (X.conj() * Z).sum(dim=1).abs()
+    evecfv[:, :n] = X / torch.sqrt(d)[:, None]
+    return evecfv
+
