@@ -12,7 +12,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexh.so")
-SOURCES = ["api.hip", "setup_kernels.hip", "match_kernel.hip", "ozaki_kernels.hip", "batch.hip"]
+SOURCES = ["api.hip", "setup_kernels.hip", "match_kernel.hip", "ozaki_kernels.hip", "batch.hip",
+           "interstitial.hip"]
+# hipFFT for the 3-D transforms of the interstitial producers
+LINK = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
 ARCH = os.environ.get("EXH_OFFLOAD_ARCH", "gfx950")
 
 
@@ -55,7 +58,7 @@ def build(verbose: bool = False) -> str:
         if verbose and msg.strip():
             sys.stderr.write(msg)
     tmp = LIB + ".tmp"
-    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, *LINK]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     for o in objs:

@@ -276,6 +276,7 @@ void exh_destroy(exh_context *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   batch_release_all(c);
+  interstitial_release(c);
   if (c->done_ev) (void)hipEventDestroy(c->done_ev);
   DevBuf *bufs[] = {&c->colA, &c->katom, &c->atom_species_d, &c->apword_d, &c->omax_d,
                     &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
@@ -283,7 +284,8 @@ void exh_destroy(exh_context *c) {
                     &c->ivg, &c->ivgig, &c->ist, &c->cf_stage, &c->vf_stage, &c->mf_stage,
                     &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex, &c->ozeb, &c->s_apwalm, &c->s_igpig,
                     &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac,
-                    &c->bbt, &c->bat, &c->ba2t, &c->bivgk};
+                    &c->bbt, &c->bat, &c->ba2t, &c->bivgk, &c->ifz, &c->igeo, &c->i_in0,
+                    &c->i_in1, &c->i_out0, &c->i_out1, &c->i_out2};
   for (DevBuf *b : bufs) release(*b);
   for (auto &r : c->recs) {
     (void)hipEventDestroy(r.a);
@@ -496,6 +498,21 @@ int exh_set_gvectors(exh_context *c, int ngrtot, int ngvec, const int *ivg, cons
   c->ngrtot = ngrtot;
   c->ngvec = ngvec;
   std::memcpy(c->intgv, intgv, sizeof(int) * 6);
+  // Is the G set the whole FFT box, with ivgig the inverse of ivg
+  // (gengvec.f90:91-101)?  The interstitial producers need exactly that.
+  {
+    const int n0 = intgv[3] - intgv[0] + 1, n1 = intgv[4] - intgv[1] + 1;
+    bool full = nbox == (size_t)ngrtot;
+    for (int ig = 0; ig < ngrtot && full; ig++) {
+      const int *v = ivg + 3 * (size_t)ig;
+      bool inside = true;
+      for (int d = 0; d < 3; d++) inside = inside && v[d] >= intgv[d] && v[d] <= intgv[3 + d];
+      full = inside && ivgig[(size_t)(v[0] - intgv[0]) +
+                             (size_t)n0 * ((v[1] - intgv[1]) + (size_t)n1 * (v[2] - intgv[2]))] ==
+                           ig + 1;
+    }
+    c->gvec_full = full;
+  }
   int rc = upload(c, c->ivg, ivg, (size_t)3 * ngrtot);
   if (!rc) rc = upload(c, c->ivgig, ivgig, nbox);
   if (!rc) rc = ensure(c, c->ist, nbox * sizeof(IstEntry));

@@ -116,6 +116,12 @@ struct exh_context {
   int intgv[6] = {0, 0, 0, 0, 0, 0};
   exh::DevBuf ivg, ivgig, ist;  // ist: IstEntry per box point
   exh::DevBuf cf_stage, vf_stage, mf_stage, m2_stage;
+  // interstitial producers: FFT work arrays (up to 3 grids), geometry,
+  // host-API staging, and hipFFT plans per batch count 1..3 (opaque here)
+  exh::DevBuf ifz, igeo, i_in0, i_in1, i_out0, i_out1, i_out2;
+  void *fft_plan[4] = {nullptr, nullptr, nullptr, nullptr};
+  int fft_grid[3] = {0, 0, 0};
+  bool gvec_full = false;  // G set = the whole FFT box, ivgig inverse of ivg
 
   // instrumentation: (kernel id, start, stop) per recorded launch
   bool prof = false;
@@ -176,6 +182,7 @@ hipError_t timed(exh_context *c, int k, hipStream_t s, F &&launch) {
   return e;
 }
 void batch_release_all(exh_context *c);
+void interstitial_release(exh_context *c);
 hipError_t mark_done(exh_context *c, hipStream_t s);
 int wait_done(exh_context *c);
 

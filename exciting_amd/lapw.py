@@ -83,6 +83,12 @@ def lib():
     L.exh_get_batch_plan.argtypes = [_p]
     L.exh_tune_batch.argtypes = [_p, _p, _i, _p, _i, C.POINTER(_i)]
     L.exh_set_batch_workspace.argtypes = [_p, C.c_size_t]
+    L.exh_set_interstitial_device.argtypes = [_p, _p, _p, _p, _p, _p, _i, _i]
+    L.exh_gencfun.argtypes = [_p, _p, _p, _i, _d, _d, _d, _p, _p]
+    L.exh_gencfun_device.argtypes = [_p, _p, _p, _p, _i, _d, _d, _d, _p, _p]
+    L.exh_genveffig.argtypes = [_p, _p, _p, _p]
+    L.exh_genmeffig.argtypes = [_p, _p, _p, _i, _i, _p, _p]
+    L.exh_genveffig_device.argtypes = [_p, _p, _p, _p, _i, _p, _p, _p]
     _lib = L
     return L
 
@@ -101,13 +107,17 @@ EXPORTS = (
     "exh_profile_read", "exh_profile_reset", "exh_setup_batch_device",
     "exh_batch_plan_count", "exh_batch_plan_info", "exh_set_batch_plan",
     "exh_get_batch_plan", "exh_tune_batch", "exh_set_batch_workspace",
+    "exh_set_interstitial_device", "exh_gencfun", "exh_gencfun_device", "exh_genveffig",
+    "exh_genmeffig", "exh_genveffig_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
-           "oz_gemm", "oz_combine", "ho_grouped", "batch_graph")
+           "oz_gemm", "oz_combine", "ho_grouped", "batch_graph", "cfunig", "ifft",
+           "iproducts", "igather")
 
 MATCH = 2   # batch: run match first
 GRAPH = 4   # batch: capture in a HIP graph, replay on identical calls
+VEFFIG, MEFFIG, M2EFFIG = 1, 2, 4   # exh_genveffig_device outputs
 
 
 def plan_count() -> int:
@@ -198,6 +208,8 @@ class Context:
         self.sys: Optional[synth.LapwSystem] = None
         self.nlotot = 0
         self.ngkmax = 0
+        self.natoms: list = []
+        self.ngrtot = self.ngvec = 0
 
     def close(self):
         self._tstream = None
@@ -266,6 +278,7 @@ class Context:
                                         apwordmax, _ptr(apword), nlomax, _ptr(nlorb),
                                         _ptr(lorbl), ngkmax), "exh_set_basis")
         self.ngkmax = ngkmax
+        self.natoms = [int(x) for x in natoms]
 
     def set_radial(self, omega, rmt, nrmt, nrmtmax, spr, apwfr):
         rmt, nrmt = _host(rmt, np.float64), _host(nrmt, np.int32)
@@ -285,13 +298,89 @@ class Context:
         ivg, intgv, ivgig = _host(ivg, np.int32), _host(intgv, np.int32), _host(ivgig, np.int32)
         self._check(lib().exh_set_gvectors(self._h, ngrtot, ngvec, _ptr(ivg), _ptr(intgv),
                                            _ptr(ivgig)), "exh_set_gvectors")
+        self.ngrtot, self.ngvec = int(ngrtot), int(ngvec)
 
     def set_interstitial(self, cfunig, veffig, meffig=None, m2effig=None,
-                         valence_rel_none=True, h1on=False):
+                         valence_rel_none=True, h1on=False, stream=None):
+        """Interstitial tables from host arrays, or from CUDA tensors through
+        exh_set_interstitial_device (no host copy)."""
+        if _is_torch(cfunig):
+            rc = self._device_call(stream, lambda st: lib().exh_set_interstitial_device(
+                self._h, st, _ptr(cfunig), _ptr(veffig), _ptr(meffig), _ptr(m2effig),
+                1 if valence_rel_none else 0, 1 if h1on else 0))
+            self._check(rc, "exh_set_interstitial_device")
+            return
         a = [_host(x, np.complex128) for x in (cfunig, veffig, meffig, m2effig)]
         self._check(lib().exh_set_interstitial(self._h, *[_ptr(x) for x in a],
                                                1 if valence_rel_none else 0,
                                                1 if h1on else 0), "exh_set_interstitial")
+
+    # ---- interstitial producers ----------------------------------------
+    def _atposc(self, atposc):
+        """(natmtot, 3) Cartesian positions (species-major) -> the reference's
+        atposc(3, maxatoms, nspecies) layout and maxatoms."""
+        a = np.asarray(atposc, dtype=np.float64).reshape(-1, 3)
+        if a.shape[0] != sum(self.natoms):
+            raise ExhError(f"atposc holds {a.shape[0]} atoms, the basis {sum(self.natoms)}")
+        maxatoms = max(self.natoms) if self.natoms else 1
+        out = np.zeros((3, maxatoms, len(self.natoms)), order="F")
+        ias = 0
+        for is_, na in enumerate(self.natoms):
+            out[:, :na, is_] = a[ias:ias + na].T
+            ias += na
+        return out, maxatoms
+
+    def gencfun(self, bvec, atposc, epslat=1e-6, cfdamp=0.0, gmaxvr=12.0, cfunig=None,
+                cfunir=None, stream=None):
+        """gencfun.f90:13-85: the smooth step function.  bvec (3,3) with the
+        reciprocal vectors as columns, atposc (natmtot, 3).  With CUDA
+        tensors cfunig (complex128, ngrtot) and cfunir (float64, ngrtot) the
+        device entry point fills them on the GPU; otherwise NumPy arrays are
+        returned.  Returns (cfunig, cfunir)."""
+        bv = np.asfortranarray(bvec, dtype=np.float64)
+        ap, maxatoms = self._atposc(atposc)
+        if _is_torch(cfunig):
+            rc = self._device_call(stream, lambda st: lib().exh_gencfun_device(
+                self._h, st, _ptr(bv), _ptr(ap), maxatoms, epslat, cfdamp, gmaxvr,
+                _ptr(cfunig), _ptr(cfunir)))
+            self._check(rc, "exh_gencfun_device")
+            return cfunig, cfunir
+        cg = np.zeros(self.ngrtot, dtype=np.complex128)
+        cr = np.zeros(self.ngrtot, dtype=np.float64)
+        self._check(lib().exh_gencfun(self._h, _ptr(bv), _ptr(ap), maxatoms, epslat, cfdamp,
+                                      gmaxvr, _ptr(cg), _ptr(cr)), "exh_gencfun")
+        return cg, cr
+
+    def genveffig(self, veffir, cfunir):
+        """genveffig.f90:13-44 from host arrays: veffig(ngvec)."""
+        v, c = _host(veffir, np.float64), _host(cfunir, np.float64)
+        out = np.zeros(self.ngvec, dtype=np.complex128)
+        self._check(lib().exh_genveffig(self._h, _ptr(v), _ptr(c), _ptr(out)), "exh_genveffig")
+        return out
+
+    def genmeffig(self, veffir, cfunir, valence_rel_none=False, iora_star=False):
+        """genmeffig.f90:13-61 from host arrays: (meffig, m2effig); None for
+        what the relativity setting does not produce."""
+        if valence_rel_none:
+            return None, None
+        v, c = _host(veffir, np.float64), _host(cfunir, np.float64)
+        m = np.zeros(self.ngvec, dtype=np.complex128)
+        m2 = np.zeros(self.ngvec, dtype=np.complex128) if iora_star else None
+        self._check(lib().exh_genmeffig(self._h, _ptr(v), _ptr(c), 0, 1 if iora_star else 0,
+                                        _ptr(m), _ptr(m2)), "exh_genmeffig")
+        return m, m2
+
+    def genveffig_device(self, veffir, cfunir, veffig=None, meffig=None, m2effig=None,
+                         stream=None):
+        """Device genveffig/genmeffig: fills whichever of the CUDA tensors
+        veffig, meffig, m2effig (complex128, ngvec) are given, from CUDA
+        tensors veffir and cfunir (float64, ngrtot), in one batched FFT."""
+        which = ((VEFFIG if veffig is not None else 0) | (MEFFIG if meffig is not None else 0)
+                 | (M2EFFIG if m2effig is not None else 0))
+        rc = self._device_call(stream, lambda st: lib().exh_genveffig_device(
+            self._h, st, _ptr(veffir), _ptr(cfunir), which, _ptr(veffig), _ptr(meffig),
+            _ptr(m2effig)))
+        self._check(rc, "exh_genveffig_device")
 
     def load(self, s: synth.LapwSystem) -> "Context":
         """Install every module-state table of a LapwSystem."""

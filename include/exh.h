@@ -124,6 +124,49 @@ int exh_set_gvectors(exh_context *ctx, int ngrtot, int ngvec, const int *ivg,
 int exh_set_interstitial(exh_context *ctx, const exh_z *cfunig,
                          const exh_z *veffig, const exh_z *meffig,
                          const exh_z *m2effig, int valence_rel_none, int h1on);
+/* Same from device memory, ordered after earlier work on `stream` (NULL =
+ * the context's stream); returns once the tables are built.  With the
+ * producers below, the interstitial inputs never leave HBM. */
+int exh_set_interstitial_device(exh_context *ctx, void *stream, const exh_z *cfunig,
+                                const exh_z *veffig, const exh_z *meffig,
+                                const exh_z *m2effig, int valence_rel_none, int h1on);
+
+/* ---- interstitial producers --------------------------------------------- */
+/* They need exh_set_gvectors with the whole FFT box as the G set (ngrtot =
+ * ngrid(1)*ngrid(2)*ngrid(3), ngrid(d) = intgv(d,2) - intgv(d,1) + 1, as
+ * gridsize.f90 makes it).  Real-space arrays are on that grid with the first
+ * index fastest; FFTs are hipFFT double-complex transforms with the
+ * reference's conventions (zfftifc.f90: sign -1 forward and scaled by 1/N,
+ * +1 backward unscaled).
+ *
+ * gencfun.f90:13-85 (gencfun + gencfunig :96-159): the smooth step function
+ * cfunig(ngrtot) in the context's G order and cfunir(ngrtot) = Re of its
+ * backward transform; cfdamp > 0 damps cfunig by exp(-(cfdamp/gmaxvr*|G|)^2)
+ * after the transform, as the reference does.  bvec(3,3) (columns b1,b2,b3,
+ * mod_lattice), atposc(3,maxatoms,nspecies) Cartesian (mod_atoms), epslat
+ * (input%structure%epslat) are host arrays; omega, rmt, nspecies and natoms
+ * come from exh_set_radial and exh_set_basis. */
+int exh_gencfun(exh_context *ctx, const double *bvec, const double *atposc, int maxatoms,
+                double epslat, double cfdamp, double gmaxvr, exh_z *cfunig, double *cfunir);
+/* Same with cfunig and cfunir in device memory, enqueued on `stream`. */
+int exh_gencfun_device(exh_context *ctx, void *stream, const double *bvec,
+                       const double *atposc, int maxatoms, double epslat, double cfdamp,
+                       double gmaxvr, exh_z *cfunig, double *cfunir);
+/* genveffig.f90:13-44: veffig(ngvec) from veffir(ngrtot) * cfunir(ngrtot). */
+int exh_genveffig(exh_context *ctx, const double *veffir, const double *cfunir,
+                  exh_z *veffig);
+/* genmeffig.f90:13-61: nothing when valence_rel_none; else meffig(ngvec) from
+ * cfunir / (1 - veffir*a2) and, when iora_star, m2effig(ngvec) from
+ * cfunir / (1 - veffir*a2)^2, a2 = alpha^2/2. */
+int exh_genmeffig(exh_context *ctx, const double *veffir, const double *cfunir,
+                  int valence_rel_none, int iora_star, exh_z *meffig, exh_z *m2effig);
+/* Device form of both: the products selected by `which` (EXH_VEFFIG,
+ * EXH_MEFFIG, EXH_M2EFFIG) are transformed as one batched FFT and gathered;
+ * outputs not selected may be NULL.  Enqueued on `stream`. */
+enum { EXH_VEFFIG = 1, EXH_MEFFIG = 2, EXH_M2EFFIG = 4 };
+int exh_genveffig_device(exh_context *ctx, void *stream, const double *veffir,
+                         const double *cfunir, int which, exh_z *veffig, exh_z *meffig,
+                         exh_z *m2effig);
 
 /* ---- hot path ----------------------------------------------------------- */
 /* match.f90:13-194.  gpc(ngkmax), tpgpc(2,ngkmax), sfacgp(ngkmax,natmtot) in;
@@ -235,7 +278,11 @@ enum {
   EXH_K_OZCOMB = 10,  /* split: Garner/CRT + interstitial + mirror      */
   EXH_K_HOGRP = 11,   /* batch: grouped persistent H/O contraction      */
   EXH_K_GRAPH = 12,   /* batch: one replay of a captured batch graph    */
-  EXH_K_COUNT = 13
+  EXH_K_CFUNIG = 13,  /* gencfunig closed form + scatter to the grid    */
+  EXH_K_IFFT = 14,    /* interstitial producers: hipFFT transforms      */
+  EXH_K_IPROD = 15,   /* genveffig/genmeffig products on the grid       */
+  EXH_K_IGATHER = 16, /* grid -> G gathers (cfunir, veffig, meffig...)  */
+  EXH_K_COUNT = 17
 };
 int exh_profile_enable(exh_context *ctx, int on);
 int exh_profile_read(exh_context *ctx, int kernel, double *total_ms, int *launches);

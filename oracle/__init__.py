@@ -72,6 +72,7 @@ def lib():
             getattr(L, name).argtypes = [_p, _i, _p, _p, _i, _p]
         L.orc_copy_ul.argtypes = [_i, _p, _p]
         L.orc_set_fault.argtypes = [_i]
+        L.orc_gencfunig.argtypes = [_i, _p, _p, _d, _i, _p, _p, _p, _d, _p]
         L.orc_nlotot.argtypes = [_p]
         L.orc_maxapw.argtypes = [_p]
         _lib = L
@@ -206,3 +207,101 @@ def setup(sys_, ik: int, apwalm: np.ndarray, fast: bool = True):
 
 def set_fault(code: int) -> None:
     lib().orc_set_fault(code)
+
+
+# --- interstitial producers (gencfun.f90, genveffig.f90, genmeffig.f90) ---
+
+ALPHA = 1.0 / 137.03599911   # genmeffig.f90:29
+
+
+def ngrid_of(intgv) -> tuple:
+    """FFT grid from the G box: intgv(:,1) = n/2 - n + 1, intgv(:,2) = n/2
+    (gridsize.f90), so n = intgv(:,2) - intgv(:,1) + 1."""
+    iv = np.asarray(intgv)
+    return tuple(int(x) for x in iv[:, 1] - iv[:, 0] + 1)
+
+
+def igfft(ngrid, ivg) -> np.ndarray:
+    """gengvec.f90:129-167 (genigfft), 0-based: the FFT grid point of each G
+    vector, j_d = i_d (i_d >= 0) or ngrid_d + i_d, m = j3*n1*n2 + j2*n1 + j1."""
+    n = np.asarray(ngrid, dtype=np.int64)
+    iv = np.asarray(ivg, dtype=np.int64)
+    j = np.where(iv >= 0, iv, n[:, None] + iv)
+    return j[2] * n[0] * n[1] + j[1] * n[0] + j[0]
+
+
+def zfftifc(ngrid, sgn: int, z: np.ndarray) -> np.ndarray:
+    """zfftifc.f90:13-110 for nd = 3: z holds the grid with the first index
+    fastest; sgn -1 is the forward transform (exp(-i...)) scaled by 1/N
+    (:70-77), +1 the unscaled backward one.  Restated with numpy's FFT; the
+    convention is pinned against the reference's own zfftifc/cfftnd by the
+    golden vectors."""
+    n1, n2, n3 = ngrid
+    a = np.asarray(z, dtype=np.complex128).reshape(n3, n2, n1)
+    if sgn == -1:
+        out = np.fft.fftn(a) * (1.0 / a.size)
+    elif sgn == 1:
+        out = np.fft.ifftn(a) * a.size
+    else:
+        raise ValueError("sgn must be -1 or 1")
+    return out.reshape(-1)
+
+
+def gencfunig(ivg, bvec, omega, natoms, rmt, atposc, epslat=1e-6) -> np.ndarray:
+    """gencfun.f90:96-159 through the C restatement orc_gencfunig.
+    atposc: (natmtot, 3) Cartesian, species-major."""
+    iv = np.ascontiguousarray(np.asarray(ivg).T, dtype=np.int32)       # (ng, 3)
+    bv = np.asfortranarray(np.asarray(bvec, dtype=np.float64))
+    na = np.ascontiguousarray(natoms, dtype=np.int32)
+    rm = np.ascontiguousarray(rmt, dtype=np.float64)
+    ap = np.ascontiguousarray(atposc, dtype=np.float64)
+    out = np.zeros(iv.shape[0], dtype=np.complex128)
+    lib().orc_gencfunig(iv.shape[0], iv.ctypes.data, bv.ctypes.data, float(omega), len(na),
+                        na.ctypes.data, rm.ctypes.data, ap.ctypes.data, float(epslat),
+                        out.ctypes.data)
+    return out
+
+
+def g_lengths(ivg, bvec) -> np.ndarray:
+    """gengvec.f90:61-70: |G| with v = i1*b1 + i2*b2 + i3*b3."""
+    iv = np.asarray(ivg, dtype=np.float64)
+    b = np.asarray(bvec, dtype=np.float64)
+    v = iv[0][None, :] * b[:, 0:1] + iv[1][None, :] * b[:, 1:2] + iv[2][None, :] * b[:, 2:3]
+    return np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+
+
+def gencfun(s, epslat=1e-6, cfdamp=0.0, gmaxvr=12.0):
+    """gencfun.f90:51-84 for a system s (synth.LapwSystem): (cfunig over all
+    ngrtot G vectors, cfunir on the FFT grid)."""
+    ngrid = ngrid_of(s.intgv)
+    cfunig = gencfunig(s.ivg, s.bvec, s.omega, s.natoms, s.rmt, s.atposc, epslat)
+    zfft = np.zeros(int(np.prod(ngrid)), dtype=np.complex128)
+    zfft[igfft(ngrid, s.ivg)] = cfunig                          # :70
+    cfunir = zfftifc(ngrid, 1, zfft).real.copy()                 # :73-74
+    if cfdamp > 0.0:                                             # :76-82
+        t1 = cfdamp / gmaxvr
+        cfunig = cfunig * np.exp(-(t1 * g_lengths(s.ivg, s.bvec)) ** 2)
+    return cfunig, cfunir
+
+
+def _forward_gather(s, zfft):
+    ngrid = ngrid_of(s.intgv)
+    z = zfftifc(ngrid, -1, zfft)
+    return z[igfft(ngrid, s.ivg[:, :s.ngvec])]
+
+
+def genveffig(s, veffir, cfunir) -> np.ndarray:
+    """genveffig.f90:31-39: veffig(1:ngvec) from veffir * cfunir."""
+    return _forward_gather(s, np.asarray(veffir) * np.asarray(cfunir) + 0j)
+
+
+def genmeffig(s, veffir, cfunir, iora_star: bool):
+    """genmeffig.f90:33-57 for ValenceRelativity != 'none': meffig from
+    cfunir / (1 - veffir*a2) and, for iora*, m2effig from
+    cfunir / (1 - veffir*a2)**2, a2 = alpha^2/2."""
+    a2 = 0.5 * ALPHA ** 2
+    v = np.asarray(veffir, dtype=np.float64)
+    c = np.asarray(cfunir, dtype=np.float64)
+    meffig = _forward_gather(s, c / (1.0 - v * a2) + 0j)
+    m2effig = _forward_gather(s, c / ((1.0 - v * a2) ** 2) + 0j) if iora_star else None
+    return meffig, m2effig

@@ -928,3 +928,40 @@ int orc_hamiltonandoverlapsetup(const orc_system *s, int ngp,
   orc_copy_ul(rank, h, o); /* overlapsetup.f90:164-166 */
   return 0;
 }
+
+/* gencfun.f90:96-159.  The G vector and its length follow gengvec.f90:61-70
+ * and :99-100 (v = i1*b1 + i2*b2 + i3*b3 summed left to right); the loop
+ * order per G (species, then atoms) is the reference's. */
+void orc_gencfunig(int ng, const int *ivg, const double *bvec, double omega,
+                   int nspecies, const int *natoms, const double *rmt,
+                   const double *atposc, double epslat, orc_z *cfunig) {
+  const double fourpi = 12.566370614359172954;
+  const double t1 = fourpi / omega; /* :120 */
+#pragma omp parallel for schedule(static)
+  for (int ig = 0; ig < ng; ig++) {
+    const double i1 = ivg[3 * ig], i2 = ivg[3 * ig + 1], i3 = ivg[3 * ig + 2];
+    double v[3];
+    for (int d = 0; d < 3; d++) v[d] = i1 * bvec[d] + i2 * bvec[3 + d] + i3 * bvec[6 + d];
+    const double gc = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const int finite = gc > epslat; /* :123-124 */
+    orc_z acc = finite ? 0.0 : 1.0; /* :126-127 */
+    int ias = 0;
+    for (int is = 0; is < nspecies; is++) {
+      double ffacg;
+      if (finite) { /* :133-137 */
+        const double t2 = gc * rmt[is];
+        ffacg = g_fault == 5 ? t1 * (sin(t2) + t2 * cos(t2)) / (gc * gc * gc) /* planted */
+                             : t1 * (sin(t2) - t2 * cos(t2)) / (gc * gc * gc);
+      } else { /* :140-143 */
+        ffacg = t1 / 3.0 * rmt[is] * rmt[is] * rmt[is];
+      }
+      for (int ia = 0; ia < natoms[is]; ia++, ias++) { /* :147-155 */
+        const double *tau = atposc + 3 * ias;
+        double t2 = -(v[0] * tau[0] + v[1] * tau[1] + v[2] * tau[2]);
+        if (g_fault == 6) t2 = -t2; /* planted mistake: phase sign */
+        acc -= ffacg * (cos(t2) + I * sin(t2));
+      }
+    }
+    cfunig[ig] = acc;
+  }
+}

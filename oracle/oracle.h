@@ -132,6 +132,14 @@ void orc_olpistln(const orc_system *s, int ngp, const int *igpig,
                   const double *vgpc, int rank, orc_z *o);
 void orc_copy_ul(int rank, orc_z *h, orc_z *o);
 
+/* gencfun.f90:96-159 (gencfunig): Fourier coefficients of the smooth step
+ * function for ng G vectors given by their integer coordinates ivg(3,ng) in
+ * the reciprocal basis bvec(3,3) (columns b1, b2, b3).  atposc(3,natmtot)
+ * holds Cartesian positions in species-major atom order; rmt(nspecies). */
+void orc_gencfunig(int ng, const int *ivg, const double *bvec, double omega,
+                   int nspecies, const int *natoms, const double *rmt,
+                   const double *atposc, double epslat, orc_z *cfunig);
+
 /* Planted-mistake switch for tests: 0 = faithful.  Nonzero values inject one
  * deliberate error so tests can show the checks catch it. */
 void orc_set_fault(int code);

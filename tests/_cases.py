@@ -32,4 +32,6 @@ def system(name: str):
         return synth.build(synth.config(0))
     if name == "sc8_k0":
         return synth.build(synth.config(1), kpoints=[0, 5])
+    if name == "sc64_k0":
+        return synth.build(synth.config(3), kpoints=[0])
     raise KeyError(name)
