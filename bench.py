@@ -312,6 +312,97 @@ def serving_sample(args):
     return out
 
 
+def interstitial_sample(ctx, s, reps=20):
+    """The interstitial producers on the workload's FFT grid, inputs resident
+    in HBM: gencfun (closed form over every G of the box, backward FFT, real
+    part), genveffig alone ('none' relativity) and genveffig + genmeffig
+    ('iora*': three products, one batched forward FFT, gathers), and the
+    whole device chain gencfun -> genveffig -> set_interstitial_device.
+    Per-kernel averages come from the library's launch events on the stream
+    the kernels run on."""
+    import torch
+    n, ng = s.ngrtot, s.ngvec
+    ngrid = tuple(int(x) for x in s.intgv[:, 1] - s.intgv[:, 0] + 1)
+    cg = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    cr = torch.zeros(n, dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(1)
+    vd = torch.from_numpy(-0.4 + 0.3 * rng.uniform(-1, 1, n)).cuda()
+    outs = [torch.zeros(ng, dtype=torch.complex128, device="cuda") for _ in range(3)]
+
+    def gencfun():
+        ctx.gencfun(s.bvec, s.atposc, cfunig=cg, cfunir=cr)
+
+    def veff():
+        ctx.genveffig_device(vd, cr, veffig=outs[0])
+
+    def veff_meff():
+        ctx.genveffig_device(vd, cr, veffig=outs[0], meffig=outs[1], m2effig=outs[2])
+
+    def chain():
+        gencfun()
+        veff()
+        ctx.set_interstitial(cg, outs[0], valence_rel_none=True)
+
+    def timed(fn):
+        fn()                                    # plans and buffers
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps * 1e3
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        k = {name: ms / cnt for name, (ms, cnt) in prof.items() if cnt}
+        return wall, k
+
+    res = {}
+    w, k = timed(gencfun)
+    t = k["cfunig"] * 1e-3
+    res["gencfun"] = {
+        "ms": w, "kernels_ms": k,
+        "cfunig_kernel": {"bound": "fp64 sincos throughput",
+                          "pairs_per_s": n * s.natmtot / t,
+                          "hbm_GBps": 44.0 * n / t / 1e9,
+                          "bytes_per_G": 44},
+    }
+    w1, k1 = timed(veff)
+    w3, k3 = timed(veff_meff)
+    fft1 = k1["ifft"] * 1e-3
+    fft3 = k3["ifft"] * 1e-3
+    res["genveffig"] = {"ms": w1, "kernels_ms": k1}
+    res["genveffig_genmeffig_iora"] = {"ms": w3, "kernels_ms": k3}
+    res["fft"] = {"grid": ngrid, "one_pass_bytes_per_grid": 32 * n,
+                  "single_GBps_one_pass": 32.0 * n / fft1 / 1e9,
+                  "batched3_GBps_one_pass": 96.0 * n / fft3 / 1e9,
+                  "frac_hbm_one_pass": 96.0 * n / fft3 / 8e12}
+    wc, _ = timed(chain)
+    res["device_chain_ms"] = wc
+    torch.cuda.synchronize()
+    # CPU: the oracle (C closed form with OpenMP, numpy FFTs) on the same grid
+    import oracle
+    t0 = time.perf_counter()
+    _, cr0 = oracle.gencfun(s)
+    t1 = time.perf_counter()
+    v = vd.cpu().numpy()
+    oracle.genveffig(s, v, cr0)
+    oracle.genmeffig(s, v, cr0, True)
+    t2 = time.perf_counter()
+    res["cpu_baseline"] = {
+        "value": 1e3 * (t2 - t0), "unit": "ms (gencfun + genveffig + genmeffig iora*)",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())), "kind": "port",
+        "sample": (f"the whole producer set once on the {ngrid} grid: oracle gencfunig "
+                   f"(C, OpenMP) {1e3 * (t1 - t0):.0f} ms + numpy FFTs and gathers "
+                   f"{1e3 * (t2 - t1):.0f} ms"),
+    }
+    res["gpu_ms_same_work"] = res["gencfun"]["ms"] + res["genveffig_genmeffig_iora"]["ms"]
+    res["workload"] = (f"{s.natmtot} atoms, FFT grid {ngrid[0]}x{ngrid[1]}x{ngrid[2]} "
+                       f"(ngrtot {n}), ngvec {ng}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,6 +439,9 @@ def main():
     ap.add_argument("--serve-moduli", type=int, default=9,
                     help="moduli of the split requests (9 keeps every cell of the mix within "
                          "1 uHa)")
+    ap.add_argument("--interstitial", choices=("on", "off", "only"), default="on",
+                    help="time the interstitial producers on the workload's grid "
+                         "('only': nothing else, one JSON line with that sub-object)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -375,6 +469,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log(f"calibration copy of {args.copy_calibration_mib} MiB done")
+
+    if args.interstitial == "only":
+        print(json.dumps({"interstitial": interstitial_sample(W.ctx, s)}), flush=True)
+        return
 
     if args.plan >= 0:
         W.ctx.batch_plan = args.plan
@@ -452,6 +550,9 @@ def main():
             out["speedup_vs_fp64_loop"] = split["value"] / fp64_loop["value"]
         if "accuracy" in split:
             out["accuracy"] = split["accuracy"]
+    if args.interstitial == "on":
+        out["interstitial"] = interstitial_sample(W.ctx, s)
+        log(f"interstitial: {out['interstitial']}")
     W.free()
 
     if args.uneven == "on":

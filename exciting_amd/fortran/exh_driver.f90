@@ -3,9 +3,13 @@
 ! k-point match (seceqn.f90:57, apwalm zeroed at :46-47) and the H/O setup
 ! (seceqnfv.f90:73-77, H and O zeroed by newsystem).
 !
+! Then the interstitial producers (gencfun, genveffig, genmeffig for iora*)
+! from the cell's bvec, atposc and a real-space veffir.
+!
 ! Usage: exh_driver <input.bin> <output.bin>
 ! The input is a stream of int32/float64/complex128 records written by
-! tests/test_gpu_fortran.py; the output holds apwalm, H and O.
+! tests/test_gpu_fortran.py; the output holds apwalm, H and O, then cfunig,
+! cfunir, veffig, meffig and m2effig.
 program exh_driver
   use, intrinsic :: iso_c_binding
   use exh_lapw
@@ -24,6 +28,10 @@ program exh_driver
   complex(c_double_complex), allocatable, target :: aa(:,:,:), loa(:,:,:), lolo(:,:,:)
   complex(c_double_complex), allocatable, target :: cfunig(:), veffig(:)
   complex(c_double_complex), allocatable :: sfacgp(:,:), apwalm(:,:,:,:), h(:,:), o(:,:)
+  integer(c_int) :: maxatoms
+  real(c_double) :: bvec(3, 3)
+  real(c_double), allocatable :: atposc(:,:,:), veffir(:), cfunir(:)
+  complex(c_double_complex), allocatable :: cfunig_p(:), veffig_p(:), meffig(:), m2effig(:)
   type(c_ptr) :: ctx
 
   call get_command_argument(1, fin)
@@ -42,6 +50,9 @@ program exh_driver
   read (u) aa, loa, lolo, oalo, ololo
   read (u) ivg, intgv, ivgig, cfunig, veffig
   read (u) gpc, tpgpc, sfacgp, igpig, vgpc
+  read (u) maxatoms
+  allocate (atposc(3, maxatoms, nspecies), veffir(ngrtot))
+  read (u) bvec, atposc, veffir
   close (u)
 
   rc = exh_create(0_c_int, ctx)
@@ -72,10 +83,19 @@ program exh_driver
   o = (0.d0, 0.d0)
   rc = exh_hamiltonandoverlapsetup(ctx, ngp, rank, apwalm, igpig, vgpc, h, o)
   call check(rc, 'exh_hamiltonandoverlapsetup')
+
+  ! gencfun.f90:13 (epslat 1e-6, no damping), genveffig.f90:13, genmeffig.f90:13
+  allocate (cfunig_p(ngrtot), cfunir(ngrtot), veffig_p(ngvec), meffig(ngvec), m2effig(ngvec))
+  rc = exh_gencfun(ctx, bvec, atposc, maxatoms, 1.d-6, 0.d0, 12.d0, cfunig_p, cfunir)
+  call check(rc, 'exh_gencfun')
+  rc = exh_genveffig(ctx, veffir, cfunir, veffig_p)
+  call check(rc, 'exh_genveffig')
+  rc = exh_genmeffig(ctx, veffir, cfunir, 0_c_int, 1_c_int, meffig, m2effig)
+  call check(rc, 'exh_genmeffig')
   call exh_destroy(ctx)
 
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u) apwalm, h, o
+  write (u) apwalm, h, o, cfunig_p, cfunir, veffig_p, meffig, m2effig
   close (u)
   print '(a,i0)', 'exh_driver: done, rank = ', rank
 

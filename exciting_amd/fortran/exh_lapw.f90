@@ -1,9 +1,11 @@
 ! ISO_C_BINDING interface to libexh (include/exh.h).
 !
 ! A Fortran caller (exciting's seceqn/seceqnfv) uses this module to reach the
-! MI355X H/O setup.  Every C entry point has an interface here with the same
-! argument order; arrays are passed by address (c_loc, or implicitly as
-! assumed-size dummies) in the reference's own column-major layouts.
+! MI355X H/O setup.  Every entry point a Fortran caller needs has an
+! interface here with the same argument order; arrays are passed by address
+! (c_loc, or implicitly as assumed-size dummies) in the reference's own
+! column-major layouts.  (The profiling calls and the single-k device entry
+! points are left to C and Python callers.)
 !
 !   exh_match                    replaces  match (ngp, gpc, tpgpc, sfacgp, apwalm)
 !                                          src/match.f90:13
@@ -12,6 +14,9 @@
 !                                          src/src_eigensystem/hamiltonandoverlapsetup.f90:8
 !                                with the semantics of hamiltonsetup followed by
 !                                overlapsetup (seceqnfv.f90:76-77).
+!   exh_gencfun                  replaces  gencfun (src/gencfun.f90:13)
+!   exh_genveffig                replaces  genveffig (src/genveffig.f90:13)
+!   exh_genmeffig                replaces  genmeffig (src/genmeffig.f90:13)
 module exh_lapw
   use, intrinsic :: iso_c_binding
   implicit none
@@ -22,6 +27,8 @@ module exh_lapw
   ! function): batch flag that runs match first
   integer(c_int), parameter, public :: EXH_BATCH_MATCH = 2
   integer(c_int), parameter, public :: EXH_GRAPH = 4   ! batch: capture / replay a HIP graph
+  ! outputs of exh_genveffig_device
+  integer(c_int), parameter, public :: EXH_VEFFIG = 1, EXH_MEFFIG = 2, EXH_M2EFFIG = 4
 
   ! One k-point of a batch (exh_kpoint in include/exh.h); device pointers.
   type, bind(C), public :: exh_kpoint
@@ -33,7 +40,9 @@ module exh_lapw
             exh_set_gvectors, exh_set_interstitial, exh_match, &
             exh_hamiltonandoverlapsetup, exh_hamiltonandoverlapsetup_ex, &
             exh_error_message, exh_split, exh_setup_batch_device, exh_tune_batch, &
-            exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace
+            exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace, &
+            exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
+            exh_genmeffig, exh_genveffig_device
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -91,6 +100,67 @@ module exh_lapw
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, cfunig, veffig, meffig, m2effig
       integer(c_int), value :: valence_rel_none, h1on
+    end function
+
+    ! Device pointers, ordered on `stream` (c_null_ptr: the context's own).
+    integer(c_int) function exh_set_interstitial_device(ctx, stream, cfunig, veffig, meffig, &
+        m2effig, valence_rel_none, h1on) bind(C, name="exh_set_interstitial_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, cfunig, veffig, meffig, m2effig
+      integer(c_int), value :: valence_rel_none, h1on
+    end function
+
+    ! gencfun.f90:13-85: bvec(3,3), atposc(3,maxatoms,nspecies) as in mod_lattice
+    ! and mod_atoms; cfunig(ngrtot), cfunir(ngrtot) out.
+    integer(c_int) function exh_gencfun(ctx, bvec, atposc, maxatoms, epslat, cfdamp, gmaxvr, &
+        cfunig, cfunir) bind(C, name="exh_gencfun")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: bvec(*), atposc(*)
+      integer(c_int), value :: maxatoms
+      real(c_double), value :: epslat, cfdamp, gmaxvr
+      complex(c_double_complex), intent(out) :: cfunig(*)
+      real(c_double), intent(out) :: cfunir(*)
+    end function
+
+    ! Same with cfunig/cfunir in device memory (bvec and atposc stay host arrays).
+    integer(c_int) function exh_gencfun_device(ctx, stream, bvec, atposc, maxatoms, epslat, &
+        cfdamp, gmaxvr, cfunig, cfunir) bind(C, name="exh_gencfun_device")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, stream
+      real(c_double), intent(in) :: bvec(*), atposc(*)
+      integer(c_int), value :: maxatoms
+      real(c_double), value :: epslat, cfdamp, gmaxvr
+      type(c_ptr), value :: cfunig, cfunir
+    end function
+
+    ! genveffig.f90:13-44: veffir(ngrtot), cfunir(ngrtot) in, veffig(ngvec) out.
+    integer(c_int) function exh_genveffig(ctx, veffir, cfunir, veffig) &
+        bind(C, name="exh_genveffig")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: veffir(*), cfunir(*)
+      complex(c_double_complex), intent(out) :: veffig(*)
+    end function
+
+    ! genmeffig.f90:13-61: meffig(ngvec) unless valence_rel_none, m2effig(ngvec)
+    ! when iora_star.
+    integer(c_int) function exh_genmeffig(ctx, veffir, cfunir, valence_rel_none, iora_star, &
+        meffig, m2effig) bind(C, name="exh_genmeffig")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: veffir(*), cfunir(*)
+      integer(c_int), value :: valence_rel_none, iora_star
+      complex(c_double_complex), intent(out) :: meffig(*), m2effig(*)
+    end function
+
+    ! Device form of both, outputs selected by `which` (EXH_VEFFIG, ...).
+    integer(c_int) function exh_genveffig_device(ctx, stream, veffir, cfunir, which, veffig, &
+        meffig, m2effig) bind(C, name="exh_genveffig_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, veffir, cfunir
+      integer(c_int), value :: which
+      type(c_ptr), value :: veffig, meffig, m2effig
     end function
 
     integer(c_int) function exh_match(ctx, ngp, gpc, tpgpc, sfacgp, apwalm) &

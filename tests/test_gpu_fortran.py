@@ -1,7 +1,9 @@
 """The Fortran ISO_C_BINDING path (exciting_amd/fortran/exh_lapw.f90) end to
-end: a flang-built program calls match and hamiltonandoverlapsetup through
-the binding module on the GPU; its apwalm, H and O are checked against the
-oracle to the same 1e-12 relative Frobenius bar as the Python path."""
+end: a flang-built program calls match and hamiltonandoverlapsetup, then the
+interstitial producers (gencfun, genveffig, genmeffig), through the binding
+module on the GPU; apwalm, H, O and the producers' outputs are checked
+against the oracle to the same 1e-12 relative Frobenius bar as the Python
+path."""
 import os
 import subprocess
 
@@ -28,6 +30,17 @@ def write_input(path, s, ik=0):
         f.write(c128(s.aa) + c128(s.loa) + c128(s.lolo) + f64(s.oalo) + f64(s.ololo))
         f.write(i32(s.ivg) + i32(s.intgv) + i32(s.ivgig) + c128(s.cfunig) + c128(s.veffig))
         f.write(f64(k.gpc) + f64(k.tpgpc) + c128(k.sfacgp) + i32(k.igpig) + f64(k.vgpc))
+        maxatoms = max(s.natoms)
+        atposc = np.zeros((3, maxatoms, s.nspecies), order="F")
+        ias = 0
+        for is_, na in enumerate(s.natoms):
+            atposc[:, :na, is_] = s.atposc[ias:ias + na].T
+            ias += na
+        f.write(i32([maxatoms]) + f64(s.bvec) + f64(atposc) + f64(veffir_of(s)))
+
+
+def veffir_of(s):
+    return -0.4 + 0.3 * np.random.default_rng(9).uniform(-1, 1, s.ngrtot)
 
 
 @pytest.mark.gpu
@@ -50,10 +63,24 @@ def test_fortran_driver(tmp_path, name):
     na = s.ngkmax * s.apwordmax * lmm * s.natmtot
     A = raw[:na].reshape((s.ngkmax, s.apwordmax, lmm, s.natmtot), order="F")
     H = raw[na:na + n * n].reshape((n, n), order="F")
-    O = raw[na + n * n:].reshape((n, n), order="F")
+    O = raw[na + n * n:na + 2 * n * n].reshape((n, n), order="F")
+    rest = np.fromfile(fout, dtype=np.uint8)[16 * (na + 2 * n * n):]
+    ng, nv = s.ngrtot, s.ngvec
+    cuts = np.cumsum([0, 16 * ng, 8 * ng, 16 * nv, 16 * nv, 16 * nv])
+    assert rest.size == cuts[-1]
+    cg, cr, vg, mg, m2g = (rest[a:b].view(dt) for a, b, dt in
+                           zip(cuts[:-1], cuts[1:], (np.complex128, np.float64, np.complex128,
+                                                     np.complex128, np.complex128)))
     S = oracle.System(s)
     A0 = oracle.match(S, 0)
     H0, O0 = oracle.setup(S, 0, A0)
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
     print(f"  fortran {name}: A {rel(A, A0):.2e} H {rel(H, H0):.2e} O {rel(O, O0):.2e}")
     assert rel(A, A0) <= 1e-12 and rel(H, H0) <= 1e-12 and rel(O, O0) <= 1e-12
+    cg0, cr0 = oracle.gencfun(s)
+    v = veffir_of(s)
+    vg0 = oracle.genveffig(s, v, cr0)
+    mg0, m2g0 = oracle.genmeffig(s, v, cr0, True)
+    errs = [rel(a, b) for a, b in ((cg, cg0), (cr, cr0), (vg, vg0), (mg, mg0), (m2g, m2g0))]
+    print("  fortran producers: " + " ".join(f"{e:.1e}" for e in errs))
+    assert max(errs) <= 1e-12
