@@ -285,6 +285,10 @@ class LapwSystem:
     m2effig: Optional[np.ndarray] = None
     valence_rel_none: bool = True
     h1on: bool = False
+    # muffin-tin potential and LO radial functions (inputs of hmlint)
+    lmaxvr: int = 0
+    veffmt: Optional[np.ndarray] = None   # (lmmaxvr, nrmtmax, natmtot) F
+    lofr: Optional[np.ndarray] = None     # (nrmtmax, 2, nlomax, natmtot) F
 
     def nmat(self, ik: int) -> int:
         return self.kpts[ik].ngk + self.nlotot
@@ -484,4 +488,36 @@ def add_relativity(sys: LapwSystem, seed: int = 5, iora: bool = True) -> LapwSys
         sys.h1loa = np.asfortranarray(0.01 * rng.uniform(-1, 1, (p, sys.nlomax, sys.natmtot)))
         sys.h1lolo = np.asfortranarray(0.01 * rng.uniform(-1, 1, (sys.nlomax, sys.nlomax, sys.natmtot)))
         sys.h1on = True
+    return sys
+
+
+def add_mt_potential(sys: LapwSystem, lmaxvr: int = 8, seed: int = 21) -> LapwSystem:
+    """Attach the inputs of hmlint: a smooth real muffin-tin potential
+    veffmt(lm, r) = c0 + c1 (r/R) + c2 (r/R)^2 per (lm, atom) with a deep
+    spherical part, and LO radial functions lofr with their derivatives
+    (cubics in r - R like the APW functions)."""
+    rng = np.random.default_rng(seed)
+    lmm = (lmaxvr + 1) ** 2
+    nr, na = sys.nrmtmax, sys.natmtot
+    veffmt = np.zeros((lmm, nr, na), order="F")
+    lofr = np.zeros((nr, 2, max(sys.nlomax, 1), na), order="F")
+    ias = 0
+    for is_ in range(sys.nspecies):
+        R = sys.rmt[is_]
+        n = sys.nrmt[is_]
+        x = sys.spr[:n, is_] / R
+        dr = sys.spr[:n, is_] - R
+        for _ in range(sys.natoms[is_]):
+            c = rng.uniform(-1, 1, (lmm, 3)) * 0.3
+            c[0] = [-6.0, 2.5, 1.0]
+            c /= (1.0 + np.sqrt(np.arange(lmm)))[:, None]
+            veffmt[:, :n, ias] = c[:, 0:1] + x[None, :] * (c[:, 1:2] + x[None, :] * c[:, 2:3])
+            for ilo in range(sys.nlorb[is_]):
+                k = rng.uniform(-0.5, 0.5, 4)
+                lofr[:n, 0, ilo, ias] = k[0] + dr * (k[1] + dr * (k[2] + dr * k[3]))
+                lofr[:n, 1, ilo, ias] = k[1] + dr * (2 * k[2] + dr * 3 * k[3])
+            ias += 1
+    sys.lmaxvr = lmaxvr
+    sys.veffmt = veffmt
+    sys.lofr = lofr
     return sys

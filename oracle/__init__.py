@@ -73,6 +73,14 @@ def lib():
         L.orc_copy_ul.argtypes = [_i, _p, _p]
         L.orc_set_fault.argtypes = [_i]
         L.orc_gencfunig.argtypes = [_i, _p, _p, _d, _i, _p, _p, _p, _d, _p]
+        L.orc_spline.argtypes = [_i, _p, _p, _p]
+        L.orc_fderiv_int.argtypes = [_i, _p, _p, _p]
+        L.orc_wigner3j.argtypes = [_i] * 6
+        L.orc_wigner3j.restype = _d
+        L.orc_gaunt_yyy.argtypes = [_i] * 6
+        L.orc_gaunt_yyy.restype = _d
+        L.orc_gaunt_yry.argtypes = [_i] * 6 + [_p]
+        L.orc_hmlint.argtypes = [_p, _i, _p, _p, _i, _p, _p, _p]
         L.orc_nlotot.argtypes = [_p]
         L.orc_maxapw.argtypes = [_p]
         _lib = L
@@ -305,3 +313,50 @@ def genmeffig(s, veffir, cfunir, iora_star: bool):
     meffig = _forward_gather(s, c / (1.0 - v * a2) + 0j)
     m2effig = _forward_gather(s, c / ((1.0 - v * a2) ** 2) + 0j) if iora_star else None
     return meffig, m2effig
+
+
+# --- muffin-tin Hamiltonian tables (hmlint.f90 -> mt_kin.f90, mt_pot.f90) ---
+
+def spline(x, f):
+    """spline.f90: cf(3, n)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    cf = np.zeros((3, x.size), order="F")
+    lib().orc_spline(x.size, x.ctypes.data, f.ctypes.data, cf.ctypes.data)
+    return cf
+
+
+def fderiv_int(x, f):
+    """fderiv.f90 with m = -1: the running spline integral g(n)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    g = np.zeros(x.size)
+    lib().orc_fderiv_int(x.size, x.ctypes.data, f.ctypes.data, g.ctypes.data)
+    return g
+
+
+def gaunt_yry(l1, l2, l3, m1, m2, m3) -> complex:
+    out = np.zeros(2)
+    lib().orc_gaunt_yry(l1, l2, l3, m1, m2, m3, out.ctypes.data)
+    return complex(out[0], out[1])
+
+
+def hmlint(sys_, lmaxvr=None, veffmt=None, lofr=None, valence_rel_none=None):
+    """mt_kin + mt_pot (hmlint.f90:53-55) from zero tables: (aa, loa, lolo)
+    with the shapes of mt_hscf%main."""
+    s = sys_.s
+    lmaxvr = s.lmaxvr if lmaxvr is None else lmaxvr
+    veffmt = s.veffmt if veffmt is None else veffmt
+    lofr = s.lofr if lofr is None else lofr
+    relnone = s.valence_rel_none if valence_rel_none is None else valence_rel_none
+    mnlo = max(s.maxnlo, 1)
+    aa = np.zeros((s.maxaa, s.maxaa, s.natmtot), dtype=np.complex128, order="F")
+    loa = np.zeros((mnlo, s.maxaa, s.natmtot), dtype=np.complex128, order="F")
+    lolo = np.zeros((mnlo, mnlo, s.natmtot), dtype=np.complex128, order="F")
+    v = np.asfortranarray(veffmt, dtype=np.float64)
+    lf = np.asfortranarray(lofr, dtype=np.float64)
+    rc = lib().orc_hmlint(sys_.ptr, lmaxvr, v.ctypes.data, lf.ctypes.data, 1 if relnone else 0,
+                          aa.ctypes.data, loa.ctypes.data, lolo.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_hmlint failed: {rc}")
+    return aa, loa, lolo

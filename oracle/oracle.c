@@ -965,3 +965,348 @@ void orc_gencfunig(int ng, const int *ivg, const double *bvec, double omega,
     cfunig[ig] = acc;
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* src/src_lib/spline.f90:13-96 (ld = 1)                                */
+/* ------------------------------------------------------------------ */
+void orc_spline(int n, const double *x, const double *f, double *cf) {
+#define CF(k, i) cf[(k - 1) + 3 * (size_t)((i) - 1)]
+#define X(i) x[(i) - 1]
+#define F(i) f[(i) - 1]
+  double *w = (double *)malloc(sizeof(double) * (size_t)n);
+  double t1, t2, t3, t4;
+  for (int i = 1; i <= n - 1; i++) { /* :31-34 */
+    w[i - 1] = 1.0 / (X(i + 1) - X(i));
+    CF(1, i) = w[i - 1] * (F(i + 1) - F(i));
+  }
+  CF(2, 1) = 1.0; /* :35-43 */
+  CF(3, 1) = (CF(1, 2) - CF(1, 1)) / (X(3) - X(1));
+  t1 = (X(2) - X(1)) * w[1];
+  t2 = t1 * CF(2, 1);
+  t3 = 1.0 / (2.0 * (t1 + 1.0));
+  CF(2, 2) = t3;
+  t4 = 3.0 * (CF(1, 2) - CF(1, 1)) * w[1] - t2 * CF(3, 1);
+  CF(3, 2) = t4;
+  for (int i = 3; i <= n - 1; i++) { /* :44-51 */
+    t1 = (X(i) - X(i - 1)) * w[i - 1];
+    t2 = t1 * t3;
+    t3 = 1.0 / (2.0 * t1 + 2.0 - t2);
+    CF(2, i) = t3;
+    t4 = 3.0 * (CF(1, i) - CF(1, i - 1)) * w[i - 1] - t2 * t4;
+    CF(3, i) = t4;
+  }
+  CF(2, n) = 1.0; /* :52-59 */
+  t1 = (CF(1, n - 1) - CF(1, n - 2)) / (X(n) - X(n - 2));
+  CF(3, n) = t1;
+  for (int i = n - 1; i >= 2; i--) {
+    t1 = CF(3, i) - t1 * CF(2, i + 1);
+    CF(3, i) = t1;
+  }
+  for (int i = 1; i <= n; i++) CF(2, i) = CF(2, i) * CF(3, i); /* :60-62 */
+  for (int i = 1; i <= n - 1; i++) { /* :63-68 */
+    t1 = 0.3333333333333333333 * (CF(2, i + 1) - CF(2, i)) * w[i - 1];
+    CF(3, i) = t1;
+    t2 = X(i + 1) - X(i);
+    CF(1, i) = CF(1, i) - (CF(2, i) + t1 * t2) * t2;
+  }
+  t1 = X(n) - X(n - 1); /* :69-72 */
+  CF(1, n) = CF(1, n - 1) + (2.0 * CF(2, n - 1) + 3.0 * CF(3, n - 1) * t1) * t1;
+  CF(3, n) = CF(3, n - 1);
+  free(w);
+#undef CF
+#undef X
+#undef F
+}
+
+/* src/src_lib/fderiv.f90:13-80, the m <= -1 branch (:64-70). */
+void orc_fderiv_int(int n, const double *x, const double *f, double *g) {
+  double *cf = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  orc_spline(n, x, f, cf);
+  g[0] = 0.0;
+  for (int i = 0; i < n - 1; i++) {
+    const double dx = x[i + 1] - x[i];
+    g[i + 1] = g[i] + (((0.25 * cf[3 * i + 2] * dx + 0.3333333333333333333 * cf[3 * i + 1]) * dx +
+                        0.5 * cf[3 * i]) * dx + f[i]) * dx;
+  }
+  free(cf);
+}
+
+/* ------------------------------------------------------------------ */
+/* src/math/wigner3j_symbol.f90                                         */
+/* ------------------------------------------------------------------ */
+double orc_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) { /* :31-80 */
+  if (j1 == 0 && j2 == 0 && j3 == 0) return 1.0;
+  const int l1 = j2 - j1 + j3, l2 = j1 - j2 + j3, l3 = j1 + j2 - j3;
+  if (m1 + m2 + m3 != 0 || l1 < 0 || l2 < 0 || l3 < 0) return 0.0;
+  const int n1 = j1 - m1, n2 = j2 + m2;
+  int k1 = 0;
+  if (j2 - j3 - m1 > k1) k1 = j2 - j3 - m1;
+  if (j1 - j3 + m2 > k1) k1 = j1 - j3 + m2;
+  int k2 = l3;
+  if (n1 < k2) k2 = n1;
+  if (n2 < k2) k2 = n2;
+  double sgn = ((k1 + j1 + j2 + m3) % 2 == 0) ? 1.0 : -1.0;
+  double add = 0.0;
+  for (int k = k1; k <= k2; k++) {
+    const double t1 = sgn * orc_factr(l1, l1 - n2 + k) * orc_factr(l2, l2 - n1 + k) *
+                      orc_factr(l3, l3 - k);
+    add = add + t1 / (orc_factnm(k, 1) * orc_factnm(n1 - k, 1) * orc_factnm(n2 - k, 1));
+    sgn = -sgn;
+  }
+  const double t1 = orc_factr(j1 + m1, l1) * orc_factr(j2 + m2, l2) * orc_factr(j3 + m3, l3) *
+                    orc_factr(j3 - m3, 1 + j1 + j2 + j3) * orc_factnm(j1 - m1, 1) *
+                    orc_factnm(j2 - m2, 1);
+  return add * sqrt(t1);
+}
+
+double orc_gaunt_yyy(int l1, int l2, int l3, int m1, int m2, int m3) { /* :115-152 */
+  const double isqrtfourpi = 1.0 / sqrt(12.566370614359172954);
+  if (m1 - m2 - m3 != 0) return 0.0;
+  const int j1 = l2 - l1 + l3, j2 = l1 - l2 + l3, j3 = l1 + l2 - l3;
+  if (j1 < 0 || j2 < 0 || j3 < 0) return 0.0;
+  const int j = l1 + l2 + l3;
+  if (j % 2 != 0) return 0.0;
+  const int jh = j / 2;
+  double t1 = sqrt((double)((2 * l1 + 1) * (2 * l2 + 1) * (2 * l3 + 1)) * orc_factr(j1, j + 1) *
+                   orc_factnm(j2, 1) * orc_factnm(j3, 1));
+  t1 = t1 * orc_factr(jh, jh - l1) / (orc_factnm(jh - l2, 1) * orc_factnm(jh - l3, 1));
+  double g = t1 * isqrtfourpi * orc_wigner3j(l1, l2, l3, -m1, m2, m3);
+  if (((m1 + jh) % 2 + 2) % 2 != 0) g = -g;
+  return g;
+}
+
+void orc_gaunt_yry(int l1, int l2, int l3, int m1, int m2, int m3, double *out) { /* :166-194 */
+  const double sqrthalf = 1.4142135623730950488 / 2.0;
+  if (m2 > 0) {
+    const double t1 = (m2 % 2 == 0) ? 1.0 : -1.0;
+    out[0] = sqrthalf * (orc_gaunt_yyy(l1, l2, l3, m1, m2, m3) +
+                         t1 * orc_gaunt_yyy(l1, l2, l3, m1, -m2, m3));
+    out[1] = 0.0;
+  } else if (m2 < 0) {
+    const double t1 = ((-m2) % 2 == 0) ? 1.0 : -1.0;
+    const double t2 = sqrthalf * (orc_gaunt_yyy(l1, l2, l3, m1, m2, m3) -
+                                  t1 * orc_gaunt_yyy(l1, l2, l3, m1, -m2, m3));
+    out[0] = 0.0;
+    out[1] = g_fault == 7 ? t2 : -t2; /* planted mistake: sign of the imaginary part */
+  } else {
+    out[0] = orc_gaunt_yyy(l1, l2, l3, m1, m2, m3);
+    out[1] = 0.0;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* hmlint.f90:53-55: mt_kin then mt_pot for the spinless Hamiltonian    */
+/* ------------------------------------------------------------------ */
+int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const double *lofr,
+               int valence_rel_none, orc_z *aa, orc_z *loa, orc_z *lolo) {
+  const double y00 = 0.28209479177387814347; /* constants.f90:90 */
+  const double alpha = 1.0 / 137.03599911;
+  const int lmmaxvr = (lmaxvr + 1) * (lmaxvr + 1);
+  const int lmaxmat = s->lmaxmat, lmmaxmat = (lmaxmat + 1) * (lmaxmat + 1);
+  const int nrmtmax = s->nrmtmax, p = s->apwordmax, nlomax = s->nlomax;
+  const int maxaa = s->maxaa, mnlo = s->maxnlo > 1 ? s->maxnlo : 1;
+  const double a = valence_rel_none ? 0.0 : 0.5 * alpha * alpha; /* mt_kin.f90:55-59 */
+  /* Gaunt list gntnonz (init1.f90:563-609): for lm1 then lm3 (both <= lmmaxmat),
+   * the lm2 <= lmmaxvr with |gntryy(lm2,lm3,lm1)| > 1e-20, in order. */
+  const int npairs = lmmaxmat * lmmaxmat;
+  int *start = (int *)calloc((size_t)npairs + 1, sizeof(int));
+  int cap = 1024, nnz = 0;
+  int *glm2 = (int *)malloc(sizeof(int) * cap);
+  orc_z *gval = (orc_z *)malloc(sizeof(orc_z) * cap);
+  for (int l1 = 0; l1 <= lmaxmat; l1++)
+    for (int m1 = -l1; m1 <= l1; m1++)
+      for (int l3 = 0; l3 <= lmaxmat; l3++)
+        for (int m3 = -l3; m3 <= l3; m3++) {
+          const int lm1 = idxlm(l1, m1), lm3 = idxlm(l3, m3);
+          start[lm1 * lmmaxmat + lm3] = nnz;
+          for (int l2 = 0; l2 <= lmaxvr; l2++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+              double g[2];
+              orc_gaunt_yry(l1, l2, l3, m1, m2, m3, g); /* gntryy(lm2,lm3,lm1), :577-589 */
+              if (cabs(zmk(g[0], g[1])) > 1e-20) {
+                if (nnz == cap) {
+                  cap *= 2;
+                  glm2 = (int *)realloc(glm2, sizeof(int) * cap);
+                  gval = (orc_z *)realloc(gval, sizeof(orc_z) * cap);
+                }
+                glm2[nnz] = idxlm(l2, m2);
+                gval[nnz] = zmk(g[0], g[1]);
+                nnz++;
+              }
+            }
+        }
+  start[npairs] = nnz;
+  /* gather the pair list end of each (lm1, lm3) */
+  int *endp = (int *)malloc(sizeof(int) * (size_t)npairs);
+  for (int q = 0; q < npairs; q++) endp[q] = q + 1 < npairs ? start[q + 1] : nnz;
+
+  double *r2 = (double *)malloc(sizeof(double) * nrmtmax);
+  double *r2inv = (double *)malloc(sizeof(double) * nrmtmax);
+  double *rmtable = (double *)malloc(sizeof(double) * nrmtmax);
+  double *fr = (double *)malloc(sizeof(double) * nrmtmax);
+  double *gr = (double *)malloc(sizeof(double) * nrmtmax);
+  /* radial functions of one atom: APW (l, io) for l <= lmaxmat, then LOs */
+  const int nfa = (lmaxmat + 1) * p, nf = nfa + nlomax;
+  double *Ipot = (double *)malloc(sizeof(double) * (size_t)nf * nf * lmmaxvr);
+  double *Ikin = (double *)malloc(sizeof(double) * (size_t)nf * nf);
+#define APWFR(ir, k, io, l, ias) \
+  s->apwfr[(ir) + (size_t)nrmtmax * ((k) + 2 * ((io) + (size_t)p * ((l) + (size_t)(s->lmaxapw + 1) * (ias))))]
+#define LOFR(ir, k, ilo, ias) lofr[(ir) + (size_t)nrmtmax * ((k) + 2 * ((ilo) + (size_t)nlomax * (ias)))]
+#define VEFF(lm, ir, ias) veffmt[(lm) + (size_t)lmmaxvr * ((ir) + (size_t)nrmtmax * (ias))]
+  int ias = 0;
+  for (int is = 0; is < s->nspecies; is++) {
+    const int nr = s->nrmt[is];
+    const double *spr = s->spr + (size_t)nrmtmax * is;
+    for (int ir = 0; ir < nr; ir++) { /* mt_kin.f90:101-104 */
+      r2[ir] = spr[ir] * spr[ir];
+      r2inv[ir] = 1.0 / r2[ir];
+    }
+    for (int ia = 0; ia < s->natoms[is]; ia++, ias++) {
+      for (int ir = 0; ir < nr; ir++) rmtable[ir] = 1.0 / (1.0 - a * VEFF(0, ir, ias) * y00);
+      /* radial function value (k = 0) or derivative (k = 1) of function fi */
+#define RF(fi, ir, k) ((fi) < nfa ? APWFR(ir, k, (fi) % p, (fi) / p, ias) : LOFR(ir, k, (fi) - nfa, ias))
+#define LOF(fi) ((fi) < nfa ? (fi) / p : lorbl_of(s, (fi) - nfa, is))
+#define VALID(fi) ((fi) < nfa ? ((fi) % p) < apword_of(s, (fi) / p, is) : ((fi) - nfa) < s->nlorb[is])
+      /* kinetic integrals, equal l only (mt_kin.f90:117-170) */
+      for (int f1 = 0; f1 < nf; f1++)
+        for (int f2 = 0; f2 < nf; f2++) {
+          Ikin[f1 * nf + f2] = 0.0;
+          if (!VALID(f1) || !VALID(f2) || LOF(f1) != LOF(f2)) continue;
+          const int l = LOF(f1);
+          const double angular = (double)(l * (l + 1));
+          for (int ir = 0; ir < nr; ir++) {
+            const double t1 = RF(f1, ir, 0) * RF(f2, ir, 0);
+            const double t2 = RF(f1, ir, 1) * RF(f2, ir, 1);
+            fr[ir] = (0.5 * t2 * rmtable[ir] + 0.5 * angular * t1 * rmtable[ir] * r2inv[ir]) * r2[ir];
+          }
+          orc_fderiv_int(nr, spr, fr, gr);
+          Ikin[f1 * nf + f2] = gr[nr - 1] / y00;
+        }
+      /* potential integrals (mt_pot.f90:131-218) */
+      for (int f1 = 0; f1 < nf; f1++)
+        for (int f2 = 0; f2 < nf; f2++) {
+          if (!VALID(f1) || !VALID(f2)) continue;
+          for (int lm2 = 0; lm2 < lmmaxvr; lm2++) {
+            for (int ir = 0; ir < nr; ir++) {
+              const double t1 = RF(f1, ir, 0) * RF(f2, ir, 0) * r2[ir];
+              fr[ir] = t1 * VEFF(lm2, ir, ias);
+            }
+            orc_fderiv_int(nr, spr, fr, gr);
+            Ipot[((size_t)f1 * nf + f2) * lmmaxvr + lm2] = gr[nr - 1];
+          }
+        }
+      /* index of the APW function f = (l, io) and its row if = (l, m, io) */
+      int if1 = 0;
+      /* mt_kin.f90:173-186 then mt_pot.f90:227-258: aa */
+      for (int l1 = 0; l1 <= lmaxmat; l1++)
+        for (int m1 = -l1; m1 <= l1; m1++)
+          for (int io1 = 0; io1 < apword_of(s, l1, is); io1++, if1++) {
+            int if3 = 0;
+            for (int l3 = 0; l3 <= lmaxmat; l3++)
+              for (int m3 = -l3; m3 <= l3; m3++)
+                for (int io2 = 0; io2 < apword_of(s, l3, is); io2++, if3++) {
+                  const int fa = l1 * p + io1, fb = l3 * p + io2;
+                  orc_z *e = aa + if1 + (size_t)maxaa * (if3 + (size_t)maxaa * ias);
+                  if (l1 == l3 && m1 == m3) /* kinetic: haaintegrals(1,io2,l1,io1,l1)*y00 */
+                    *e = zadd(*e, zmk(Ikin[fb * nf + fa] * y00, 0.0));
+                }
+          }
+      if1 = 0;
+      for (int l1 = 0; l1 <= lmaxmat; l1++)
+        for (int m1 = -l1; m1 <= l1; m1++)
+          for (int io1 = 0; io1 < apword_of(s, l1, is); io1++, if1++) {
+            int if3 = 0;
+            for (int l3 = 0; l3 <= lmaxmat; l3++)
+              for (int m3 = -l3; m3 <= l3; m3++)
+                for (int io2 = 0; io2 < apword_of(s, l3, is); io2++, if3++) {
+                  const int q = idxlm(l1, m1) * lmmaxmat + idxlm(l3, m3);
+                  const int fa = l1 * p + io1, fb = l3 * p + io2;
+                  orc_z zsum = 0.0;
+                  for (int t = start[q]; t < endp[q]; t++) /* haaintegrals(lm2,io2,l3,io1,l1) */
+                    zsum = zadd(zsum, zmul(gval[t], zmk(Ipot[((size_t)fb * nf + fa) * lmmaxvr + glm2[t]], 0.0)));
+                  orc_z *e = aa + if1 + (size_t)maxaa * (if3 + (size_t)maxaa * ias);
+                  *e = zadd(*e, zsum);
+                }
+          }
+      /* loa (mt_kin.f90:189-206, mt_pot.f90:260-287) and lolo (mt_kin.f90:208-226,
+       * mt_pot.f90:289-311) */
+      if (s->maxnlo > 0) {
+        int r1 = 0;
+        for (int ilo = 0; ilo < s->nlorb[is]; ilo++) {
+          const int l1 = lorbl_of(s, ilo, is);
+          for (int m1 = -l1; m1 <= l1; m1++, r1++) {
+            int if3 = 0;
+            for (int l3 = 0; l3 <= lmaxmat; l3++)
+              for (int m3 = -l3; m3 <= l3; m3++)
+                for (int io = 0; io < apword_of(s, l3, is); io++, if3++) {
+                  const int fl = nfa + ilo, fb = l3 * p + io;
+                  orc_z *e = loa + r1 + (size_t)mnlo * (if3 + (size_t)maxaa * ias);
+                  if (idxlm(l1, m1) == idxlm(l3, m3))
+                    *e = zadd(*e, zmk(Ikin[fb * nf + fl] * y00, 0.0));
+                }
+          }
+        }
+        r1 = 0;
+        for (int ilo = 0; ilo < s->nlorb[is]; ilo++) {
+          const int l1 = lorbl_of(s, ilo, is);
+          for (int m1 = -l1; m1 <= l1; m1++, r1++) {
+            int if3 = 0;
+            for (int l3 = 0; l3 <= lmaxmat; l3++)
+              for (int m3 = -l3; m3 <= l3; m3++)
+                for (int io = 0; io < apword_of(s, l3, is); io++, if3++) {
+                  const int q = idxlm(l1, m1) * lmmaxmat + idxlm(l3, m3);
+                  const int fl = nfa + ilo, fb = l3 * p + io;
+                  orc_z zsum = 0.0;
+                  for (int t = start[q]; t < endp[q]; t++) /* halointegrals(lm2,io,l3,ilo) */
+                    zsum = zadd(zsum, zmul(gval[t], zmk(Ipot[((size_t)fl * nf + fb) * lmmaxvr + glm2[t]], 0.0)));
+                  orc_z *e = loa + r1 + (size_t)mnlo * (if3 + (size_t)maxaa * ias);
+                  *e = zadd(*e, zsum);
+                }
+          }
+        }
+        r1 = 0;
+        for (int ilo1 = 0; ilo1 < s->nlorb[is]; ilo1++) {
+          const int l1 = lorbl_of(s, ilo1, is);
+          for (int m1 = -l1; m1 <= l1; m1++, r1++) {
+            int r3 = 0;
+            for (int ilo2 = 0; ilo2 < s->nlorb[is]; ilo2++) {
+              const int l3 = lorbl_of(s, ilo2, is);
+              for (int m3 = -l3; m3 <= l3; m3++, r3++) {
+                orc_z *e = lolo + r1 + (size_t)mnlo * (r3 + (size_t)mnlo * ias);
+                if (idxlm(l1, m1) == idxlm(l3, m3))
+                  *e = zadd(*e, zmk(Ikin[(nfa + ilo1) * nf + nfa + ilo2] * y00, 0.0));
+              }
+            }
+          }
+        }
+        r1 = 0;
+        for (int ilo1 = 0; ilo1 < s->nlorb[is]; ilo1++) {
+          const int l1 = lorbl_of(s, ilo1, is);
+          for (int m1 = -l1; m1 <= l1; m1++, r1++) {
+            int r3 = 0;
+            for (int ilo2 = 0; ilo2 < s->nlorb[is]; ilo2++) {
+              const int l3 = lorbl_of(s, ilo2, is);
+              for (int m3 = -l3; m3 <= l3; m3++, r3++) {
+                const int q = idxlm(l1, m1) * lmmaxmat + idxlm(l3, m3);
+                orc_z zsum = 0.0;
+                for (int t = start[q]; t < endp[q]; t++) /* hlolointegrals(lm2,ilo1,ilo2) */
+                  zsum = zadd(zsum, zmul(gval[t], zmk(Ipot[((size_t)(nfa + ilo1) * nf + nfa + ilo2) * lmmaxvr + glm2[t]], 0.0)));
+                orc_z *e = lolo + r1 + (size_t)mnlo * (r3 + (size_t)mnlo * ias);
+                *e = zadd(*e, zsum);
+              }
+            }
+          }
+        }
+      }
+#undef RF
+#undef LOF
+#undef VALID
+    }
+  }
+#undef APWFR
+#undef LOFR
+#undef VEFF
+  free(start); free(endp); free(glm2); free(gval); free(r2); free(r2inv); free(rmtable);
+  free(fr); free(gr); free(Ipot); free(Ikin);
+  return 0;
+}

@@ -140,6 +140,27 @@ void orc_gencfunig(int ng, const int *ivg, const double *bvec, double omega,
                    int nspecies, const int *natoms, const double *rmt,
                    const double *atposc, double epslat, orc_z *cfunig);
 
+/* src/src_lib/spline.f90:13-96: cubic spline coefficients cf(3,n) of f on
+ * the grid x (ld = 1).  src/src_lib/fderiv.f90:13-80 for m = -1: the running
+ * integral g(n) of f.  n >= 3 (the reference reads x(3)). */
+void orc_spline(int n, const double *x, const double *f, double *cf);
+void orc_fderiv_int(int n, const double *x, const double *f, double *g);
+
+/* src/math/wigner3j_symbol.f90: wigner3j (:31-80), gaunt_yyy (:115-152),
+ * gaunt_yry (:166-194, value through out[0..1]). */
+double orc_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
+double orc_gaunt_yyy(int l1, int l2, int l3, int m1, int m2, int m3);
+void orc_gaunt_yry(int l1, int l2, int l3, int m1, int m2, int m3, double *out);
+
+/* hmlint.f90:53-55 for the spinless main Hamiltonian: mt_kin.f90:14-237
+ * then mt_pot.f90:14-323, accumulated into aa(maxaa,maxaa,natmtot),
+ * loa(maxnlo,maxaa,natmtot), lolo(maxnlo,maxnlo,natmtot) (maxnlo >= 1 as
+ * allocated).  veffmt(lmmaxvr,nrmtmax,natmtot), lmmaxvr = (lmaxvr+1)^2;
+ * lofr(nrmtmax,2,nlomax,natmtot); apwfr and spr from the system.  The Gaunt
+ * list is init1.f90:563-609. */
+int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const double *lofr,
+               int valence_rel_none, orc_z *aa, orc_z *loa, orc_z *lolo);
+
 /* Planted-mistake switch for tests: 0 = faithful.  Nonzero values inject one
  * deliberate error so tests can show the checks catch it. */
 void orc_set_fault(int code);
