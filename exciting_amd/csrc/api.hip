@@ -285,7 +285,10 @@ void exh_destroy(exh_context *c) {
                     &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex, &c->ozeb, &c->s_apwalm, &c->s_igpig,
                     &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac,
                     &c->bbt, &c->bat, &c->ba2t, &c->bivgk, &c->ifz, &c->igeo, &c->i_in0,
-                    &c->i_in1, &c->i_out0, &c->i_out1, &c->i_out2};
+                    &c->i_in1, &c->i_out0, &c->i_out1, &c->i_out2, &c->apwfr_d, &c->hml_fl,
+                    &c->hml_arow, &c->hml_lrow, &c->hml_count, &c->hml_gstart, &c->hml_glm2,
+                    &c->hml_gval, &c->hml_w, &c->hml_spr, &c->hml_ipot, &c->hml_ikin,
+                    &c->hml_vin, &c->hml_loin, &c->hml_aa, &c->hml_loa, &c->hml_lolo};
   for (DevBuf *b : bufs) release(*b);
   for (auto &r : c->recs) {
     (void)hipEventDestroy(r.a);
@@ -416,6 +419,7 @@ int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, 
   if (!rc) rc = upload(c, c->lorow, c->lorow_h.data(), c->lorow_h.size());
   if (rc) return rc;
   c->have_basis = true;
+  c->basis_epoch++;
   return 0;
 }
 
@@ -435,7 +439,9 @@ int exh_set_radial(exh_context *c, double omega, const double *rmt, const int *n
   c->nrmtmax = nrmtmax;
   c->spr_h.assign(spr, spr + (size_t)nrmtmax * c->nspecies);
   c->apwfr_h.assign(apwfr, apwfr + (size_t)nrmtmax * 2 * c->apwordmax * (c->lmaxapw + 1) * c->natmtot);
+  c->radial_epoch++;
   int rc = upload(c, c->rmt_d, c->rmt.data(), c->rmt.size());
+  if (!rc) rc = upload(c, c->apwfr_d, c->apwfr_h.data(), c->apwfr_h.size());
   if (!rc) rc = build_match_tables(c);
   if (rc) return rc;
   c->have_radial = true;

@@ -122,6 +122,12 @@ struct exh_context {
   void *fft_plan[4] = {nullptr, nullptr, nullptr, nullptr};
   int fft_grid[3] = {0, 0, 0};
   bool gvec_full = false;  // G set = the whole FFT box, ivgig inverse of ivg
+  // muffin-tin table producer (hmlint): apwfr on the device, species and
+  // Gaunt tables, spline weights, radial integrals, host-API staging
+  exh::DevBuf apwfr_d, hml_fl, hml_arow, hml_lrow, hml_count, hml_gstart, hml_glm2, hml_gval;
+  exh::DevBuf hml_w, hml_spr, hml_ipot, hml_ikin, hml_vin, hml_loin, hml_aa, hml_loa, hml_lolo;
+  int hml_lmaxvr = -1;
+  long long radial_epoch = 0, basis_epoch = 0, hml_radial_epoch = -1, hml_basis_epoch = -1;
 
   // instrumentation: (kernel id, start, stop) per recorded launch
   bool prof = false;

@@ -1,0 +1,609 @@
+// Muffin-tin Hamiltonian tables on the device: hmlint.f90:53-55 for the
+// spinless main Hamiltonian, i.e. mt_kin.f90:14-237 (kinetic, diagonal in
+// (l, m)) followed by mt_pot.f90:14-323 (potential, coupled through the
+// complex Gaunt coefficients of init1.f90:563-609), producing
+// mt_hscf%main%aa(maxaa,maxaa,natmtot), %loa(maxnlo,maxaa,natmtot) and
+// %lolo(maxnlo,maxnlo,natmtot).  exh_set_mt_device installs such tables
+// (and olprad's oalo/ololo, iora*'s h1 terms) from device memory.
+//
+// Two kernels:
+//   radial    every radial integral of one atom: for each pair of radial
+//             functions (APW (l, io) for l <= lmaxmat, then the LOs) and each
+//             lm of the potential, the spline integral of f1 f2 r^2 V_lm;
+//             plus the kinetic integral of equal-l pairs.  The spline
+//             integral of fderiv.f90 (m = -1) with spline.f90 is linear in
+//             the integrand, so it is a weighted sum with per-species
+//             weights computed once on the host by applying that rule to unit
+//             vectors.  Radial functions and a slab of potential channels are
+//             staged in LDS per chunk of grid points.
+//   contract  one table element per thread: the sum over the Gaunt list of
+//             (lm1, lm3) of coefficient x radial integral, plus the kinetic
+//             integral when lm1 = lm3, in the reference's order.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "exh_device.h"
+
+namespace exh {
+namespace {
+
+constexpr int kRC = 32;        // grid points per LDS chunk
+constexpr int kCH = 8;         // potential channels (lm) per workgroup
+constexpr int kNfMax = 63;     // radial functions per atom (APW + LO)
+constexpr int kThreads = 256;
+
+struct HmlArgs {
+  int nf, nfa, p, lmaxapw, nrmtmax, nlomax, lmmaxvr, nchunk;
+  double zora;   // alpha^2/2 for non-'none' relativity, else 0 (mt_kin.f90:55-59)
+};
+
+__device__ __forceinline__ double radial_value(const HmlArgs &a, const double *__restrict__ apwfr,
+                                               const double *__restrict__ lofr, int ias, int f,
+                                               int ir, int k) {
+  if (f < a.nfa) {
+    const int l = f / a.p, io = f - l * a.p;
+    return apwfr[ir + (size_t)a.nrmtmax *
+                          (k + 2 * (io + (size_t)a.p * (l + (size_t)(a.lmaxapw + 1) * ias)))];
+  }
+  return lofr[ir + (size_t)a.nrmtmax * (k + 2 * ((f - a.nfa) + (size_t)a.nlomax * ias))];
+}
+
+// pair t of the upper triangle (f1 <= f2) of nf functions
+__device__ __forceinline__ void pair_of(int t, int &f1, int &f2) {
+  int b = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((b + 1) * (b + 2) / 2 <= t) b++;
+  while (b * (b + 1) / 2 > t) b--;
+  f2 = b;
+  f1 = t - b * (b + 1) / 2;
+}
+
+// blockIdx.x = atom, blockIdx.y = potential slab (lm0 = y * kCH) or, for
+// y == nchunk, the kinetic integrals.  ipot[ias][f1][f2][lm], ikin[ias][f1][f2],
+// both written for (f1,f2) and (f2,f1).
+template <int PPT>
+__global__ __launch_bounds__(kThreads) void hml_radial_kernel(
+    HmlArgs a, const int *__restrict__ atom_species, const int *__restrict__ nrmt,
+    const double *__restrict__ wts, const double *__restrict__ spr,
+    const int *__restrict__ fl, const double *__restrict__ apwfr,
+    const double *__restrict__ lofr, const double *__restrict__ veffmt,
+    double *__restrict__ ipot, double *__restrict__ ikin) {
+  __shared__ double sF[kRC][kNfMax + 1];
+  __shared__ double sD[kRC][kNfMax + 1];
+  __shared__ double sV[kRC][kCH];
+  __shared__ double sW[kRC], sR2[kRC];
+  const int ias = blockIdx.x, is = atom_species[ias], nr = nrmt[is];
+  const bool kin = (int)blockIdx.y == a.nchunk;
+  const int lm0 = blockIdx.y * kCH;
+  const int nf = a.nf, npair = nf * (nf + 1) / 2, tid = threadIdx.x;
+  const double y00 = 0.28209479177387814347;
+  double acc[PPT][kCH];
+#pragma unroll
+  for (int q = 0; q < PPT; q++)
+#pragma unroll
+    for (int k = 0; k < kCH; k++) acc[q][k] = 0.0;
+  int pf1[PPT], pf2[PPT];
+  double lfac[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    const int t = tid + q * kThreads;
+    pf1[q] = pf2[q] = 0;
+    lfac[q] = 0.0;
+    if (t < npair) {
+      pair_of(t, pf1[q], pf2[q]);
+      const int l = fl[is * kNfMax + pf1[q]];
+      lfac[q] = (double)(l * (l + 1));
+    }
+  }
+  for (int r0 = 0; r0 < nr; r0 += kRC) {
+    for (int i = tid; i < kRC * nf; i += kThreads) {
+      const int r = i / nf, f = i - r * nf, ir = r0 + r;
+      sF[r][f] = ir < nr ? radial_value(a, apwfr, lofr, ias, f, ir, 0) : 0.0;
+      if (kin) sD[r][f] = ir < nr ? radial_value(a, apwfr, lofr, ias, f, ir, 1) : 0.0;
+    }
+    if (kin) {
+      for (int r = tid; r < kRC; r += kThreads) {
+        const int ir = r0 + r;
+        double w = 0.0, r2 = 0.0;
+        if (ir < nr) {
+          const double x = spr[ir + (size_t)a.nrmtmax * is];
+          r2 = x * x;
+          const double v00 = veffmt[(size_t)a.lmmaxvr * (ir + (size_t)a.nrmtmax * ias)];
+          w = 0.5 * wts[ir + (size_t)a.nrmtmax * is] / (1.0 - a.zora * v00 * y00);
+        }
+        sW[r] = w;
+        sR2[r] = r2;
+      }
+    } else {
+      for (int i = tid; i < kRC * kCH; i += kThreads) {
+        const int r = i / kCH, k = i - r * kCH, ir = r0 + r, lm = lm0 + k;
+        sV[r][k] = (ir < nr && lm < a.lmmaxvr)
+                       ? veffmt[lm + (size_t)a.lmmaxvr * (ir + (size_t)a.nrmtmax * ias)]
+                       : 0.0;
+      }
+      for (int r = tid; r < kRC; r += kThreads) {
+        const int ir = r0 + r;
+        double w = 0.0;
+        if (ir < nr) {
+          const double x = spr[ir + (size_t)a.nrmtmax * is];
+          w = wts[ir + (size_t)a.nrmtmax * is] * (x * x);
+        }
+        sW[r] = w;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+      if (tid + q * kThreads >= npair) continue;
+      const int f1 = pf1[q], f2 = pf2[q];
+      if (kin) {
+        double s = acc[q][0];
+        for (int r = 0; r < kRC; r++)
+          s += sW[r] * (sD[r][f1] * sD[r][f2] * sR2[r] + lfac[q] * (sF[r][f1] * sF[r][f2]));
+        acc[q][0] = s;
+      } else {
+        for (int r = 0; r < kRC; r++) {
+          const double pr = sF[r][f1] * sF[r][f2] * sW[r];
+#pragma unroll
+          for (int k = 0; k < kCH; k++) acc[q][k] += pr * sV[r][k];
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    if (tid + q * kThreads >= npair) continue;
+    const int f1 = pf1[q], f2 = pf2[q];
+    if (kin) {
+      const bool same = fl[is * kNfMax + f1] == fl[is * kNfMax + f2];
+      const double v = same ? acc[q][0] : 0.0;
+      ikin[((size_t)ias * nf + f1) * nf + f2] = v;
+      ikin[((size_t)ias * nf + f2) * nf + f1] = v;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kCH; k++) {
+        const int lm = lm0 + k;
+        if (lm >= a.lmmaxvr) break;
+        ipot[(((size_t)ias * nf + f1) * nf + f2) * a.lmmaxvr + lm] = acc[q][k];
+        ipot[(((size_t)ias * nf + f2) * nf + f1) * a.lmmaxvr + lm] = acc[q][k];
+      }
+    }
+  }
+}
+
+// Rows of one table kind per species: int2 {lm (0-based), radial function}.
+struct RowTable {
+  const int2 *rows;    // [nspecies][ld]
+  const int *count;    // [nspecies]
+  int ld;
+};
+
+// out(i, j, ias) (+)= [lm_i == lm_j] kin(f_i, f_j) + sum_t g_t ipot(f_i, f_j, lm2_t)
+// over the Gaunt list of (lm_i, lm_j): aa (APW x APW), loa (LO x APW) and
+// lolo (LO x LO) of mt_kin.f90:173-226 and mt_pot.f90:227-311.
+__global__ __launch_bounds__(kThreads) void hml_contract_kernel(
+    int nf, int lmmaxvr, int lmmaxmat, const int *__restrict__ atom_species, RowTable R,
+    RowTable C, int ldo, int nco, const int *__restrict__ gstart, const int *__restrict__ glm2,
+    const double2 *__restrict__ gval, const double *__restrict__ ipot,
+    const double *__restrict__ ikin, double2 *__restrict__ out, int accumulate) {
+  const int ias = blockIdx.y;
+  const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= (long long)ldo * nco) return;
+  const int i = (int)(e % ldo), j = (int)(e / ldo);
+  const int is = atom_species[ias];
+  double2 *o = out + (size_t)i + (size_t)ldo * (j + (size_t)nco * ias);
+  double re = 0.0, im = 0.0;
+  if (accumulate) {
+    re = o->x;
+    im = o->y;
+  }
+  if (i < R.count[is] && j < C.count[is]) {
+    const int2 ri = R.rows[is * R.ld + i], cj = C.rows[is * C.ld + j];
+    const size_t base = ((size_t)ias * nf + ri.y) * nf + cj.y;
+    if (ri.x == cj.x) re += ikin[base];
+    double zr = 0.0, zi = 0.0;
+    const int q = ri.x * lmmaxmat + cj.x;
+    const double *ip = ipot + base * lmmaxvr;
+    for (int t = gstart[q]; t < gstart[q + 1]; t++) {
+      const double v = ip[glm2[t]];
+      const double2 g = gval[t];
+      zr += g.x * v;
+      zi += g.y * v;
+    }
+    re += zr;
+    im += zi;
+  }
+  *o = make_double2(re, im);
+}
+
+// ---- host-side tables --------------------------------------------------
+
+// Weights w with sum_i w_i f_i equal to the integral over the whole grid that
+// fderiv.f90 (m = -1) returns: its spline (spline.f90) and the per-interval
+// cubic integral are linear in f, so w_j is that rule applied to the j-th
+// unit vector.  The steps are the reference's (the rule has to be the same
+// for the same numbers).
+void spline_integral_weights(int n, const double *x, double *w) {
+  std::vector<double> cf(3 * (size_t)n), iw(n), f(n);
+  for (int j = 0; j < n; j++) {
+    std::fill(f.begin(), f.end(), 0.0);
+    f[j] = 1.0;
+    for (int i = 0; i + 1 < n; i++) {
+      iw[i] = 1.0 / (x[i + 1] - x[i]);
+      cf[3 * i] = iw[i] * (f[i + 1] - f[i]);
+    }
+    cf[1] = 1.0;
+    cf[2] = (cf[3] - cf[0]) / (x[2] - x[0]);
+    double t1 = (x[1] - x[0]) * iw[1];
+    double t2 = t1 * cf[1];
+    double t3 = 1.0 / (2.0 * (t1 + 1.0));
+    cf[3 + 1] = t3;
+    double t4 = 3.0 * (cf[3] - cf[0]) * iw[1] - t2 * cf[2];
+    cf[3 + 2] = t4;
+    for (int i = 2; i + 1 < n; i++) {
+      t1 = (x[i] - x[i - 1]) * iw[i];
+      t2 = t1 * t3;
+      t3 = 1.0 / (2.0 * t1 + 2.0 - t2);
+      cf[3 * i + 1] = t3;
+      t4 = 3.0 * (cf[3 * i] - cf[3 * (i - 1)]) * iw[i] - t2 * t4;
+      cf[3 * i + 2] = t4;
+    }
+    cf[3 * (n - 1) + 1] = 1.0;
+    t1 = (cf[3 * (n - 2)] - cf[3 * (n - 3)]) / (x[n - 1] - x[n - 3]);
+    cf[3 * (n - 1) + 2] = t1;
+    for (int i = n - 2; i >= 1; i--) {
+      t1 = cf[3 * i + 2] - t1 * cf[3 * (i + 1) + 1];
+      cf[3 * i + 2] = t1;
+    }
+    for (int i = 0; i < n; i++) cf[3 * i + 1] *= cf[3 * i + 2];
+    for (int i = 0; i + 1 < n; i++) {
+      t1 = (1.0 / 3.0) * (cf[3 * (i + 1) + 1] - cf[3 * i + 1]) * iw[i];
+      cf[3 * i + 2] = t1;
+      t2 = x[i + 1] - x[i];
+      cf[3 * i] = cf[3 * i] - (cf[3 * i + 1] + t1 * t2) * t2;
+    }
+    double g = 0.0;
+    for (int i = 0; i + 1 < n; i++) {
+      const double dx = x[i + 1] - x[i];
+      g += (((0.25 * cf[3 * i + 2] * dx + (1.0 / 3.0) * cf[3 * i + 1]) * dx + 0.5 * cf[3 * i]) *
+                dx + f[i]) * dx;
+    }
+    w[j] = g;
+  }
+}
+
+// Wigner 3j symbol by Racah's formula, in long double.
+long double fact(int n) {
+  long double r = 1.0L;
+  for (int i = 2; i <= n; i++) r *= i;
+  return r;
+}
+
+long double w3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+  if (m1 + m2 + m3 != 0) return 0.0L;
+  if (j3 < std::abs(j1 - j2) || j3 > j1 + j2) return 0.0L;
+  if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.0L;
+  const long double tri = fact(j1 + j2 - j3) * fact(j1 - j2 + j3) * fact(-j1 + j2 + j3) /
+                          fact(j1 + j2 + j3 + 1);
+  const long double pre = std::sqrt(tri * fact(j1 + m1) * fact(j1 - m1) * fact(j2 + m2) *
+                                    fact(j2 - m2) * fact(j3 + m3) * fact(j3 - m3));
+  const int kmin = std::max(0, std::max(j2 - j3 - m1, j1 - j3 + m2));
+  const int kmax = std::min(j1 + j2 - j3, std::min(j1 - m1, j2 + m2));
+  long double sum = 0.0L;
+  for (int k = kmin; k <= kmax; k++) {
+    const long double d = fact(k) * fact(j3 - j2 + k + m1) * fact(j3 - j1 + k - m2) *
+                          fact(j1 + j2 - j3 - k) * fact(j1 - k - m1) * fact(j2 - k + m2);
+    sum += ((k % 2) ? -1.0L : 1.0L) / d;
+  }
+  const int ph = j1 - j2 - m3;
+  return ((ph % 2 + 2) % 2 ? -1.0L : 1.0L) * pre * sum;
+}
+
+// Integral of conj(Y_l1m1) Y_l2m2 Y_l3m3 over the sphere.
+long double gaunt_complex(int l1, int l2, int l3, int m1, int m2, int m3) {
+  if (m1 != m2 + m3) return 0.0L;
+  const long double pi = 3.14159265358979323846264338327950288L;
+  const long double f =
+      std::sqrt((long double)((2 * l1 + 1) * (2 * l2 + 1) * (2 * l3 + 1)) / (4.0L * pi));
+  return ((m1 % 2 + 2) % 2 ? -1.0L : 1.0L) * f * w3j(l1, l2, l3, 0, 0, 0) *
+         w3j(l1, l2, l3, -m1, m2, m3);
+}
+
+// Same with the real harmonic R_l2m2 of genrlm.f90 (sqrt(2) Re Y_lm for m > 0,
+// Re Y_l0, sqrt(2) Im Y_lm for m < 0) in the middle.
+void gaunt_real_middle(int l1, int l2, int l3, int m1, int m2, int m3, double *re, double *im) {
+  const long double h = std::sqrt(0.5L);
+  const long double s = (std::abs(m2) % 2) ? -1.0L : 1.0L;
+  *re = *im = 0.0;
+  if (m2 > 0) {
+    *re = (double)(h * (gaunt_complex(l1, l2, l3, m1, m2, m3) +
+                        s * gaunt_complex(l1, l2, l3, m1, -m2, m3)));
+  } else if (m2 < 0) {
+    *im = (double)(-h * (gaunt_complex(l1, l2, l3, m1, m2, m3) -
+                         s * gaunt_complex(l1, l2, l3, m1, -m2, m3)));
+  } else {
+    *re = (double)gaunt_complex(l1, l2, l3, m1, 0, m3);
+  }
+}
+
+int cdiv_ll(long long a, int b) { return (int)((a + b - 1) / b); }
+
+// Species tables, Gaunt list and spline weights for (lmaxvr, the basis, the
+// radial grid): rebuilt when any of them changed.
+int prepare(exh_context *c, int lmaxvr) {
+  const int p = c->apwordmax, lmaxmat = c->lmaxmat;
+  const int nfa = (lmaxmat + 1) * p, nf = nfa + c->nlomax;
+  if (nf > kNfMax) return fail(c, EXH_ERANGE, "hmlint: more than 63 radial functions per atom");
+  for (int is = 0; is < c->nspecies; is++)
+    for (int ilo = 0; ilo < c->nlorb[is]; ilo++)
+      if (c->lorbl[ilo + c->nlomax * is] > lmaxmat)
+        return fail(c, EXH_ERANGE, "hmlint: a local orbital has l > lmaxmat");
+  if (c->hml_lmaxvr == lmaxvr && c->hml_radial_epoch == c->radial_epoch &&
+      c->hml_basis_epoch == c->basis_epoch)
+    return 0;
+  const int ns = c->nspecies, lmmaxmat = (lmaxmat + 1) * (lmaxmat + 1);
+  // l of each radial function per species (-1 = not a function of the species)
+  std::vector<int> fl((size_t)ns * kNfMax, -1);
+  std::vector<int2> arow((size_t)ns * c->maxaa), lrow((size_t)ns * std::max(c->maxnlo, 1));
+  std::vector<int> acount(ns), lcount(ns);
+  for (int is = 0; is < ns; is++) {
+    for (int l = 0; l <= lmaxmat; l++)
+      for (int io = 0; io < c->apword[l + (c->lmaxapw + 1) * is]; io++) fl[is * kNfMax + l * p + io] = l;
+    int na = 0;
+    for (int l = 0; l <= lmaxmat; l++)
+      for (int m = -l; m <= l; m++)
+        for (int io = 0; io < c->apword[l + (c->lmaxapw + 1) * is]; io++)
+          arow[(size_t)is * c->maxaa + na++] = make_int2(l * (l + 1) + m, l * p + io);
+    acount[is] = na;
+    int nl = 0;
+    for (int ilo = 0; ilo < c->nlorb[is]; ilo++) {
+      const int l = c->lorbl[ilo + c->nlomax * is];
+      fl[is * kNfMax + nfa + ilo] = l;
+      for (int m = -l; m <= l; m++)
+        lrow[(size_t)is * std::max(c->maxnlo, 1) + nl++] = make_int2(l * (l + 1) + m, nfa + ilo);
+    }
+    lcount[is] = nl;
+  }
+  // Gaunt list: for lm1, then lm3 (both <= lmmaxmat), the lm2 <= lmmaxvr with a
+  // nonzero coefficient, in increasing lm2 (the order of init1.f90:592-609)
+  std::vector<int> gstart((size_t)lmmaxmat * lmmaxmat + 1), glm2;
+  std::vector<double2> gval;
+  for (int l1 = 0; l1 <= lmaxmat; l1++)
+    for (int m1 = -l1; m1 <= l1; m1++)
+      for (int l3 = 0; l3 <= lmaxmat; l3++)
+        for (int m3 = -l3; m3 <= l3; m3++) {
+          gstart[(size_t)(l1 * (l1 + 1) + m1) * lmmaxmat + l3 * (l3 + 1) + m3] = (int)glm2.size();
+          for (int l2 = 0; l2 <= lmaxvr; l2++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+              double re, im;
+              gaunt_real_middle(l1, l2, l3, m1, m2, m3, &re, &im);
+              if (std::sqrt(re * re + im * im) > 1e-20) {
+                glm2.push_back(l2 * (l2 + 1) + m2);
+                gval.push_back(make_double2(re, im));
+              }
+            }
+        }
+  gstart.back() = (int)glm2.size();
+  // spline weights per species
+  std::vector<double> w((size_t)c->nrmtmax * ns, 0.0);
+  for (int is = 0; is < ns; is++) {
+    if (c->nrmt_h[is] < 3) return fail(c, EXH_EINVAL, "hmlint: fewer than 3 radial points");
+    spline_integral_weights(c->nrmt_h[is], c->spr_h.data() + (size_t)c->nrmtmax * is,
+                            w.data() + (size_t)c->nrmtmax * is);
+  }
+  std::vector<int> sp_int(c->nrmt_h.begin(), c->nrmt_h.end());
+  auto put = [&](DevBuf &b, const void *src, size_t bytes) {
+    int rc = ensure(c, b, bytes);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "hmlint tables");
+  };
+  int rc = put(c->hml_fl, fl.data(), fl.size() * sizeof(int));
+  if (!rc) rc = put(c->hml_arow, arow.data(), arow.size() * sizeof(int2));
+  if (!rc) rc = put(c->hml_lrow, lrow.data(), lrow.size() * sizeof(int2));
+  std::vector<int> counts(acount);
+  counts.insert(counts.end(), lcount.begin(), lcount.end());
+  counts.insert(counts.end(), sp_int.begin(), sp_int.end());
+  if (!rc) rc = put(c->hml_count, counts.data(), counts.size() * sizeof(int));
+  if (!rc) rc = put(c->hml_gstart, gstart.data(), gstart.size() * sizeof(int));
+  if (!rc) rc = put(c->hml_glm2, glm2.data(), std::max<size_t>(glm2.size(), 1) * sizeof(int));
+  if (!rc) rc = put(c->hml_gval, gval.data(), std::max<size_t>(gval.size(), 1) * sizeof(double2));
+  if (!rc) rc = put(c->hml_w, w.data(), w.size() * sizeof(double));
+  std::vector<double> spr(c->spr_h);
+  if (!rc) rc = put(c->hml_spr, spr.data(), spr.size() * sizeof(double));
+  if (rc) return rc;
+  c->hml_lmaxvr = lmaxvr;
+  c->hml_radial_epoch = c->radial_epoch;
+  c->hml_basis_epoch = c->basis_epoch;
+  return 0;
+}
+
+template <typename T>
+int copy_in(exh_context *c, DevBuf &b, const T *src, size_t n) {
+  int rc = ensure(c, b, n * sizeof(T));
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "upload");
+}
+
+int copy_out(exh_context *c, void *dst, const DevBuf &b, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, b.p, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "download");
+}
+
+__global__ void add_tables_kernel(size_t n, const double *__restrict__ a,
+                                  const double *__restrict__ b, double *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a ? a[i] + (b ? b[i] : 0.0) : 0.0;
+}
+
+}  // namespace
+}  // namespace exh
+
+using namespace exh;
+
+extern "C" {
+
+int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *veffmt,
+                      const double *lofr, int valence_rel_none, exh_z *aa, exh_z *loa,
+                      exh_z *lolo, int flags) {
+  if (!c) return EXH_EINVAL;
+  if (int rc = wait_done(c)) return rc;
+  if (!c->have_basis || !c->have_radial)
+    return fail(c, EXH_ESTATE, "hmlint: basis and radial data not set");
+  if (lmaxvr < 0 || lmaxvr > 50) return fail(c, EXH_EINVAL, "hmlint: lmaxvr outside 0..50");
+  if (!veffmt || !aa || (c->nlomax > 0 && (!lofr || !loa || !lolo)))
+    return fail(c, EXH_EINVAL, "hmlint: null array");
+  if (flags & ~EXH_ACCUMULATE) return fail(c, EXH_EINVAL, "hmlint: unknown flags");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  if (int rc = prepare(c, lmaxvr)) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int p = c->apwordmax, nfa = (c->lmaxmat + 1) * p, nf = nfa + c->nlomax;
+  const int lmmaxvr = (lmaxvr + 1) * (lmaxvr + 1), nchunk = (lmmaxvr + kCH - 1) / kCH;
+  const int natm = c->natmtot, mnlo = std::max(c->maxnlo, 1);
+  int rc = ensure(c, c->hml_ipot, (size_t)natm * nf * nf * lmmaxvr * sizeof(double));
+  if (!rc) rc = ensure(c, c->hml_ikin, (size_t)natm * nf * nf * sizeof(double));
+  if (rc) return rc;
+  const double alpha = 1.0 / 137.03599911;
+  HmlArgs a{nf, nfa, p, c->lmaxapw, c->nrmtmax, std::max(c->nlomax, 1), lmmaxvr, nchunk,
+            valence_rel_none ? 0.0 : 0.5 * alpha * alpha};
+  const int *counts = (const int *)c->hml_count.p;
+  const int npair = nf * (nf + 1) / 2;
+  const double *lf = lofr ? lofr : (const double *)c->hml_w.p;  // never read when nlomax = 0
+  hipError_t e = timed(c, EXH_K_HMLRAD, s, [&] {
+    dim3 grid(natm, nchunk + 1);
+#define LAUNCH(P)                                                                               \
+  hipLaunchKernelGGL(hml_radial_kernel<P>, grid, dim3(kThreads), 0, s, a,                       \
+                     (const int *)c->atom_species_d.p, counts + 2 * c->nspecies,                \
+                     (const double *)c->hml_w.p, (const double *)c->hml_spr.p,                  \
+                     (const int *)c->hml_fl.p, (const double *)c->apwfr_d.p, lf, veffmt,        \
+                     (double *)c->hml_ipot.p, (double *)c->hml_ikin.p)
+    if (npair <= kThreads) LAUNCH(1);
+    else if (npair <= 2 * kThreads) LAUNCH(2);
+    else if (npair <= 4 * kThreads) LAUNCH(4);
+    else LAUNCH(8);
+#undef LAUNCH
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return hip_fail(c, e, "hmlint radial");
+  const int lmmaxmat = (c->lmaxmat + 1) * (c->lmaxmat + 1);
+  RowTable A{(const int2 *)c->hml_arow.p, counts, c->maxaa};
+  RowTable L{(const int2 *)c->hml_lrow.p, counts + c->nspecies, mnlo};
+  struct Part {
+    RowTable r, col;
+    int ldo, nco;
+    exh_z *out;
+  } parts[3] = {{A, A, c->maxaa, c->maxaa, aa}, {L, A, mnlo, c->maxaa, loa}, {L, L, mnlo, mnlo, lolo}};
+  const int nparts = c->nlomax > 0 ? 3 : 1;
+  e = timed(c, EXH_K_HMLCON, s, [&] {
+    hipError_t le = hipSuccess;
+    for (int k = 0; k < nparts && le == hipSuccess; k++) {
+      const Part &P = parts[k];
+      dim3 grid(cdiv_ll((long long)P.ldo * P.nco, kThreads), natm);
+      hipLaunchKernelGGL(hml_contract_kernel, grid, dim3(kThreads), 0, s, nf, lmmaxvr, lmmaxmat,
+                         (const int *)c->atom_species_d.p, P.r, P.col, P.ldo, P.nco,
+                         (const int *)c->hml_gstart.p, (const int *)c->hml_glm2.p,
+                         (const double2 *)c->hml_gval.p, (const double *)c->hml_ipot.p,
+                         (const double *)c->hml_ikin.p, (double2 *)P.out,
+                         (flags & EXH_ACCUMULATE) ? 1 : 0);
+      le = hipGetLastError();
+    }
+    return le;
+  });
+  if (e == hipSuccess) e = mark_done(c, s);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "hmlint contract");
+}
+
+int exh_hmlint(exh_context *c, int lmaxvr, const double *veffmt, const double *lofr,
+               int valence_rel_none, exh_z *aa, exh_z *loa, exh_z *lolo, int flags) {
+  if (!c) return EXH_EINVAL;
+  if (!c->have_basis || !c->have_radial)
+    return fail(c, EXH_ESTATE, "hmlint: basis and radial data not set");
+  if (lmaxvr < 0 || lmaxvr > 50) return fail(c, EXH_EINVAL, "hmlint: lmaxvr outside 0..50");
+  if (!veffmt || !aa || (c->nlomax > 0 && (!lofr || !loa || !lolo)))
+    return fail(c, EXH_EINVAL, "hmlint: null array");
+  if (int rc = wait_done(c)) return rc;
+  const size_t na = c->natmtot, mnlo = std::max(c->maxnlo, 1);
+  const size_t lmm = (size_t)(lmaxvr + 1) * (lmaxvr + 1);
+  const size_t naa = (size_t)c->maxaa * c->maxaa * na, nloa = mnlo * c->maxaa * na,
+               nlolo = mnlo * mnlo * na;
+  int rc = copy_in(c, c->hml_vin, veffmt, lmm * c->nrmtmax * na);
+  if (!rc && c->nlomax > 0)
+    rc = copy_in(c, c->hml_loin, lofr, (size_t)c->nrmtmax * 2 * c->nlomax * na);
+  const bool acc = flags & EXH_ACCUMULATE;
+  if (!rc) rc = acc ? copy_in(c, c->hml_aa, (const double2 *)aa, naa)
+                    : ensure(c, c->hml_aa, naa * sizeof(double2));
+  if (!rc && c->nlomax > 0) {
+    rc = acc ? copy_in(c, c->hml_loa, (const double2 *)loa, nloa)
+             : ensure(c, c->hml_loa, nloa * sizeof(double2));
+    if (!rc)
+      rc = acc ? copy_in(c, c->hml_lolo, (const double2 *)lolo, nlolo)
+               : ensure(c, c->hml_lolo, nlolo * sizeof(double2));
+  }
+  if (rc) return rc;
+  rc = exh_hmlint_device(c, nullptr, lmaxvr, (const double *)c->hml_vin.p,
+                         c->nlomax > 0 ? (const double *)c->hml_loin.p : nullptr,
+                         valence_rel_none, (exh_z *)c->hml_aa.p,
+                         c->nlomax > 0 ? (exh_z *)c->hml_loa.p : nullptr,
+                         c->nlomax > 0 ? (exh_z *)c->hml_lolo.p : nullptr, flags);
+  if (!rc) rc = copy_out(c, aa, c->hml_aa, naa * sizeof(double2));
+  if (!rc && c->nlomax > 0) rc = copy_out(c, loa, c->hml_loa, nloa * sizeof(double2));
+  if (!rc && c->nlomax > 0) rc = copy_out(c, lolo, c->hml_lolo, nlolo * sizeof(double2));
+  return rc;
+}
+
+int exh_set_mt_device(exh_context *c, void *stream, int maxaa, int maxnlo, const exh_z *aa,
+                      const exh_z *loa, const exh_z *lolo, const double *oalo,
+                      const double *ololo, const double *h1aa, const double *h1loa,
+                      const double *h1lolo) {
+  if (!c) return EXH_EINVAL;
+  if (int rc = wait_done(c)) return rc;
+  if (!c->have_basis) return fail(c, EXH_ESTATE, "set_mt: call exh_set_basis first");
+  if (maxaa != c->maxaa || maxnlo != c->maxnlo)
+    return fail(c, EXH_EINVAL, "set_mt: maxaa/maxnlo differ from the basis");
+  if (!aa || (c->nlotot > 0 && (!loa || !lolo || !oalo || !ololo)))
+    return fail(c, EXH_EINVAL, "set_mt: null table");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const size_t na = c->natmtot;
+  const size_t mnlo = std::max(maxnlo, 1), nlom = std::max(c->nlomax, 1);
+  const size_t naa = (size_t)maxaa * maxaa * na, nloa = mnlo * maxaa * na, nlolo = mnlo * mnlo * na;
+  const size_t nsalo = (size_t)c->apwordmax * nlom * na, nolo = nlom * nlom * na;
+  const size_t nh1 = (size_t)c->apwordmax * c->apwordmax * (c->lmaxapw + 1) * na;
+  int rc = ensure(c, c->aa, naa * sizeof(double2));
+  if (!rc) rc = ensure(c, c->loa, loa ? nloa * sizeof(double2) : 16);
+  if (!rc) rc = ensure(c, c->lolo, lolo ? nlolo * sizeof(double2) : 16);
+  if (!rc) rc = ensure(c, c->salo, nsalo * sizeof(double));
+  if (!rc) rc = ensure(c, c->ololo_eff, nolo * sizeof(double));
+  if (!rc && h1aa) rc = ensure(c, c->h1aa, nh1 * sizeof(double));
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(c->aa.p, aa, naa * sizeof(double2), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && loa)
+    e = hipMemcpyAsync(c->loa.p, loa, nloa * sizeof(double2), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && lolo)
+    e = hipMemcpyAsync(c->lolo.p, lolo, nlolo * sizeof(double2), hipMemcpyDeviceToDevice, s);
+  // salo = oalo + h1loa, ololo_eff = ololo + h1lolo (zero without LOs)
+  const bool lo = c->nlomax > 0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(add_tables_kernel, dim3(cdiv_ll(nsalo, 256)), dim3(256), 0, s, nsalo,
+                       lo ? oalo : nullptr, h1loa, (double *)c->salo.p);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(add_tables_kernel, dim3(cdiv_ll(nolo, 256)), dim3(256), 0, s, nolo,
+                       lo ? ololo : nullptr, h1lolo, (double *)c->ololo_eff.p);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && h1aa)
+    e = hipMemcpyAsync(c->h1aa.p, h1aa, nh1 * sizeof(double), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(c, e, "set_mt_device");
+  c->have_h1aa = h1aa != nullptr;
+  c->have_mt = true;
+  return 0;
+}
+
+}  // extern "C"

@@ -89,6 +89,9 @@ def lib():
     L.exh_genveffig.argtypes = [_p, _p, _p, _p]
     L.exh_genmeffig.argtypes = [_p, _p, _p, _i, _i, _p, _p]
     L.exh_genveffig_device.argtypes = [_p, _p, _p, _p, _i, _p, _p, _p]
+    L.exh_set_mt_device.argtypes = [_p, _p, _i, _i] + [_p] * 8
+    L.exh_hmlint.argtypes = [_p, _i, _p, _p, _i, _p, _p, _p, _i]
+    L.exh_hmlint_device.argtypes = [_p, _p, _i, _p, _p, _i, _p, _p, _p, _i]
     _lib = L
     return L
 
@@ -108,12 +111,13 @@ EXPORTS = (
     "exh_batch_plan_count", "exh_batch_plan_info", "exh_set_batch_plan",
     "exh_get_batch_plan", "exh_tune_batch", "exh_set_batch_workspace",
     "exh_set_interstitial_device", "exh_gencfun", "exh_gencfun_device", "exh_genveffig",
-    "exh_genmeffig", "exh_genveffig_device",
+    "exh_genmeffig", "exh_genveffig_device", "exh_set_mt_device", "exh_hmlint",
+    "exh_hmlint_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
            "oz_gemm", "oz_combine", "ho_grouped", "batch_graph", "cfunig", "ifft",
-           "iproducts", "igather")
+           "iproducts", "igather", "hml_radial", "hml_contract")
 
 MATCH = 2   # batch: run match first
 GRAPH = 4   # batch: capture in a HIP graph, replay on identical calls
@@ -209,6 +213,7 @@ class Context:
         self.nlotot = 0
         self.ngkmax = 0
         self.natoms: list = []
+        self.maxaa = self.maxnlo = 0
         self.ngrtot = self.ngvec = 0
 
     def close(self):
@@ -279,6 +284,13 @@ class Context:
                                         _ptr(lorbl), ngkmax), "exh_set_basis")
         self.ngkmax = ngkmax
         self.natoms = [int(x) for x in natoms]
+        # maxaa (MaxAPWs) and maxnlo (largest losize) as the library derives them
+        ap = np.asarray(apword).reshape((lmaxapw + 1, len(self.natoms)), order="F")
+        lb = np.asarray(lorbl).reshape((max(nlomax, 1), len(self.natoms)), order="F")
+        self.maxaa = max(sum((2 * l + 1) * int(ap[l, i]) for l in range(lmaxmat + 1))
+                         for i in range(len(self.natoms)))
+        self.maxnlo = max([sum(2 * int(lb[j, i]) + 1 for j in range(int(nlorb[i])))
+                           for i in range(len(self.natoms))] + [0])
 
     def set_radial(self, omega, rmt, nrmt, nrmtmax, spr, apwfr):
         rmt, nrmt = _host(rmt, np.float64), _host(nrmt, np.int32)
@@ -287,7 +299,15 @@ class Context:
                                          _ptr(spr), _ptr(apwfr)), "exh_set_radial")
 
     def set_mt(self, maxaa, maxnlo, aa, loa, lolo, oalo, ololo, h1aa=None, h1loa=None,
-               h1lolo=None):
+               h1lolo=None, stream=None):
+        """Muffin-tin tables from host arrays, or from CUDA tensors through
+        exh_set_mt_device (no host copy)."""
+        if _is_torch(aa):
+            rc = self._device_call(stream, lambda st: lib().exh_set_mt_device(
+                self._h, st, maxaa, maxnlo, *[_ptr(a) for a in (aa, loa, lolo, oalo, ololo,
+                                                                  h1aa, h1loa, h1lolo)]))
+            self._check(rc, "exh_set_mt_device")
+            return
         arrs = [_host(aa, np.complex128), _host(loa, np.complex128),
                 _host(lolo, np.complex128), _host(oalo, np.float64), _host(ololo, np.float64),
                 _host(h1aa, np.float64), _host(h1loa, np.float64), _host(h1lolo, np.float64)]
@@ -314,6 +334,38 @@ class Context:
         self._check(lib().exh_set_interstitial(self._h, *[_ptr(x) for x in a],
                                                1 if valence_rel_none else 0,
                                                1 if h1on else 0), "exh_set_interstitial")
+
+    # ---- muffin-tin table producer --------------------------------------
+    def hmlint(self, lmaxvr, veffmt, lofr, valence_rel_none=True, aa=None, loa=None, lolo=None,
+               flags: int = 0, stream=None):
+        """hmlint.f90:53-55 (mt_kin + mt_pot): the muffin-tin tables
+        (aa, loa, lolo).  With CUDA tensors (veffmt, lofr and the three
+        outputs, Fortran layouts flattened) the device entry point fills the
+        outputs; otherwise NumPy arrays are returned (accumulated into the
+        given ones with flags=ACCUMULATE)."""
+        if _is_torch(veffmt):
+            rc = self._device_call(stream, lambda st: lib().exh_hmlint_device(
+                self._h, st, lmaxvr, _ptr(veffmt), _ptr(lofr), 1 if valence_rel_none else 0,
+                _ptr(aa), _ptr(loa), _ptr(lolo), flags))
+            self._check(rc, "exh_hmlint_device")
+            return aa, loa, lolo
+        v = _host(veffmt, np.float64)
+        lf = _host(lofr, np.float64) if lofr is not None else None
+        na = sum(self.natoms)
+        maxaa, mnlo = self.maxaa, max(self.maxnlo, 1)
+        outs = []
+        for x, shape in ((aa, (maxaa, maxaa, na)), (loa, (mnlo, maxaa, na)),
+                         (lolo, (mnlo, mnlo, na))):
+            if x is None:
+                x = np.zeros(shape, dtype=np.complex128, order="F")
+            elif not (isinstance(x, np.ndarray) and x.flags.f_contiguous
+                      and x.dtype == np.complex128):
+                raise ExhError("hmlint tables must be Fortran-ordered complex128 arrays")
+            outs.append(x)
+        self._check(lib().exh_hmlint(self._h, lmaxvr, _ptr(v), _ptr(lf),
+                                     1 if valence_rel_none else 0, *[_ptr(x) for x in outs],
+                                     flags), "exh_hmlint")
+        return tuple(outs)
 
     # ---- interstitial producers ----------------------------------------
     def _atposc(self, atposc):

@@ -131,6 +131,34 @@ int exh_set_interstitial_device(exh_context *ctx, void *stream, const exh_z *cfu
                                 const exh_z *veffig, const exh_z *meffig,
                                 const exh_z *m2effig, int valence_rel_none, int h1on);
 
+/* Same from device memory (the tables exh_hmlint_device makes, oalo/ololo,
+ * h1 terms), ordered after earlier work on `stream`; returns once installed.
+ * Tables of the same sizes are rewritten in place. */
+int exh_set_mt_device(exh_context *ctx, void *stream, int maxaa, int maxnlo,
+                      const exh_z *aa, const exh_z *loa, const exh_z *lolo,
+                      const double *oalo, const double *ololo, const double *h1aa,
+                      const double *h1loa, const double *h1lolo);
+
+/* ---- muffin-tin table producer ------------------------------------------ */
+/* hmlint.f90:53-55 for the spinless Hamiltonian: mt_kin.f90:14-237 then
+ * mt_pot.f90:14-323 into aa(maxaa,maxaa,natmtot), loa(maxnlo,maxaa,natmtot),
+ * lolo(maxnlo,maxnlo,natmtot) (the layouts of mt_hscf%main; maxnlo >= 1 as
+ * allocated).  veffmt(lmmaxvr,nrmtmax,natmtot) with lmmaxvr = (lmaxvr+1)^2
+ * (mod_potential_and_density), lofr(nrmtmax,2,nlomax,natmtot) (mod_APW_LO);
+ * apwfr, spr and nrmt come from exh_set_radial, the basis from
+ * exh_set_basis.  valence_rel_none selects the kinetic mass factor of
+ * mt_kin.f90:55-59.  flags: EXH_ACCUMULATE adds to the tables as the
+ * reference does (it zeroes them before hmlint), 0 overwrites.  Radial
+ * integrals use the spline rule of fderiv.f90; the Gaunt coefficients are
+ * those of init1.f90:563-609 (lm2 <= lmmaxvr, lm1 and lm3 <= lmmaxmat).  At
+ * most 63 radial functions per atom ((lmaxmat+1)*apwordmax + nlomax). */
+int exh_hmlint(exh_context *ctx, int lmaxvr, const double *veffmt, const double *lofr,
+               int valence_rel_none, exh_z *aa, exh_z *loa, exh_z *lolo, int flags);
+/* Same with every array in device memory, enqueued on `stream`. */
+int exh_hmlint_device(exh_context *ctx, void *stream, int lmaxvr, const double *veffmt,
+                      const double *lofr, int valence_rel_none, exh_z *aa, exh_z *loa,
+                      exh_z *lolo, int flags);
+
 /* ---- interstitial producers --------------------------------------------- */
 /* They need exh_set_gvectors with the whole FFT box as the G set (ngrtot =
  * ngrid(1)*ngrid(2)*ngrid(3), ngrid(d) = intgv(d,2) - intgv(d,1) + 1, as
@@ -282,7 +310,9 @@ enum {
   EXH_K_IFFT = 14,    /* interstitial producers: hipFFT transforms      */
   EXH_K_IPROD = 15,   /* genveffig/genmeffig products on the grid       */
   EXH_K_IGATHER = 16, /* grid -> G gathers (cfunir, veffig, meffig...)  */
-  EXH_K_COUNT = 17
+  EXH_K_HMLRAD = 17,  /* hmlint: radial integrals (potential, kinetic)  */
+  EXH_K_HMLCON = 18,  /* hmlint: Gaunt contraction into aa/loa/lolo     */
+  EXH_K_COUNT = 19
 };
 int exh_profile_enable(exh_context *ctx, int on);
 int exh_profile_read(exh_context *ctx, int kernel, double *total_ms, int *launches);

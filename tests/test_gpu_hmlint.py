@@ -1,0 +1,140 @@
+"""The muffin-tin table producer on the GPU (exh_hmlint*, exh_set_mt_device)
+against the oracle (oracle.hmlint, pinned by test_oracle_hmlint.py).
+
+Tolerance (FP64): relative error of the largest element <= 1e-12 for aa,
+loa and lolo.  The device integrates with precomputed spline weights and
+its own Gaunt coefficients, so it is not bitwise the oracle.  End to end,
+H and O set up from device-made, device-installed tables equal (bitwise) a
+setup from the same tables uploaded from the host, and match the oracle's
+setup on the oracle's tables to 1e-12.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from exciting_amd import lapw, synth
+from tests import _cases
+
+TOL = 1e-12
+
+
+def with_potential(name, lmaxvr):
+    s = _cases.system(name)
+    s = dataclasses.replace(s)      # own copy: the cached system stays as it is
+    return synth.add_mt_potential(s, lmaxvr=lmaxvr, seed=5)
+
+
+def err(got, want):
+    return np.abs(got - want).max() / max(np.abs(want).max(), 1e-300)
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).cuda()
+
+
+def host(t, like):
+    return t.cpu().numpy().reshape(like.shape, order="F")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,lmaxvr,relnone", [("tiny", 4, True), ("tiny_zora", 6, False),
+                                                 ("tiny_lmaxmat", 3, True), ("tiny_nolo", 4, True),
+                                                 ("fcc2", 8, True), ("sc8_k0", 8, True)])
+def test_hmlint_matches_the_oracle(name, lmaxvr, relnone):
+    import torch
+    s = with_potential(name, lmaxvr)
+    s.valence_rel_none = relnone
+    print(f"[gpu-hmlint] case {name} lmaxvr {lmaxvr} natmtot {s.natmtot}", flush=True)
+    want = oracle.hmlint(oracle.System(s))
+    ctx = lapw.Context(0).load(s)
+    try:
+        got = ctx.hmlint(lmaxvr, s.veffmt, s.lofr, relnone)
+        for g, w, what in zip(got, want, ("aa", "loa", "lolo")):
+            e = err(g, w)
+            print(f"  {what}: {e:.1e}")
+            assert e <= TOL, what
+        # device entry point: bitwise the host one
+        outs = [torch.zeros(w.size, dtype=torch.complex128, device="cuda") for w in want]
+        ctx.hmlint(lmaxvr, dev(s.veffmt), dev(s.lofr), relnone, *outs)
+        torch.cuda.synchronize()
+        for t, g in zip(outs, got):
+            assert np.array_equal(host(t, g), g)
+        # accumulate: adds to what is there
+        rng = np.random.default_rng(1)
+        pre = [np.asfortranarray(rng.normal(size=w.shape) + 1j * rng.normal(size=w.shape))
+               for w in want]
+        acc = ctx.hmlint(lmaxvr, s.veffmt, s.lofr, relnone, *[p.copy(order="F") for p in pre],
+                         flags=lapw.ACCUMULATE)
+        for a_, p, g in zip(acc, pre, got):
+            assert np.abs(a_ - (p + g)).max() <= 1e-14 * max(1.0, np.abs(g).max())
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tiny", "sc8_k0"])
+def test_setup_from_device_made_tables(name):
+    import torch
+    s = with_potential(name, 6)
+    ctx = lapw.Context(0).load(s)
+    try:
+        mnlo = max(s.maxnlo, 1)
+        outs = [torch.zeros(n, dtype=torch.complex128, device="cuda")
+                for n in (s.maxaa * s.maxaa * s.natmtot, mnlo * s.maxaa * s.natmtot,
+                          mnlo * mnlo * s.natmtot)]
+        ctx.hmlint(s.lmaxvr, dev(s.veffmt), dev(s.lofr), True, *outs)
+        ctx.set_mt(s.maxaa, s.maxnlo, *outs, dev(s.oalo), dev(s.ololo))
+        d = lapw.DeviceKPoint(s, 0)
+        H = torch.zeros(d.rank * d.rank, dtype=torch.complex128, device="cuda")
+        O = torch.zeros_like(H)
+
+        def setup():
+            H.zero_()
+            O.zero_()
+            ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, d.apwalm)
+            ctx.hamiltonandoverlapsetup(lapw.EvSystem(lapw.HermitianMatrix(d.rank, H),
+                                                      lapw.HermitianMatrix(d.rank, O)),
+                                        d.ngk, d.apwalm, d.igpig, d.vgpc, flags=0)
+            torch.cuda.synchronize()
+            return H.cpu().numpy(), O.cpu().numpy()
+
+        Hd, Od = setup()
+        tabs = [t.cpu().numpy() for t in outs]
+        aa = tabs[0].reshape((s.maxaa, s.maxaa, s.natmtot), order="F")
+        loa = tabs[1].reshape((mnlo, s.maxaa, s.natmtot), order="F")
+        lolo = tabs[2].reshape((mnlo, mnlo, s.natmtot), order="F")
+        ctx.set_mt(s.maxaa, s.maxnlo, aa, loa, lolo, s.oalo, s.ololo)
+        Hh, Oh = setup()
+        assert np.array_equal(Hd, Hh) and np.array_equal(Od, Oh)
+        # oracle: its own tables, then its own setup
+        a0, l0, ll0 = oracle.hmlint(oracle.System(s))
+        S = oracle.System(dataclasses.replace(s, aa=a0, loa=l0, lolo=ll0))
+        A = oracle.match(S, 0)
+        H0, O0 = oracle.setup(S, 0, A)
+        n = d.rank
+        eh = np.linalg.norm(Hd.reshape(n, n).T - H0) / np.linalg.norm(H0)
+        eo = np.linalg.norm(Od.reshape(n, n).T - O0) / np.linalg.norm(O0)
+        print(f"  {name}: H {eh:.1e} O {eo:.1e}")
+        assert eh <= TOL and eo <= TOL
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_hmlint_rejects_bad_arguments():
+    s = with_potential("tiny", 4)
+    ctx = lapw.Context(0).load(s)
+    try:
+        with pytest.raises(lapw.ExhError, match="lmaxvr"):
+            ctx.hmlint(51, s.veffmt, s.lofr)
+        with pytest.raises(lapw.ExhError, match="null"):
+            ctx.hmlint(4, s.veffmt, None)
+        with pytest.raises(lapw.ExhError, match="maxaa"):
+            import torch
+            z = torch.zeros(4, dtype=torch.complex128, device="cuda")
+            ctx.set_mt(s.maxaa + 1, s.maxnlo, z, z, z, z, z)
+    finally:
+        ctx.close()
