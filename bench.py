@@ -403,6 +403,78 @@ def interstitial_sample(ctx, s, reps=20):
     return res
 
 
+def hmlint_sample(ctx, s, lmaxvr=8, reps=10, cpu=True):
+    """The muffin-tin table producer on the workload's cell, inputs resident
+    in HBM: hmlint (radial integrals + Gaunt contraction into aa, loa, lolo)
+    and exh_set_mt_device installing them.  Synthetic smooth potential with
+    lmaxvr = 8 (the reference's default) and cubic LO radial functions."""
+    import dataclasses
+    import torch
+    from exciting_amd import synth
+    sp = synth.add_mt_potential(dataclasses.replace(s), lmaxvr=lmaxvr, seed=5)
+    dev = lambda a: torch.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).cuda()
+    v, lf = dev(sp.veffmt), dev(sp.lofr)
+    mnlo = max(s.maxnlo, 1)
+    outs = [torch.zeros(n, dtype=torch.complex128, device="cuda")
+            for n in (s.maxaa * s.maxaa * s.natmtot, mnlo * s.maxaa * s.natmtot,
+                      mnlo * mnlo * s.natmtot)]
+    oalo, ololo = dev(s.oalo), dev(s.ololo)
+
+    def run():
+        ctx.hmlint(lmaxvr, v, lf, True, *outs)
+
+    def run_install():
+        run()
+        ctx.set_mt(s.maxaa, s.maxnlo, *outs, oalo, ololo)
+
+    res = {}
+    for label, fn in (("hmlint", run), ("hmlint_and_set_mt_device", run_install)):
+        fn()
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps * 1e3
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        res[label] = {"ms": wall, "kernels_ms": {k: ms / n for k, (ms, n) in prof.items() if n}}
+    # restore the workload's own tables
+    ctx.set_mt(s.maxaa, s.maxnlo, s.aa, s.loa, s.lolo, s.oalo, s.ololo, s.h1aa, s.h1loa,
+               s.h1lolo)
+    p = s.apwordmax
+    nf = (s.lmaxmat + 1) * p + s.nlomax
+    npair = nf * (nf + 1) // 2
+    lmm = (lmaxvr + 1) ** 2
+    nr = int(s.nrmt.max())
+    flops = 2.0 * s.natmtot * npair * nr * lmm
+    t_rad = res["hmlint"]["kernels_ms"]["hml_radial"] * 1e-3
+    nel = s.maxaa * s.maxaa + mnlo * s.maxaa + mnlo * mnlo
+    t_con = res["hmlint"]["kernels_ms"]["hml_contract"] * 1e-3
+    res["radial_kernel"] = {"bound": "fp64 vector (LDS-fed)", "flops": flops,
+                            "TFLOPs": flops / t_rad / 1e12,
+                            "frac_fp64_peak": flops / t_rad / 78.6e12,
+                            "per_atom": f"{npair} radial pairs x {lmm} lm x {nr} points"}
+    res["contract_kernel"] = {"bound": "hbm (table writes)", "bytes": 16.0 * nel * s.natmtot,
+                              "GBps": 16.0 * nel * s.natmtot / t_con / 1e9,
+                              "frac_hbm": 16.0 * nel * s.natmtot / t_con / 8e12}
+    res["tables_MB"] = 16.0 * nel * s.natmtot / 1e6
+    res["workload"] = (f"{s.natmtot} atoms, lmaxmat {s.lmaxmat}, APW order {p}, "
+                       f"{s.nlomax} LOs per atom, lmaxvr {lmaxvr}, {nr} radial points")
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.hmlint(oracle.System(sp))
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": 1e3 * t, "unit": "ms per hmlint of the cell",
+                               "cores": 1, "kind": "port",
+                               "sample": (f"oracle hmlint (C restatement of mt_kin + mt_pot, "
+                                          f"one thread) on all {s.natmtot} atoms")}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,6 +514,9 @@ def main():
     ap.add_argument("--interstitial", choices=("on", "off", "only"), default="on",
                     help="time the interstitial producers on the workload's grid "
                          "('only': nothing else, one JSON line with that sub-object)")
+    ap.add_argument("--hmlint", choices=("on", "off", "only"), default="on",
+                    help="time the muffin-tin table producer on the workload's cell "
+                         "('only': nothing else)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -472,6 +547,10 @@ def main():
 
     if args.interstitial == "only":
         print(json.dumps({"interstitial": interstitial_sample(W.ctx, s)}), flush=True)
+        return
+    if args.hmlint == "only":
+        print(json.dumps({"hmlint": hmlint_sample(W.ctx, s, cpu=args.cpu_baseline == "on")}),
+              flush=True)
         return
 
     if args.plan >= 0:
@@ -553,6 +632,9 @@ def main():
     if args.interstitial == "on":
         out["interstitial"] = interstitial_sample(W.ctx, s)
         log(f"interstitial: {out['interstitial']}")
+    if args.hmlint == "on":
+        out["hmlint"] = hmlint_sample(W.ctx, s, cpu=args.cpu_baseline == "on")
+        log(f"hmlint: {out['hmlint']}")
     W.free()
 
     if args.uneven == "on":

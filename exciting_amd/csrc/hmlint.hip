@@ -31,7 +31,8 @@ namespace {
 constexpr int kRC = 32;        // grid points per LDS chunk
 constexpr int kCH = 8;         // potential channels (lm) per workgroup
 constexpr int kNfMax = 63;     // radial functions per atom (APW + LO)
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;    // contraction kernel
+constexpr int kRadThreads = 384; // radial kernel: one pair per thread up to 27 functions
 
 struct HmlArgs {
   int nf, nfa, p, lmaxapw, nrmtmax, nlomax, lmmaxvr, nchunk;
@@ -62,7 +63,7 @@ __device__ __forceinline__ void pair_of(int t, int &f1, int &f2) {
 // y == nchunk, the kinetic integrals.  ipot[ias][f1][f2][lm], ikin[ias][f1][f2],
 // both written for (f1,f2) and (f2,f1).
 template <int PPT>
-__global__ __launch_bounds__(kThreads) void hml_radial_kernel(
+__global__ __launch_bounds__(kRadThreads) void hml_radial_kernel(
     HmlArgs a, const int *__restrict__ atom_species, const int *__restrict__ nrmt,
     const double *__restrict__ wts, const double *__restrict__ spr,
     const int *__restrict__ fl, const double *__restrict__ apwfr,
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(kThreads) void hml_radial_kernel(
   double lfac[PPT];
 #pragma unroll
   for (int q = 0; q < PPT; q++) {
-    const int t = tid + q * kThreads;
+    const int t = tid + q * kRadThreads;
     pf1[q] = pf2[q] = 0;
     lfac[q] = 0.0;
     if (t < npair) {
@@ -96,13 +97,13 @@ __global__ __launch_bounds__(kThreads) void hml_radial_kernel(
     }
   }
   for (int r0 = 0; r0 < nr; r0 += kRC) {
-    for (int i = tid; i < kRC * nf; i += kThreads) {
+    for (int i = tid; i < kRC * nf; i += kRadThreads) {
       const int r = i / nf, f = i - r * nf, ir = r0 + r;
       sF[r][f] = ir < nr ? radial_value(a, apwfr, lofr, ias, f, ir, 0) : 0.0;
       if (kin) sD[r][f] = ir < nr ? radial_value(a, apwfr, lofr, ias, f, ir, 1) : 0.0;
     }
     if (kin) {
-      for (int r = tid; r < kRC; r += kThreads) {
+      for (int r = tid; r < kRC; r += kRadThreads) {
         const int ir = r0 + r;
         double w = 0.0, r2 = 0.0;
         if (ir < nr) {
@@ -115,13 +116,13 @@ __global__ __launch_bounds__(kThreads) void hml_radial_kernel(
         sR2[r] = r2;
       }
     } else {
-      for (int i = tid; i < kRC * kCH; i += kThreads) {
+      for (int i = tid; i < kRC * kCH; i += kRadThreads) {
         const int r = i / kCH, k = i - r * kCH, ir = r0 + r, lm = lm0 + k;
         sV[r][k] = (ir < nr && lm < a.lmmaxvr)
                        ? veffmt[lm + (size_t)a.lmmaxvr * (ir + (size_t)a.nrmtmax * ias)]
                        : 0.0;
       }
-      for (int r = tid; r < kRC; r += kThreads) {
+      for (int r = tid; r < kRC; r += kRadThreads) {
         const int ir = r0 + r;
         double w = 0.0;
         if (ir < nr) {
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void hml_radial_kernel(
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
-      if (tid + q * kThreads >= npair) continue;
+      if (tid + q * kRadThreads >= npair) continue;
       const int f1 = pf1[q], f2 = pf2[q];
       if (kin) {
         double s = acc[q][0];
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void hml_radial_kernel(
   }
 #pragma unroll
   for (int q = 0; q < PPT; q++) {
-    if (tid + q * kThreads >= npair) continue;
+    if (tid + q * kRadThreads >= npair) continue;
     const int f1 = pf1[q], f2 = pf2[q];
     if (kin) {
       const bool same = fl[is * kNfMax + f1] == fl[is * kNfMax + f2];
@@ -476,14 +477,14 @@ int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *ve
   hipError_t e = timed(c, EXH_K_HMLRAD, s, [&] {
     dim3 grid(natm, nchunk + 1);
 #define LAUNCH(P)                                                                               \
-  hipLaunchKernelGGL(hml_radial_kernel<P>, grid, dim3(kThreads), 0, s, a,                       \
+  hipLaunchKernelGGL(hml_radial_kernel<P>, grid, dim3(kRadThreads), 0, s, a,                       \
                      (const int *)c->atom_species_d.p, counts + 2 * c->nspecies,                \
                      (const double *)c->hml_w.p, (const double *)c->hml_spr.p,                  \
                      (const int *)c->hml_fl.p, (const double *)c->apwfr_d.p, lf, veffmt,        \
                      (double *)c->hml_ipot.p, (double *)c->hml_ikin.p)
-    if (npair <= kThreads) LAUNCH(1);
-    else if (npair <= 2 * kThreads) LAUNCH(2);
-    else if (npair <= 4 * kThreads) LAUNCH(4);
+    if (npair <= kRadThreads) LAUNCH(1);
+    else if (npair <= 2 * kRadThreads) LAUNCH(2);
+    else if (npair <= 4 * kRadThreads) LAUNCH(4);
     else LAUNCH(8);
 #undef LAUNCH
     return hipGetLastError();

@@ -4,12 +4,13 @@
 ! (seceqnfv.f90:73-77, H and O zeroed by newsystem).
 !
 ! Then the interstitial producers (gencfun, genveffig, genmeffig for iora*)
-! from the cell's bvec, atposc and a real-space veffir.
+! from the cell's bvec, atposc and a real-space veffir, and the muffin-tin
+! table producer (hmlint) from veffmt and lofr.
 !
 ! Usage: exh_driver <input.bin> <output.bin>
 ! The input is a stream of int32/float64/complex128 records written by
 ! tests/test_gpu_fortran.py; the output holds apwalm, H and O, then cfunig,
-! cfunir, veffig, meffig and m2effig.
+! cfunir, veffig, meffig and m2effig, then the hmlint tables aa, loa, lolo.
 program exh_driver
   use, intrinsic :: iso_c_binding
   use exh_lapw
@@ -32,6 +33,9 @@ program exh_driver
   real(c_double) :: bvec(3, 3)
   real(c_double), allocatable :: atposc(:,:,:), veffir(:), cfunir(:)
   complex(c_double_complex), allocatable :: cfunig_p(:), veffig_p(:), meffig(:), m2effig(:)
+  integer(c_int) :: lmaxvr
+  real(c_double), allocatable :: veffmt(:,:,:), lofr(:,:,:,:)
+  complex(c_double_complex), allocatable :: aa2(:,:,:), loa2(:,:,:), lolo2(:,:,:)
   type(c_ptr) :: ctx
 
   call get_command_argument(1, fin)
@@ -53,6 +57,9 @@ program exh_driver
   read (u) maxatoms
   allocate (atposc(3, maxatoms, nspecies), veffir(ngrtot))
   read (u) bvec, atposc, veffir
+  read (u) lmaxvr
+  allocate (veffmt((lmaxvr + 1)**2, nrmtmax, natmtot), lofr(nrmtmax, 2, max(nlomax, 1), natmtot))
+  read (u) veffmt, lofr
   close (u)
 
   rc = exh_create(0_c_int, ctx)
@@ -92,10 +99,18 @@ program exh_driver
   call check(rc, 'exh_genveffig')
   rc = exh_genmeffig(ctx, veffir, cfunir, 0_c_int, 1_c_int, meffig, m2effig)
   call check(rc, 'exh_genmeffig')
+
+  ! hmlint.f90:53-55 into zeroed tables (MTInitAll zeroes mt_hscf before it)
+  allocate (aa2(maxaa, maxaa, natmtot), loa2(maxnlo, maxaa, natmtot), lolo2(maxnlo, maxnlo, natmtot))
+  aa2 = (0.d0, 0.d0)
+  loa2 = (0.d0, 0.d0)
+  lolo2 = (0.d0, 0.d0)
+  rc = exh_hmlint(ctx, lmaxvr, veffmt, lofr, 1_c_int, aa2, loa2, lolo2, EXH_ACCUMULATE)
+  call check(rc, 'exh_hmlint')
   call exh_destroy(ctx)
 
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u) apwalm, h, o, cfunig_p, cfunir, veffig_p, meffig, m2effig
+  write (u) apwalm, h, o, cfunig_p, cfunir, veffig_p, meffig, m2effig, aa2, loa2, lolo2
   close (u)
   print '(a,i0)', 'exh_driver: done, rank = ', rank
 

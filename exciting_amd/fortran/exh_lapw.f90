@@ -17,6 +17,8 @@
 !   exh_gencfun                  replaces  gencfun (src/gencfun.f90:13)
 !   exh_genveffig                replaces  genveffig (src/genveffig.f90:13)
 !   exh_genmeffig                replaces  genmeffig (src/genmeffig.f90:13)
+!   exh_hmlint                   replaces  hmlint's spinless part
+!                                          (src/src_eigensystem/hmlint.f90:53-55)
 module exh_lapw
   use, intrinsic :: iso_c_binding
   implicit none
@@ -42,7 +44,8 @@ module exh_lapw
             exh_error_message, exh_split, exh_setup_batch_device, exh_tune_batch, &
             exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace, &
             exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
-            exh_genmeffig, exh_genveffig_device
+            exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
+            exh_set_mt_device
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -152,6 +155,34 @@ module exh_lapw
       real(c_double), intent(in) :: veffir(*), cfunir(*)
       integer(c_int), value :: valence_rel_none, iora_star
       complex(c_double_complex), intent(out) :: meffig(*), m2effig(*)
+    end function
+
+    ! hmlint.f90:53-55 (mt_kin + mt_pot): veffmt(lmmaxvr,nrmtmax,natmtot),
+    ! lofr(nrmtmax,2,nlomax,natmtot) in; mt_hscf%main%aa/loa/lolo out (flags:
+    ! EXH_ACCUMULATE adds as the reference does, 0 overwrites).
+    integer(c_int) function exh_hmlint(ctx, lmaxvr, veffmt, lofr, valence_rel_none, aa, loa, &
+        lolo, flags) bind(C, name="exh_hmlint")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lmaxvr, valence_rel_none, flags
+      real(c_double), intent(in) :: veffmt(*), lofr(*)
+      complex(c_double_complex), intent(inout) :: aa(*), loa(*), lolo(*)
+    end function
+
+    integer(c_int) function exh_hmlint_device(ctx, stream, lmaxvr, veffmt, lofr, &
+        valence_rel_none, aa, loa, lolo, flags) bind(C, name="exh_hmlint_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, veffmt, lofr, aa, loa, lolo
+      integer(c_int), value :: lmaxvr, valence_rel_none, flags
+    end function
+
+    ! exh_set_mt from device memory.
+    integer(c_int) function exh_set_mt_device(ctx, stream, maxaa, maxnlo, aa, loa, lolo, oalo, &
+        ololo, h1aa, h1loa, h1lolo) bind(C, name="exh_set_mt_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: maxaa, maxnlo
+      type(c_ptr), value :: aa, loa, lolo, oalo, ololo, h1aa, h1loa, h1lolo
     end function
 
     ! Device form of both, outputs selected by `which` (EXH_VEFFIG, ...).

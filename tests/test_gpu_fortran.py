@@ -4,6 +4,7 @@ interstitial producers (gencfun, genveffig, genmeffig), through the binding
 module on the GPU; apwalm, H, O and the producers' outputs are checked
 against the oracle to the same 1e-12 relative Frobenius bar as the Python
 path."""
+import dataclasses
 import os
 import subprocess
 
@@ -12,6 +13,7 @@ import pytest
 
 import oracle
 from exciting_amd import build as exhbuild
+from exciting_amd import synth
 from tests import _cases
 
 
@@ -37,6 +39,7 @@ def write_input(path, s, ik=0):
             atposc[:, :na, is_] = s.atposc[ias:ias + na].T
             ias += na
         f.write(i32([maxatoms]) + f64(s.bvec) + f64(atposc) + f64(veffir_of(s)))
+        f.write(i32([s.lmaxvr]) + f64(s.veffmt) + f64(s.lofr))
 
 
 def veffir_of(s):
@@ -48,7 +51,7 @@ def veffir_of(s):
 def test_fortran_driver(tmp_path, name):
     if not os.path.exists(exhbuild.DRIVER):
         pytest.fail("Fortran driver not built (python -m exciting_amd.build)")
-    s = _cases.system(name)
+    s = synth.add_mt_potential(dataclasses.replace(_cases.system(name)), lmaxvr=5, seed=3)
     k = s.kpts[0]
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     write_input(fin, s)
@@ -66,8 +69,12 @@ def test_fortran_driver(tmp_path, name):
     O = raw[na + n * n:na + 2 * n * n].reshape((n, n), order="F")
     rest = np.fromfile(fout, dtype=np.uint8)[16 * (na + 2 * n * n):]
     ng, nv = s.ngrtot, s.ngvec
+    mnlo = max(s.maxnlo, 1)
+    nt = [s.maxaa * s.maxaa * s.natmtot, mnlo * s.maxaa * s.natmtot, mnlo * mnlo * s.natmtot]
     cuts = np.cumsum([0, 16 * ng, 8 * ng, 16 * nv, 16 * nv, 16 * nv])
-    assert rest.size == cuts[-1]
+    assert rest.size == cuts[-1] + 16 * sum(nt)
+    tabs = rest[cuts[-1]:].view(np.complex128)
+    tabs = np.split(tabs, np.cumsum(nt)[:-1])
     cg, cr, vg, mg, m2g = (rest[a:b].view(dt) for a, b, dt in
                            zip(cuts[:-1], cuts[1:], (np.complex128, np.float64, np.complex128,
                                                      np.complex128, np.complex128)))
@@ -84,3 +91,7 @@ def test_fortran_driver(tmp_path, name):
     errs = [rel(a, b) for a, b in ((cg, cg0), (cr, cr0), (vg, vg0), (mg, mg0), (m2g, m2g0))]
     print("  fortran producers: " + " ".join(f"{e:.1e}" for e in errs))
     assert max(errs) <= 1e-12
+    want = [t.ravel(order="F") for t in oracle.hmlint(oracle.System(s))]
+    terr = [np.abs(g - w).max() / max(np.abs(w).max(), 1e-300) for g, w in zip(tabs, want)]
+    print("  fortran hmlint: " + " ".join(f"{e:.1e}" for e in terr))
+    assert max(terr) <= 1e-12
