@@ -475,6 +475,57 @@ def hmlint_sample(ctx, s, lmaxvr=8, reps=10, cpu=True):
     return res
 
 
+def vnlmat_sample(W, nstfv=400, reps=3, cpu=True):
+    """calc_vnlmat on k-point 0 of the workload: match + overlapsetup alone
+    (O without the Hamiltonian's products) + the three products, inputs
+    resident in HBM; random orthonormal-free states and a Hermitian vxnl of
+    nstfv states."""
+    import torch
+    from exciting_amd import hybrids, lapw
+    d = W.dks[0]
+    n = d.rank
+    g = torch.Generator(device="cuda").manual_seed(3)
+    X = torch.randn(nstfv, n, dtype=torch.complex128, device="cuda", generator=g)
+    V = torch.randn(nstfv, nstfv, dtype=torch.complex128, device="cuda", generator=g)
+    V = V + V.conj().transpose(0, 1)
+    O = W.O[0]
+    hybrids.vnlmat_kpoint(W.ctx, d, X, V, O=O)            # warm-up
+    torch.cuda.synchronize()
+    W.ctx.profile(True)
+    W.ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hybrids.vnlmat_kpoint(W.ctx, d, X, V, O=O)
+    torch.cuda.synchronize()
+    total = (time.perf_counter() - t0) / reps * 1e3
+    prof = W.ctx.profile_read()
+    W.ctx.profile(False)
+    k = {name: ms / cnt for name, (ms, cnt) in prof.items() if cnt}
+    setup_ms = sum(k.get(x, 0.0) for x in ("match", "ivgk", "pack", "a2t", "ho", "lo_border",
+                                           "lolo"))
+    f_o = flops_ho(W.s.natmtot, W.s.maxaa, d.ngk) / 2      # O alone: half the H/O products
+    res = {"ms": total, "kernels_ms": k, "setup_ms": setup_ms,
+           "ho_o_only": {"ms": k.get("ho", 0.0), "TFLOPs": f_o / (k.get("ho", 1.0) * 1e-3) / 1e12,
+                         "frac_fp64_peak": f_o / (k.get("ho", 1.0) * 1e-3) / 78.6e12},
+           "gemm_ms": total - setup_ms, "n": n, "nstfv": nstfv,
+           "workload": f"k-point 0 of {W.spec.name}: n = {n}, nstfv = {nstfv}"}
+    if cpu:
+        import oracle
+        S = oracle.System(W.s)
+        t0 = time.perf_counter()
+        A = oracle.match(S, 0)
+        _, Oc = oracle.overlapsetup(S, 0, A)
+        t1 = time.perf_counter()
+        Xc, Vc = X.cpu().numpy().T.copy(), V.cpu().numpy()
+        oracle.calc_vnlmat(Oc, Xc, Vc)
+        t2 = time.perf_counter()
+        res["cpu_baseline"] = {"value": 1e3 * (t2 - t0), "unit": "ms per k-point", "cores": 16,
+                               "kind": "port",
+                               "sample": (f"oracle match + overlapsetup {t1 - t0:.1f} s and NumPy "
+                                          f"products {t2 - t1:.1f} s on k-point 0")}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -517,6 +568,9 @@ def main():
     ap.add_argument("--hmlint", choices=("on", "off", "only"), default="on",
                     help="time the muffin-tin table producer on the workload's cell "
                          "('only': nothing else)")
+    ap.add_argument("--vnlmat", choices=("on", "off", "only"), default="on",
+                    help="time calc_vnlmat (match + overlapsetup alone + 3 products) on "
+                         "k-point 0 ('only': nothing else)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -547,6 +601,10 @@ def main():
 
     if args.interstitial == "only":
         print(json.dumps({"interstitial": interstitial_sample(W.ctx, s)}), flush=True)
+        return
+    if args.vnlmat == "only":
+        print(json.dumps({"vnlmat": vnlmat_sample(W, cpu=args.cpu_baseline == "on")}),
+              flush=True)
         return
     if args.hmlint == "only":
         print(json.dumps({"hmlint": hmlint_sample(W.ctx, s, cpu=args.cpu_baseline == "on")}),
@@ -635,6 +693,9 @@ def main():
     if args.hmlint == "on":
         out["hmlint"] = hmlint_sample(W.ctx, s, cpu=args.cpu_baseline == "on")
         log(f"hmlint: {out['hmlint']}")
+    if args.vnlmat == "on":
+        out["vnlmat"] = vnlmat_sample(W, cpu=args.cpu_baseline == "on")
+        log(f"vnlmat: {out['vnlmat']}")
     W.free()
 
     if args.uneven == "on":

@@ -567,14 +567,18 @@ int exh_match_device(exh_context *c, void *stream, int ngp, const double *gpc,
   return e == hipSuccess ? 0 : hip_fail(c, e, "match kernel");
 }
 
-int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_z *apwalm,
-                     const int *igpig, const double *vgpc, exh_z *h, exh_z *o, int flags) {
+// The single-k setup.  o_only: overlapsetup.f90 alone -- O accumulated and
+// mirrored, H (may be null) only mirrored (hamiltonoverlapocopy_UL is called
+// from overlapsetup.f90:164-166 for both matrices).
+static int setup_impl(exh_context *c, void *stream, int ngp, int rank, const exh_z *apwalm,
+                      const int *igpig, const double *vgpc, exh_z *h, exh_z *o, int flags,
+                      bool o_only) {
   EXH_CHECK_CTX(c);
   if (!c->have_basis || !c->have_mt || !c->have_gvec || !c->have_ist)
     return fail(c, EXH_ESTATE, "setup: basis, muffin-tin tables and interstitial data must be set");
   if (ngp < 0 || ngp > c->ngkmax) return fail(c, EXH_EINVAL, "setup: ngp outside 0..ngkmax");
   if (rank != ngp + c->nlotot) return fail(c, EXH_EINVAL, "setup: rank != ngp + nlotot");
-  if (!h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
+  if ((!h && !o_only) || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
     return fail(c, EXH_EINVAL, "setup: null array");
   const int nmod = (flags >> 8) & 0xff;
   if (flags & ~(EXH_ACCUMULATE | 0xff00)) return fail(c, EXH_ENOTSUP, "setup: unknown flag");
@@ -582,10 +586,12 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
     return fail(c, EXH_EINVAL, "setup: EXH_SPLIT(n) needs 6 <= n <= 12");
   if (nmod && c->have_h1aa)
     return fail(c, EXH_ENOTSUP, "setup: split precision with h1aa (iora*) is not implemented");
+  if (nmod && o_only) return fail(c, EXH_ENOTSUP, "overlapsetup: split precision is not implemented");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const int ldb = (c->ngkmax + 63) / 64 * 64;
-  int rc = ensure(c, c->bt, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
+  exh_z *hw = o_only ? nullptr : h;  // H the kernels write
+  int rc = o_only ? 0 : ensure(c, c->bt, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
   if (!rc && c->have_h1aa) rc = ensure(c, c->a2t, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
   if (!rc && !c->contig) rc = ensure(c, c->at, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
   if (!rc) rc = ensure(c, c->ivgk, (size_t)std::max(c->ngkmax, 1) * sizeof(int4));
@@ -596,7 +602,7 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
   });
   if (e == hipSuccess && !c->contig)
     e = timed(c, EXH_K_PACK, s, [&] { return launch_pack(s, c, ngp, A); });
-  if (e == hipSuccess)
+  if (e == hipSuccess && !o_only)
     e = timed(c, EXH_K_BT, s, [&] { return launch_bt(s, c, ngp, A, (double2 *)c->bt.p, ldb); });
   if (e == hipSuccess && c->have_h1aa)
     e = timed(c, EXH_K_A2T, s, [&] { return launch_a2t(s, c, ngp, A, (double2 *)c->a2t.p, ldb); });
@@ -604,7 +610,7 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
     e = timed(c, EXH_K_HO, s, [&] {
       return launch_ho(s, c, ngp, A, (const double2 *)c->bt.p,
                        c->have_h1aa ? (const double2 *)c->a2t.p : nullptr, ldb,
-                       (const int4 *)c->ivgk.p, vgpc, (double2 *)h, (double2 *)o, rank, flags);
+                       (const int4 *)c->ivgk.p, vgpc, (double2 *)hw, (double2 *)o, rank, flags);
     });
   if (e == hipSuccess && nmod > 0) {
     OzLayout L;
@@ -627,12 +633,66 @@ int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_
   }
   if (e == hipSuccess)
     e = timed(c, EXH_K_LOBORDER, s, [&] {
-      return launch_lo_border(s, c, ngp, A, (double2 *)h, (double2 *)o, rank, flags);
+      return launch_lo_border(s, c, ngp, A, (double2 *)hw, (double2 *)o, rank, flags);
     });
   if (e == hipSuccess)
-    e = timed(c, EXH_K_LOLO, s, [&] { return launch_lolo(s, c, ngp, (double2 *)h, (double2 *)o, rank, flags); });
+    e = timed(c, EXH_K_LOLO, s, [&] { return launch_lolo(s, c, ngp, (double2 *)hw, (double2 *)o, rank, flags); });
+  if (e == hipSuccess && o_only && h) e = launch_mirror(s, rank, (double2 *)h, rank);
   if (e == hipSuccess) e = mark_done(c, s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "setup kernels");
+}
+
+int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_z *apwalm,
+                     const int *igpig, const double *vgpc, exh_z *h, exh_z *o, int flags) {
+  return setup_impl(c, stream, ngp, rank, apwalm, igpig, vgpc, h, o, flags, false);
+}
+
+int exh_overlapsetup_device(exh_context *c, void *stream, int ngp, int rank,
+                            const exh_z *apwalm, const int *igpig, const double *vgpc,
+                            exh_z *h, exh_z *o, int flags) {
+  return setup_impl(c, stream, ngp, rank, apwalm, igpig, vgpc, h, o, flags, true);
+}
+
+int exh_overlapsetup(exh_context *c, int ngp, int rank, const exh_z *apwalm, const int *igpig,
+                     const double *vgpc, exh_z *h, exh_z *o) {
+  EXH_CHECK_CTX(c);
+  if (!c->have_basis) return fail(c, EXH_ESTATE, "setup: basis not set");
+  if (ngp < 0 || ngp > c->ngkmax) return fail(c, EXH_EINVAL, "setup: ngp outside 0..ngkmax");
+  if (rank != ngp + c->nlotot) return fail(c, EXH_EINVAL, "setup: rank != ngp + nlotot");
+  if (!o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
+    return fail(c, EXH_EINVAL, "setup: null array");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  const size_t ld = c->ngkmax, ncol = (size_t)c->apwordmax * c->lmmax * c->natmtot;
+  const size_t nn = (size_t)rank * rank;
+  int rc = ensure(c, c->s_apwalm, ld * ncol * sizeof(double2));
+  if (!rc) rc = ensure(c, c->s_igpig, ld * sizeof(int));
+  if (!rc) rc = ensure(c, c->s_vgpc, 3 * ld * sizeof(double));
+  if (!rc && h) rc = ensure(c, c->s_h, std::max<size_t>(nn, 1) * sizeof(double2));
+  if (!rc) rc = ensure(c, c->s_o, std::max<size_t>(nn, 1) * sizeof(double2));
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  hipError_t e = hipSuccess;
+  if (ngp > 0) {
+    e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
+                         ngp * sizeof(double2), ncol, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->s_igpig.p, igpig, ngp * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->s_vgpc.p, vgpc, 3 * (size_t)ngp * sizeof(double), hipMemcpyHostToDevice, s);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(c->s_o.p, o, nn * sizeof(double2), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && h)
+    e = hipMemcpyAsync(c->s_h.p, h, nn * sizeof(double2), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(c, e, "overlapsetup: upload");
+  rc = setup_impl(c, s, ngp, rank, (const exh_z *)c->s_apwalm.p, (const int *)c->s_igpig.p,
+                  (const double *)c->s_vgpc.p, h ? (exh_z *)c->s_h.p : nullptr,
+                  (exh_z *)c->s_o.p, EXH_ACCUMULATE, true);
+  if (rc) return rc;
+  e = hipMemcpyAsync(o, c->s_o.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && h)
+    e = hipMemcpyAsync(h, c->s_h.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "overlapsetup: download");
 }
 
 int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,

@@ -104,4 +104,13 @@ __device__ __forceinline__ void store_upper(PH H, PH O, long long ldh, int i, in
   }
 }
 
+// O alone (overlapsetup without hamiltonsetup): the upper element and its
+// mirror.
+template <typename PH>
+__device__ __forceinline__ void store_upper_o(PH O, long long ldh, int i, int j, double orr,
+                                              double oi) {
+  st(O + (size_t)i + (size_t)j * ldh, z2(orr, oi));
+  if (i != j) st(O + (size_t)j + (size_t)i * ldh, z2(orr, -oi));
+}
+
 }  // namespace exh

@@ -213,6 +213,7 @@ hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
 hipError_t launch_lo_border(hipStream_t s, const exh_context *c, int ngp,
                             const double2 *apwalm, double2 *h, double2 *o,
                             int ldh, int flags);
+hipError_t launch_mirror(hipStream_t s, int n, double2 *m, int ld);
 hipError_t launch_lolo(hipStream_t s, const exh_context *c, int ngp,
                        double2 *h, double2 *o, int ldh, int flags);
 const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, long long *ld);

@@ -312,8 +312,9 @@ struct HoShape {
   static_assert(TK % RX == 0 && TK % RY == 0 && TK % 4 == 0, "K slab");
 };
 
-// One TMI x TMJ upper tile (I0, J0) of H and O for one k-point.
-template <int TMI, int TMJ, int TK, typename PX, typename PH, typename B>
+// One TMI x TMJ upper tile (I0, J0) of H and O for one k-point.  WH = false
+// computes O alone (overlapsetup): YB and H are not touched.
+template <int TMI, int TMJ, int TK, typename PX, typename PH, typename B, bool WH = true>
 __device__ __forceinline__ void ho_tile(
     int ngp, int Ktot, PX __restrict__ X, long long ldx, PX __restrict__ YB, long long ldb,
     PX __restrict__ YA, long long lda, const B &box, PH __restrict__ H,
@@ -363,7 +364,7 @@ __device__ __forceinline__ void ho_tile(
       const int k = kb + yk + S::RY * q;
       fy[q] = k < Ktot ? 1.0 : 0.0;
       const size_t kc = k < Ktot ? k : Ktot - 1;
-      pb[q] = ld(bp + kc * ldb);
+      if (WH) pb[q] = ld(bp + kc * ldb);
       pa[q] = ld(ap + kc * lda);
     }
   };
@@ -377,8 +378,10 @@ __device__ __forceinline__ void ho_tile(
 #pragma unroll
     for (int q = 0; q < S::QY; q++) {
       const int kk = yk + S::RY * q;
-      sY[buf][0][kk][yc] = fy[q] * pb[q].x;
-      sY[buf][1][kk][yc] = fy[q] * pb[q].y;
+      if (WH) {
+        sY[buf][0][kk][yc] = fy[q] * pb[q].x;
+        sY[buf][1][kk][yc] = fy[q] * pb[q].y;
+      }
       sY[buf][2][kk][yc] = fy[q] * pa[q].x;
       sY[buf][3][kk][yc] = fy[q] * pa[q].y;
     }
@@ -396,8 +399,10 @@ __device__ __forceinline__ void ho_tile(
       }
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        br[b] = sY[buf][0][kk][wc * 32 + b * 16 + lr];
-        bi[b] = sY[buf][1][kk][wc * 32 + b * 16 + lr];
+        if (WH) {
+          br[b] = sY[buf][0][kk][wc * 32 + b * 16 + lr];
+          bi[b] = sY[buf][1][kk][wc * 32 + b * 16 + lr];
+        }
         ar[b] = sY[buf][2][kk][wc * 32 + b * 16 + lr];
         ai[b] = sY[buf][3][kk][wc * 32 + b * 16 + lr];
       }
@@ -406,8 +411,10 @@ __device__ __forceinline__ void ho_tile(
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-          hre[a][b] = mfma(xr[a], br[b], hre[a][b]);
-          him[a][b] = mfma(xr[a], bi[b], him[a][b]);
+          if (WH) {
+            hre[a][b] = mfma(xr[a], br[b], hre[a][b]);
+            him[a][b] = mfma(xr[a], bi[b], him[a][b]);
+          }
           ore[a][b] = mfma(xr[a], ar[b], ore[a][b]);
           oim[a][b] = mfma(xr[a], ai[b], oim[a][b]);
         }
@@ -415,8 +422,10 @@ __device__ __forceinline__ void ho_tile(
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-          hre[a][b] = mfma(xi[a], bi[b], hre[a][b]);
-          him[a][b] = mfma(nxi[a], br[b], him[a][b]);
+          if (WH) {
+            hre[a][b] = mfma(xi[a], bi[b], hre[a][b]);
+            him[a][b] = mfma(nxi[a], br[b], him[a][b]);
+          }
           ore[a][b] = mfma(xi[a], ai[b], ore[a][b]);
           oim[a][b] = mfma(nxi[a], ar[b], oim[a][b]);
         }
@@ -459,20 +468,26 @@ __device__ __forceinline__ void ho_tile(
         double orr = ore[a][b][r], oi = oim[a][b][r];
         if (accumulate) {
           const size_t up = (size_t)i + (size_t)j * ldh;
-          const double2 h0 = ld(H + up), o0 = ld(O + up);
-          hr += h0.x;
-          hi += h0.y;
+          if (WH) {
+            const double2 h0 = ld(H + up);
+            hr += h0.x;
+            hi += h0.y;
+          }
+          const double2 o0 = ld(O + up);
           orr += o0.x;
           oi += o0.y;
         }
         ist_add(box, i, j, hr, hi, orr, oi);
-        store_upper(H, O, ldh, i, j, hr, hi, orr, oi, 0);
+        if (WH) store_upper(H, O, ldh, i, j, hr, hi, orr, oi, 0);
+        else store_upper_o(O, ldh, i, j, orr, oi);
       }
   }
 }
 
 // Single k-point: one workgroup per upper 64 x 64 tile pair I <= J, dealt to
-// the 8 XCDs in contiguous ranges, J-major within each range.
+// the 8 XCDs in contiguous ranges, J-major within each range.  WH = false:
+// O alone.
+template <bool WH>
 __global__ __launch_bounds__(NTH) void ho_kernel(
     int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
     const double2 *__restrict__ YB, long long ldb,
@@ -489,8 +504,8 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   while (J * (J + 1) / 2 > t) J--;
   const int I = t - J * (J + 1) / 2;
   IstBox box{ivgk, vgpc, T, lo0, lo1, lo2, n0, n1, n2};
-  ho_tile<TM, TM, TK>(ngp, Ktot, X, ldx, YB, ldb, YA, lda, box, H, O, ldh, accumulate, I * TM,
-                      J * TM);
+  ho_tile<TM, TM, TK, const double2 *, double2 *, IstBox, WH>(
+      ngp, Ktot, X, ldx, YB, ldb, YA, lda, box, H, O, ldh, accumulate, I * TM, J * TM);
 }
 
 // Grouped over k-points: a list of (k, I0, J0) tiles (k < 0 = padding) is
@@ -565,6 +580,7 @@ __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp, PA __res
   const int M = Ms[ias];
   const double2 *L = loa + (size_t)ias * maxnlo * maxaa;
 
+  if (H) {  // null for overlapsetup alone
   double2 acc[MAXLO];
 #pragma unroll
   for (int r = 0; r < MAXLO; r++) acc[r] = z2(0, 0);
@@ -592,6 +608,7 @@ __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp, PA __res
       st(H + e, v);
       st(H + (size_t)g + (size_t)(r0 + r) * ldh, z2(v.x, -v.y));
     }
+  }
 
   int c = r0;
   for (int ilo = 0; ilo < nlorb[is]; ilo++) {
@@ -650,8 +667,9 @@ __device__ __forceinline__ void lolo_body(int r, int c, int ngp, int nlotot,
   const LoRow P = lorow[r], Q = lorow[c];
   const size_t up = (size_t)(ngp + r) + (size_t)(ngp + c) * ldh;
   double2 hv = z2(0, 0), ov = z2(0, 0);
+  const bool wh = H != nullptr;  // null for overlapsetup alone
   if (accumulate) {
-    hv = ld(H + up);
+    if (wh) hv = ld(H + up);
     ov = ld(O + up);
   }
   if (P.ias == Q.ias) {
@@ -662,11 +680,11 @@ __device__ __forceinline__ void lolo_body(int r, int c, int ngp, int nlotot,
     if (P.l == Q.l && P.mi == Q.mi)
       ov.x += ololo_eff[P.ilo + nlomax * (Q.ilo + (size_t)nlomax * P.ias)];
   }
-  st(H + up, hv);
+  if (wh) st(H + up, hv);
   st(O + up, ov);
   if (r != c) {
     const size_t lo = (size_t)(ngp + c) + (size_t)(ngp + r) * ldh;
-    st(H + lo, z2(hv.x, -hv.y));
+    if (wh) st(H + lo, z2(hv.x, -hv.y));
     st(O + lo, z2(ov.x, -ov.y));
   }
 }
@@ -797,10 +815,15 @@ hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
   const double2 *X = k_major_view(c, apwalm, &ldx);
   const double2 *YA = a2t ? a2t : X;
   const long long lda = a2t ? (long long)ldb : ldx;
-  hipLaunchKernelGGL(ho_kernel, dim3((unsigned)npairs), dim3(NTH), 0, s, ngp, c->Ktot,
-                     X, ldx, bt, (long long)ldb, YA, lda, ivgk, vgpc,
-                     (const IstEntry *)c->ist.p, lo0, lo1, lo2, n0, n1, n2, h, o,
-                     (long long)ldh, flags & EXH_ACCUMULATE);
+  if (h)
+    hipLaunchKernelGGL(ho_kernel<true>, dim3((unsigned)npairs), dim3(NTH), 0, s, ngp, c->Ktot,
+                       X, ldx, bt, (long long)ldb, YA, lda, ivgk, vgpc,
+                       (const IstEntry *)c->ist.p, lo0, lo1, lo2, n0, n1, n2, h, o,
+                       (long long)ldh, flags & EXH_ACCUMULATE);
+  else  // overlapsetup alone: bt and H are not read
+    hipLaunchKernelGGL(ho_kernel<false>, dim3((unsigned)npairs), dim3(NTH), 0, s, ngp, c->Ktot,
+                       X, ldx, X, ldx, YA, lda, ivgk, vgpc, (const IstEntry *)c->ist.p, lo0,
+                       lo1, lo2, n0, n1, n2, h, o, (long long)ldh, flags & EXH_ACCUMULATE);
   return hipGetLastError();
 }
 
@@ -854,6 +877,21 @@ hipError_t launch_lolo(hipStream_t s, const exh_context *c, int ngp, double2 *h,
                      (const double2 *)c->lolo.p, c->maxnlo,
                      (const double *)c->ololo_eff.p, c->nlomax, h, o, (long long)ldh,
                      flags & EXH_ACCUMULATE);
+  return hipGetLastError();
+}
+
+// hamiltonoverlapocopy_UL.f90:18-22 for one matrix: lower := conj(upper).
+__global__ void mirror_kernel(int n, long long ld, double2 *__restrict__ M) {
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i >= n || j >= n || i <= j) return;
+  const double2 u = M[(size_t)j + (size_t)i * ld];
+  M[(size_t)i + (size_t)j * ld] = z2(u.x, -u.y);
+}
+
+hipError_t launch_mirror(hipStream_t s, int n, double2 *m, int ld) {
+  if (n <= 1) return hipSuccess;
+  hipLaunchKernelGGL(mirror_kernel, dim3(cdiv(n, 16), cdiv(n, 16)), dim3(256), 0, s, n,
+                     (long long)ld, m);
   return hipGetLastError();
 }
 

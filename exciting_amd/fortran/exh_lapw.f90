@@ -17,6 +17,9 @@
 !   exh_gencfun                  replaces  gencfun (src/gencfun.f90:13)
 !   exh_genveffig                replaces  genveffig (src/genveffig.f90:13)
 !   exh_genmeffig                replaces  genmeffig (src/genmeffig.f90:13)
+!   exh_overlapsetup             replaces  overlapsetup (system, ngp, apwalm, igpig,
+!                                          vgpc) called alone, as calc_vnlmat does
+!                                          (src/src_hybrids/calc_vnlmat.f90:52-53)
 !   exh_hmlint                   replaces  hmlint's spinless part
 !                                          (src/src_eigensystem/hmlint.f90:53-55)
 module exh_lapw
@@ -45,7 +48,7 @@ module exh_lapw
             exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace, &
             exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
-            exh_set_mt_device
+            exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -155,6 +158,26 @@ module exh_lapw
       real(c_double), intent(in) :: veffir(*), cfunir(*)
       integer(c_int), value :: valence_rel_none, iora_star
       complex(c_double_complex), intent(out) :: meffig(*), m2effig(*)
+    end function
+
+    ! overlapsetup.f90 alone: o accumulated; h only mirrored (may be c_null_ptr
+    ! through the device form).
+    integer(c_int) function exh_overlapsetup(ctx, ngp, rank, apwalm, igpig, vgpc, h, o) &
+        bind(C, name="exh_overlapsetup")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp, rank
+      complex(c_double_complex), intent(in) :: apwalm(*)
+      integer(c_int), intent(in) :: igpig(*)
+      real(c_double), intent(in) :: vgpc(*)
+      complex(c_double_complex), intent(inout) :: h(*), o(*)
+    end function
+
+    integer(c_int) function exh_overlapsetup_device(ctx, stream, ngp, rank, apwalm, igpig, &
+        vgpc, h, o, flags) bind(C, name="exh_overlapsetup_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, apwalm, igpig, vgpc, h, o
+      integer(c_int), value :: ngp, rank, flags
     end function
 
     ! hmlint.f90:53-55 (mt_kin + mt_pot): veffmt(lmmaxvr,nrmtmax,natmtot),

@@ -92,6 +92,8 @@ def lib():
     L.exh_set_mt_device.argtypes = [_p, _p, _i, _i] + [_p] * 8
     L.exh_hmlint.argtypes = [_p, _i, _p, _p, _i, _p, _p, _p, _i]
     L.exh_hmlint_device.argtypes = [_p, _p, _i, _p, _p, _i, _p, _p, _p, _i]
+    L.exh_overlapsetup.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
+    L.exh_overlapsetup_device.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
     _lib = L
     return L
 
@@ -112,7 +114,7 @@ EXPORTS = (
     "exh_get_batch_plan", "exh_tune_batch", "exh_set_batch_workspace",
     "exh_set_interstitial_device", "exh_gencfun", "exh_gencfun_device", "exh_genveffig",
     "exh_genmeffig", "exh_genveffig_device", "exh_set_mt_device", "exh_hmlint",
-    "exh_hmlint_device",
+    "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
@@ -487,6 +489,32 @@ class Context:
                                                          _ptr(igpig), _ptr(vgpc), _ptr(h),
                                                          _ptr(o), flags),
                     "exh_hamiltonandoverlapsetup")
+        return system
+
+    def overlapsetup(self, system: EvSystem, ngp, apwalm, igpig, vgpc,
+                     flags: int = ACCUMULATE, stream=None):
+        """overlapsetup.f90 alone: accumulates system%overlap%za and mirrors
+        both matrices' upper triangles into their lower ones (the reference's
+        hamiltonoverlapocopy_UL); H itself is not computed.  system.hamilton
+        may be None."""
+        h = system.hamilton.za if system.hamilton is not None else None
+        o, rank = system.overlap.za, system.overlap.rank
+        if _is_torch(o):
+            rc = self._device_call(stream, lambda st: lib().exh_overlapsetup_device(
+                self._h, st, ngp, rank, _ptr(apwalm), _ptr(igpig), _ptr(vgpc), _ptr(h), _ptr(o),
+                flags))
+            self._check(rc, "exh_overlapsetup_device")
+            return system
+        for m in (h, o):
+            if m is not None and not (isinstance(m, np.ndarray) and m.flags.f_contiguous
+                                      and m.dtype == np.complex128 and m.shape == (rank, rank)):
+                raise ExhError("H and O must be Fortran-ordered complex128 rank x rank arrays")
+        if flags != ACCUMULATE:
+            raise ExhError("the host overlapsetup accumulates, as the reference does")
+        apwalm = _host(apwalm, np.complex128)
+        igpig, vgpc = _host(igpig, np.int32), _host(vgpc, np.float64)
+        self._check(lib().exh_overlapsetup(self._h, ngp, rank, _ptr(apwalm), _ptr(igpig),
+                                           _ptr(vgpc), _ptr(h), _ptr(o)), "exh_overlapsetup")
         return system
 
     # ---- batched k-points -------------------------------------------------

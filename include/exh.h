@@ -225,6 +225,19 @@ int exh_setup_device(exh_context *ctx, void *stream, int ngp, int rank,
                      const exh_z *apwalm, const int *igpig, const double *vgpc,
                      exh_z *h, exh_z *o, int flags);
 
+/* overlapsetup.f90:8-166 alone: O(rank,rank) accumulated (muffin-tin, lo and
+ * interstitial parts, upper triangle) and mirrored; then, as the reference
+ * calls hamiltonoverlapocopy_UL for both matrices (:164-166), the lower
+ * triangle of h(rank,rank) is set to the conjugate of its upper one (h may be
+ * NULL to leave it alone).  Used without hamiltonsetup by calc_vnlmat
+ * (src_hybrids/calc_vnlmat.f90:49-60). */
+int exh_overlapsetup(exh_context *ctx, int ngp, int rank, const exh_z *apwalm,
+                     const int *igpig, const double *vgpc, exh_z *h, exh_z *o);
+/* Same in device memory; flags EXH_ACCUMULATE or 0 (overwrite O). */
+int exh_overlapsetup_device(exh_context *ctx, void *stream, int ngp, int rank,
+                            const exh_z *apwalm, const int *igpig, const double *vgpc,
+                            exh_z *h, exh_z *o, int flags);
+
 /* ---- batched k-points ---------------------------------------------------- */
 /* No reference counterpart as one call: the reference runs match and
  * hamiltonandoverlapsetup once per k-point inside the k loop of the SCF cycle

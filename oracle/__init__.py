@@ -360,3 +360,34 @@ def hmlint(sys_, lmaxvr=None, veffmt=None, lofr=None, valence_rel_none=None):
     if rc != 0:
         raise RuntimeError(f"orc_hmlint failed: {rc}")
     return aa, loa, lolo
+
+
+def overlapsetup(sys_, ik: int, apwalm: np.ndarray, h: np.ndarray = None, fast: bool = True):
+    """overlapsetup.f90:8-166 alone on zero O: the muffin-tin, lo and
+    interstitial parts, then hamiltonoverlapocopy_UL (:164-166) on O and on h
+    (a zero matrix unless given).  Returns (h, O)."""
+    s = sys_.s
+    k = s.kpts[ik]
+    n = k.ngk + s.nlotot
+    o = np.zeros((n, n), dtype=np.complex128, order="F")
+    h = np.zeros((n, n), dtype=np.complex128, order="F") if h is None else np.asfortranarray(h)
+    zg = scipy_zgemm() if fast else None
+    a = np.asfortranarray(apwalm)
+    ig = np.ascontiguousarray(k.igpig, dtype=np.int32)
+    vg = np.asfortranarray(k.vgpc)
+    rc = lib().orc_overlapsetup(sys_.ptr, k.ngk, a.ctypes.data, ig.ctypes.data, vg.ctypes.data,
+                                n, o.ctypes.data, zg)
+    if rc != 0:
+        raise RuntimeError(f"orc_overlapsetup failed: {rc}")
+    lib().orc_copy_ul(n, h.ctypes.data, o.ctypes.data)
+    return h, o
+
+
+def calc_vnlmat(O: np.ndarray, evec: np.ndarray, vxnl: np.ndarray) -> np.ndarray:
+    """src_hybrids/calc_vnlmat.f90:54-73: with X = evec(1:n, :), temp = X^H O,
+    temp1 = vxnl temp, vnlmat = temp^H temp1 (NumPy products)."""
+    n = O.shape[0]
+    X = evec[:n, :]
+    temp = X.conj().T @ O
+    temp1 = vxnl @ temp
+    return temp.conj().T @ temp1
