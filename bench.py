@@ -526,6 +526,53 @@ def vnlmat_sample(W, nstfv=400, reps=3, cpu=True):
     return res
 
 
+def kinetic_sample(W, nstfv=400, reps=3, cpu=True, cpu_states=4):
+    """kinetic_energy.f90 on k-point 0 of the workload ('none' relativity):
+    muffin-tin part (kinetic radial integrals, A^T x per atom, quadratic
+    forms) and interstitial part (3 directions x a batched FFT pair over all
+    states), inputs resident in HBM; random states."""
+    import dataclasses
+    import torch
+    from exciting_amd import kinetic, synth
+    s = synth.add_mt_potential(dataclasses.replace(W.s), lmaxvr=8, seed=5)
+    d = W.dks[0]
+    W.ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, d.apwalm)
+    n = d.rank
+    g = torch.Generator(device="cuda").manual_seed(4)
+    X = torch.randn(nstfv, n, dtype=torch.complex128, device="cuda", generator=g)
+    rng = np.random.default_rng(3)
+    cfunir = torch.from_numpy(0.5 + 0.5 * rng.uniform(0, 1, s.ngrtot)).cuda()
+    dev = lambda a: torch.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).cuda()
+    lofr, veffmt = dev(s.lofr), dev(s.veffmt)
+    W.ctx.sys = s
+    plan = kinetic.KineticPlan(s)
+    kinetic.kinetic_energy(W.ctx, d, X, cfunir, lofr, True, veffmt, 8, plan=plan)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        kinetic.kinetic_energy(W.ctx, d, X, cfunir, lofr, True, veffmt, 8, plan=plan)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    res = {"ms": ms, "nstfv": nstfv, "n": n,
+           "ffts": 6 * nstfv, "grid": [int(x) for x in s.intgv[:, 1] - s.intgv[:, 0] + 1],
+           "workload": f"k-point 0 of {W.spec.name}: n = {n}, nstfv = {nstfv}"}
+    if cpu:
+        import oracle
+        S = oracle.System(s)
+        A = d.apwalm.cpu().numpy().reshape(d.apwalm_shape, order="F")
+        Xc = X[:cpu_states].cpu().numpy().T.copy()
+        t0 = time.perf_counter()
+        oracle.kinetic_energy(S, 0, Xc, A, cfunir.cpu().numpy(), s.lofr, s.veffmt, True)
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": 1e3 * t * nstfv / cpu_states,
+                               "unit": f"ms per k-point ({nstfv} states, scaled)",
+                               "cores": 1, "kind": "port",
+                               "sample": (f"oracle kinetic_energy (NumPy restatement) on "
+                                          f"{cpu_states} states: {t:.1f} s, scaled to {nstfv}")}
+    W.ctx.sys = W.s
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -571,6 +618,8 @@ def main():
     ap.add_argument("--vnlmat", choices=("on", "off", "only"), default="on",
                     help="time calc_vnlmat (match + overlapsetup alone + 3 products) on "
                          "k-point 0 ('only': nothing else)")
+    ap.add_argument("--kinetic", choices=("on", "off", "only"), default="on",
+                    help="time kinetic_energy on k-point 0 ('only': nothing else)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -601,6 +650,10 @@ def main():
 
     if args.interstitial == "only":
         print(json.dumps({"interstitial": interstitial_sample(W.ctx, s)}), flush=True)
+        return
+    if args.kinetic == "only":
+        print(json.dumps({"kinetic": kinetic_sample(W, cpu=args.cpu_baseline == "on")}),
+              flush=True)
         return
     if args.vnlmat == "only":
         print(json.dumps({"vnlmat": vnlmat_sample(W, cpu=args.cpu_baseline == "on")}),
@@ -696,6 +749,9 @@ def main():
     if args.vnlmat == "on":
         out["vnlmat"] = vnlmat_sample(W, cpu=args.cpu_baseline == "on")
         log(f"vnlmat: {out['vnlmat']}")
+    if args.kinetic == "on":
+        out["kinetic"] = kinetic_sample(W, cpu=args.cpu_baseline == "on")
+        log(f"kinetic: {out['kinetic']}")
     W.free()
 
     if args.uneven == "on":

@@ -36,7 +36,8 @@ constexpr int kRadThreads = 384; // radial kernel: one pair per thread up to 27 
 
 struct HmlArgs {
   int nf, nfa, p, lmaxapw, nrmtmax, nlomax, lmmaxvr, nchunk;
-  double zora;   // alpha^2/2 for non-'none' relativity, else 0 (mt_kin.f90:55-59)
+  double zora;   // alpha^2/2 when a mass factor is applied (mt_kin.f90:55-59)
+  int kmass;     // kinetic mass factor on APW-APW pairs (1) / pairs with an LO (2)
 };
 
 __device__ __forceinline__ double radial_value(const HmlArgs &a, const double *__restrict__ apwfr,
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kRadThreads) void hml_radial_kernel(
   __shared__ double sF[kRC][kNfMax + 1];
   __shared__ double sD[kRC][kNfMax + 1];
   __shared__ double sV[kRC][kCH];
-  __shared__ double sW[kRC], sR2[kRC];
+  __shared__ double sW[kRC], sR2[kRC], sRM[kRC];
   const int ias = blockIdx.x, is = atom_species[ias], nr = nrmt[is];
   const bool kin = (int)blockIdx.y == a.nchunk;
   const int lm0 = blockIdx.y * kCH;
@@ -105,15 +106,19 @@ __global__ __launch_bounds__(kRadThreads) void hml_radial_kernel(
     if (kin) {
       for (int r = tid; r < kRC; r += kRadThreads) {
         const int ir = r0 + r;
-        double w = 0.0, r2 = 0.0;
+        double w = 0.0, r2 = 0.0, rm = 1.0;
         if (ir < nr) {
           const double x = spr[ir + (size_t)a.nrmtmax * is];
           r2 = x * x;
-          const double v00 = veffmt[(size_t)a.lmmaxvr * (ir + (size_t)a.nrmtmax * ias)];
-          w = 0.5 * wts[ir + (size_t)a.nrmtmax * is] / (1.0 - a.zora * v00 * y00);
+          w = 0.5 * wts[ir + (size_t)a.nrmtmax * is];
+          if (a.kmass) {
+            const double v00 = veffmt[(size_t)a.lmmaxvr * (ir + (size_t)a.nrmtmax * ias)];
+            rm = 1.0 / (1.0 - a.zora * v00 * y00);
+          }
         }
         sW[r] = w;
         sR2[r] = r2;
+        sRM[r] = rm;
       }
     } else {
       for (int i = tid; i < kRC * kCH; i += kRadThreads) {
@@ -137,11 +142,16 @@ __global__ __launch_bounds__(kRadThreads) void hml_radial_kernel(
     for (int q = 0; q < PPT; q++) {
       if (tid + q * kRadThreads >= npair) continue;
       const int f1 = pf1[q], f2 = pf2[q];
-      if (kin) {
-        double s = acc[q][0];
-        for (int r = 0; r < kRC; r++)
-          s += sW[r] * (sD[r][f1] * sD[r][f2] * sR2[r] + lfac[q] * (sF[r][f1] * sF[r][f2]));
-        acc[q][0] = s;
+      if (kin) {  // acc[q][0]: plain, acc[q][1]: with the mass factor
+        double s0 = acc[q][0], s1 = acc[q][1];
+        for (int r = 0; r < kRC; r++) {
+          const double t =
+              sW[r] * (sD[r][f1] * sD[r][f2] * sR2[r] + lfac[q] * (sF[r][f1] * sF[r][f2]));
+          s0 += t;
+          s1 += t * sRM[r];
+        }
+        acc[q][0] = s0;
+        acc[q][1] = s1;
       } else {
         for (int r = 0; r < kRC; r++) {
           const double pr = sF[r][f1] * sF[r][f2] * sW[r];
@@ -158,7 +168,9 @@ __global__ __launch_bounds__(kRadThreads) void hml_radial_kernel(
     const int f1 = pf1[q], f2 = pf2[q];
     if (kin) {
       const bool same = fl[is * kNfMax + f1] == fl[is * kNfMax + f2];
-      const double v = same ? acc[q][0] : 0.0;
+      const bool lo = f1 >= a.nfa || f2 >= a.nfa;
+      const bool mass = a.kmass & (lo ? 2 : 1);
+      const double v = same ? (mass ? acc[q][1] : acc[q][0]) : 0.0;
       ikin[((size_t)ias * nf + f1) * nf + f2] = v;
       ikin[((size_t)ias * nf + f2) * nf + f1] = v;
     } else {
@@ -470,7 +482,7 @@ int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *ve
   if (rc) return rc;
   const double alpha = 1.0 / 137.03599911;
   HmlArgs a{nf, nfa, p, c->lmaxapw, c->nrmtmax, std::max(c->nlomax, 1), lmmaxvr, nchunk,
-            valence_rel_none ? 0.0 : 0.5 * alpha * alpha};
+            0.5 * alpha * alpha, valence_rel_none ? 0 : 3};
   const int *counts = (const int *)c->hml_count.p;
   const int npair = nf * (nf + 1) / 2;
   const double *lf = lofr ? lofr : (const double *)c->hml_w.p;  // never read when nlomax = 0
@@ -516,6 +528,48 @@ int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *ve
   });
   if (e == hipSuccess) e = mark_done(c, s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "hmlint contract");
+}
+
+int exh_mt_kinetic_device(exh_context *c, void *stream, int lmaxvr, const double *veffmt,
+                          const double *lofr, int mass_flags, double *tkin) {
+  if (!c) return EXH_EINVAL;
+  if (int rc = wait_done(c)) return rc;
+  if (!c->have_basis || !c->have_radial)
+    return fail(c, EXH_ESTATE, "mt_kinetic: basis and radial data not set");
+  if (mass_flags & ~3) return fail(c, EXH_EINVAL, "mt_kinetic: unknown mass flags");
+  if (!tkin || (c->nlomax > 0 && !lofr) || (mass_flags && !veffmt))
+    return fail(c, EXH_EINVAL, "mt_kinetic: null array");
+  if (mass_flags && (lmaxvr < 0 || lmaxvr > 50))
+    return fail(c, EXH_EINVAL, "mt_kinetic: lmaxvr outside 0..50");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  const int lmv = mass_flags ? lmaxvr : 0;
+  if (int rc = prepare(c, c->hml_lmaxvr >= 0 ? c->hml_lmaxvr : lmv)) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int p = c->apwordmax, nfa = (c->lmaxmat + 1) * p, nf = nfa + c->nlomax;
+  const double alpha = 1.0 / 137.03599911;
+  HmlArgs a{nf, nfa, p, c->lmaxapw, c->nrmtmax, std::max(c->nlomax, 1), (lmv + 1) * (lmv + 1), 0,
+            0.5 * alpha * alpha, mass_flags};
+  const int *counts = (const int *)c->hml_count.p;
+  const int npair = nf * (nf + 1) / 2;
+  const double *lf = lofr ? lofr : (const double *)c->hml_w.p;  // never read when nlomax = 0
+  const double *vm = veffmt ? veffmt : (const double *)c->hml_w.p;  // read only with a mass flag
+  hipError_t e = timed(c, EXH_K_HMLRAD, s, [&] {
+    dim3 grid(c->natmtot, 1);
+#define LAUNCH(P)                                                                               \
+  hipLaunchKernelGGL(hml_radial_kernel<P>, grid, dim3(kRadThreads), 0, s, a,                    \
+                     (const int *)c->atom_species_d.p, counts + 2 * c->nspecies,                \
+                     (const double *)c->hml_w.p, (const double *)c->hml_spr.p,                  \
+                     (const int *)c->hml_fl.p, (const double *)c->apwfr_d.p, lf, vm,            \
+                     (double *)nullptr, tkin)
+    if (npair <= kRadThreads) LAUNCH(1);
+    else if (npair <= 2 * kRadThreads) LAUNCH(2);
+    else if (npair <= 4 * kRadThreads) LAUNCH(4);
+    else LAUNCH(8);
+#undef LAUNCH
+    return hipGetLastError();
+  });
+  if (e == hipSuccess) e = mark_done(c, s);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "mt_kinetic");
 }
 
 int exh_hmlint(exh_context *c, int lmaxvr, const double *veffmt, const double *lofr,

@@ -34,6 +34,8 @@ module exh_lapw
   integer(c_int), parameter, public :: EXH_GRAPH = 4   ! batch: capture / replay a HIP graph
   ! outputs of exh_genveffig_device
   integer(c_int), parameter, public :: EXH_VEFFIG = 1, EXH_MEFFIG = 2, EXH_M2EFFIG = 4
+  ! mass factors of exh_mt_kinetic_device
+  integer(c_int), parameter, public :: EXH_KIN_MASS_APW = 1, EXH_KIN_MASS_LO = 2
 
   ! One k-point of a batch (exh_kpoint in include/exh.h); device pointers.
   type, bind(C), public :: exh_kpoint
@@ -48,7 +50,7 @@ module exh_lapw
             exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace, &
             exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
-            exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device
+            exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -197,6 +199,14 @@ module exh_lapw
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, stream, veffmt, lofr, aa, loa, lolo
       integer(c_int), value :: lmaxvr, valence_rel_none, flags
+    end function
+
+    ! The kinetic radial integrals alone (mt_kin, kinetic_energy): tkin(nf,nf,natmtot).
+    integer(c_int) function exh_mt_kinetic_device(ctx, stream, lmaxvr, veffmt, lofr, &
+        mass_flags, tkin) bind(C, name="exh_mt_kinetic_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, veffmt, lofr, tkin
+      integer(c_int), value :: lmaxvr, mass_flags
     end function
 
     ! exh_set_mt from device memory.

@@ -94,6 +94,7 @@ def lib():
     L.exh_hmlint_device.argtypes = [_p, _p, _i, _p, _p, _i, _p, _p, _p, _i]
     L.exh_overlapsetup.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
     L.exh_overlapsetup_device.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
+    L.exh_mt_kinetic_device.argtypes = [_p, _p, _i, _p, _p, _i, _p]
     _lib = L
     return L
 
@@ -115,6 +116,7 @@ EXPORTS = (
     "exh_set_interstitial_device", "exh_gencfun", "exh_gencfun_device", "exh_genveffig",
     "exh_genmeffig", "exh_genveffig_device", "exh_set_mt_device", "exh_hmlint",
     "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
+    "exh_mt_kinetic_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
@@ -124,6 +126,7 @@ KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_
 MATCH = 2   # batch: run match first
 GRAPH = 4   # batch: capture in a HIP graph, replay on identical calls
 VEFFIG, MEFFIG, M2EFFIG = 1, 2, 4   # exh_genveffig_device outputs
+KIN_MASS_APW, KIN_MASS_LO = 1, 2     # exh_mt_kinetic_device mass factors
 
 
 def plan_count() -> int:
@@ -368,6 +371,20 @@ class Context:
                                      1 if valence_rel_none else 0, *[_ptr(x) for x in outs],
                                      flags), "exh_hmlint")
         return tuple(outs)
+
+    def mt_kinetic(self, lofr, mass_flags=0, veffmt=None, lmaxvr=0, stream=None):
+        """The kinetic radial integrals of every atom (exh_mt_kinetic_device):
+        a CUDA float64 tensor (natmtot, nf, nf), nf = (lmaxmat+1)*apwordmax
+        + nlomax, from CUDA tensors lofr (and veffmt with a mass flag,
+        KIN_MASS_APW | KIN_MASS_LO)."""
+        import torch
+        nf = (self.sys.lmaxmat + 1) * self.sys.apwordmax + self.sys.nlomax
+        out = torch.zeros((sum(self.natoms), nf, nf), dtype=torch.float64,
+                          device=f"cuda:{self.device}")
+        rc = self._device_call(stream, lambda st: lib().exh_mt_kinetic_device(
+            self._h, st, lmaxvr, _ptr(veffmt), _ptr(lofr), mass_flags, _ptr(out)))
+        self._check(rc, "exh_mt_kinetic_device")
+        return out
 
     # ---- interstitial producers ----------------------------------------
     def _atposc(self, atposc):

@@ -159,6 +159,23 @@ int exh_hmlint_device(exh_context *ctx, void *stream, int lmaxvr, const double *
                       const double *lofr, int valence_rel_none, exh_z *aa, exh_z *loa,
                       exh_z *lolo, int flags);
 
+/* The kinetic radial integrals alone, as mt_kin.f90:117-170 and
+ * kinetic_energy.f90:50-183 (t_aa, t_alo, t_lolo) take them: for the radial
+ * functions f of each atom -- APW (l, io) as f = l*apwordmax + io for
+ * l <= lmaxmat, then LO ilo as f = (lmaxmat+1)*apwordmax + ilo -- and f1, f2
+ * of equal l, the spline integral of 1/2 m(r) (u1' u2' r^2 + l(l+1) u1 u2);
+ * zero for pairs of different l.  m(r) = 1/(1 - alpha^2/2 veffmt(1,r) y00)
+ * on APW-APW pairs with EXH_KIN_MASS_APW and on pairs with an LO with
+ * EXH_KIN_MASS_LO, else 1.  mt_kin uses both for non-'none' relativity and
+ * neither for 'none'; kinetic_energy.f90 uses both for ZORA and, for 'none',
+ * only EXH_KIN_MASS_LO (its t_alo/t_lolo always carry the factor,
+ * :139-150, :159-167).  tkin(nf, nf, natmtot) in device memory,
+ * nf = (lmaxmat+1)*apwordmax + nlomax.  veffmt (lmaxvr) is read only with a
+ * mass flag. */
+enum { EXH_KIN_MASS_APW = 1, EXH_KIN_MASS_LO = 2 };
+int exh_mt_kinetic_device(exh_context *ctx, void *stream, int lmaxvr, const double *veffmt,
+                          const double *lofr, int mass_flags, double *tkin);
+
 /* ---- interstitial producers --------------------------------------------- */
 /* They need exh_set_gvectors with the whole FFT box as the G set (ngrtot =
  * ngrid(1)*ngrid(2)*ngrid(3), ngrid(d) = intgv(d,2) - intgv(d,1) + 1, as

@@ -391,3 +391,115 @@ def calc_vnlmat(O: np.ndarray, evec: np.ndarray, vxnl: np.ndarray) -> np.ndarray
     temp = X.conj().T @ O
     temp1 = vxnl @ temp
     return temp.conj().T @ temp1
+
+
+def kinetic_energy(sys_, ik, evecfv, apwalm, cfunir, lofr, veffmt, valence_rel_none=True,
+                   veffir=None):
+    """kinetic_energy.f90:1-230 for 'none' and ZORA: engyknst(1:nstfv).
+    evecfv (nmat, nstfv), apwalm (ngkmax, apwordmax, lmmaxapw, natmtot),
+    cfunir/veffir (ngrtot), lofr (nrmtmax, 2, nlomax, natmtot),
+    veffmt (lmmaxvr, nrmtmax, natmtot).  Restated step by step: spline
+    integrals by fderiv, the zgemm of :122-135 by NumPy, the FFTs by
+    zfftifc."""
+    s = sys_.s
+    y00 = 0.28209479177387814347
+    k = s.kpts[ik]
+    ngp = k.ngk
+    nst = evecfv.shape[1]
+    p, lmaxmat = s.apwordmax, s.lmaxmat
+    lmm = (s.lmaxapw + 1) ** 2
+    e = np.zeros(nst)
+    relnone = valence_rel_none
+    loofs = ngp
+    ias = 0
+    for is_ in range(s.nspecies):
+        nr = int(s.nrmt[is_])
+        r = s.spr[:nr, is_]
+        r2 = r * r
+        r2inv = 1.0 / r2
+        for _ in range(s.natoms[is_]):
+            rm = 1.0 / (1.0 - 0.5 * ALPHA ** 2 * veffmt[0, :nr, ias] * y00)   # :63-65
+            u = lambda io, l, c: s.apwfr[:nr, c, io, l, ias]                  # noqa: E731
+            t_aa = np.zeros((p, p, lmaxmat + 1))
+            for l1 in range(lmaxmat + 1):                                     # :66-121
+                ang = float(l1 * (l1 + 1))
+                for io1 in range(s.apword[l1, is_]):
+                    for io2 in range(s.apword[l1, is_]):
+                        t1 = u(io1, l1, 0) * u(io2, l1, 0)
+                        t2 = u(io1, l1, 1) * u(io2, l1, 1)
+                        if relnone:
+                            fr = r2 * (0.5 * t2 + 0.5 * ang * t1 * r2inv)
+                        else:
+                            fr = (0.5 * t2 * rm + 0.5 * ang * t1 * rm * r2inv) * r2
+                        t_aa[io2, io1, l1] = fderiv_int(r, fr)[-1]
+            A = apwalm[:ngp, :, :, ias].reshape(ngp, p * lmm, order="F")
+            zm = A.T @ evecfv[:ngp, :]                                        # :122-135
+            nlo = s.nlorb[is_]
+            t_alo = np.zeros((p, max(s.nlomax, 1)))
+            t_lolo = np.zeros((max(s.nlomax, 1), max(s.nlomax, 1)))
+            for ilo in range(nlo):                                            # :136-158
+                l1 = s.lorbl[ilo, is_]
+                ang = float(l1 * (l1 + 1))
+                for io in range(s.apword[l1, is_] if l1 <= lmaxmat else 0):
+                    t1 = u(io, l1, 0) * lofr[:nr, 0, ilo, ias]
+                    t2 = u(io, l1, 1) * lofr[:nr, 1, ilo, ias]
+                    fr = (0.5 * t2 * rm + 0.5 * ang * t1 * rm * r2inv) * r2
+                    t_alo[io, ilo] = fderiv_int(r, fr)[-1]
+            for ilo1 in range(nlo):                                           # :159-183
+                for ilo2 in range(nlo):
+                    l1, l3 = s.lorbl[ilo1, is_], s.lorbl[ilo2, is_]
+                    if l1 != l3:
+                        continue
+                    ang = float(l1 * (l1 + 1))
+                    t1 = lofr[:nr, 0, ilo1, ias] * lofr[:nr, 0, ilo2, ias]
+                    t2 = lofr[:nr, 1, ilo1, ias] * lofr[:nr, 1, ilo2, ias]
+                    fr = (0.5 * t2 * rm + 0.5 * ang * t1 * rm * r2inv) * r2
+                    t_lolo[ilo1, ilo2] = fderiv_int(r, fr)[-1]
+            los = int(s.losize[is_])
+            for ist in range(nst):                                            # :184-199
+                zvec = np.zeros(p * lmm, dtype=complex)
+                zveclo = np.zeros(max(los, 1), dtype=complex)
+                for l3 in range(lmaxmat + 1):
+                    for m3 in range(-l3, l3 + 1):
+                        lm3 = l3 * (l3 + 1) + m3
+                        for io2 in range(s.apword[l3, is_]):
+                            for io1 in range(s.apword[l3, is_]):
+                                zvec[p * lm3 + io2] += t_aa[io1, io2, l3] * zm[p * lm3 + io1, ist]
+                if3 = 0
+                for ilo in range(nlo):
+                    l = s.lorbl[ilo, is_]
+                    for m in range(-l, l + 1):
+                        lm = l * (l + 1) + m
+                        for io1 in range(s.apword[l, is_]):
+                            zvec[p * lm + io1] += t_alo[io1, ilo] * evecfv[loofs + if3 + m + l, ist]
+                            zveclo[if3 + m + l] += t_alo[io1, ilo] * zm[p * lm + io1, ist]
+                    if1 = 0
+                    for ilo2 in range(nlo):
+                        if s.lorbl[ilo2, is_] == l:
+                            for m in range(-l, l + 1):
+                                zveclo[if3 + m + l] += (t_lolo[ilo, ilo2]
+                                                        * evecfv[loofs + if1 + m + l, ist])
+                        if1 += 2 * s.lorbl[ilo2, is_] + 1
+                    if3 += 2 * l + 1
+                e[ist] += np.vdot(zm[:, ist], zvec).real
+                if nlo:
+                    e[ist] += np.vdot(evecfv[loofs:loofs + los, ist], zveclo[:los]).real
+            loofs += los
+            ias += 1
+    # interstitial part (:205-234)
+    ngrid = ngrid_of(s.intgv)
+    idx = igfft(ngrid, s.ivg[:, k.igpig[:ngp] - 1])
+    vg = np.asarray(k.vgpc)[:, :ngp]
+    cf = np.asarray(cfunir, dtype=float)
+    if not relnone:
+        cf = cf / (1.0 - 0.5 * ALPHA * ALPHA * np.asarray(veffir))
+    ntot = int(np.prod(ngrid))
+    for ist in range(nst):
+        for ix in range(3):
+            z = np.zeros(ntot, dtype=complex)
+            z[idx] = evecfv[:ngp, ist] * vg[ix]
+            z = zfftifc(ngrid, 1, z) * cf
+            z = zfftifc(ngrid, -1, z)
+            zvec = z[idx] * vg[ix]
+            e[ist] += 0.5 * np.vdot(evecfv[:ngp, ist], zvec).real
+    return e
