@@ -573,6 +573,60 @@ def kinetic_sample(W, nstfv=400, reps=3, cpu=True, cpu_states=4):
     return res
 
 
+def forces_sample(W, nstfv=400, reps=2, cpu=True):
+    """forcek.f90 for k-point 0 of the workload (64 atoms, n = 10364) and of
+    the 8-atom cell, with nstfv random states; the CPU oracle (a literal
+    NumPy restatement with explicit n x n derivative matrices) is timed on
+    the 8-atom cell, where it finishes in seconds."""
+    import dataclasses
+    import torch
+    import oracle
+    from exciting_amd import forces, lapw, synth
+
+    def run(ctx, s, d, nst):
+        n = d.rank
+        g = torch.Generator(device="cuda").manual_seed(8)
+        X = torch.randn(nst, n, dtype=torch.complex128, device="cuda", generator=g)
+        e = torch.rand(nst, dtype=torch.float64, device="cuda", generator=g) - 0.5
+        occ = 2.0 * torch.rand(nst, dtype=torch.float64, device="cuda", generator=g)
+        ff = torch.from_numpy(np.stack([oracle.genffacg(s, i) for i in range(s.nspecies)],
+                                       axis=1)).cuda()
+        sg = torch.from_numpy(oracle.sfacg_of(s)).cuda()
+        forces.forcek(ctx, d, X, e, occ, 1.0, ff, sg)                   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            forces.forcek(ctx, d, X, e, occ, 1.0, ff, sg)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3, (X, e, occ, ff, sg)
+
+    res = {}
+    ms, _ = run(W.ctx, W.s, W.dks[0], nstfv)
+    res["sc64_k0_ms"] = ms
+    s8 = synth.build(synth.config(1), kpoints=[0])
+    ctx8 = lapw.Context(0).load(s8)
+    try:
+        d8 = lapw.DeviceKPoint(s8, 0)
+        ms8, (X, e, occ, ff, sg) = run(ctx8, s8, d8, nstfv)
+        res["sc8_k0_ms"] = ms8
+        if cpu:
+            t0 = time.perf_counter()
+            oracle.forcek(oracle.System(s8), 0, X.cpu().numpy().T.copy(), e.cpu().numpy(),
+                          occ.cpu().numpy(), 1.0, ff.cpu().numpy(), sg.cpu().numpy())
+            t = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": 1e3 * t, "unit": "ms per k-point of the 8-atom cell",
+                                   "cores": 16, "kind": "port",
+                                   "sample": (f"oracle forcek (NumPy, explicit derivative "
+                                              f"matrices) on the 8-atom cell's k-point 0, "
+                                              f"n = {d8.rank}, {nstfv} states; GPU on the same: "
+                                              f"{ms8:.1f} ms")}
+    finally:
+        ctx8.close()
+    res["workload"] = (f"{nstfv} states; k-point 0 of sc64 (n = {W.dks[0].rank}) and of sc8 "
+                       f"(n = {s8.kpts[0].ngk + s8.nlotot})")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -620,6 +674,8 @@ def main():
                          "k-point 0 ('only': nothing else)")
     ap.add_argument("--kinetic", choices=("on", "off", "only"), default="on",
                     help="time kinetic_energy on k-point 0 ('only': nothing else)")
+    ap.add_argument("--forces", choices=("on", "off", "only"), default="on",
+                    help="time forcek on k-point 0 ('only': nothing else)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -650,6 +706,10 @@ def main():
 
     if args.interstitial == "only":
         print(json.dumps({"interstitial": interstitial_sample(W.ctx, s)}), flush=True)
+        return
+    if args.forces == "only":
+        print(json.dumps({"forces": forces_sample(W, cpu=args.cpu_baseline == "on")}),
+              flush=True)
         return
     if args.kinetic == "only":
         print(json.dumps({"kinetic": kinetic_sample(W, cpu=args.cpu_baseline == "on")}),
@@ -752,6 +812,9 @@ def main():
     if args.kinetic == "on":
         out["kinetic"] = kinetic_sample(W, cpu=args.cpu_baseline == "on")
         log(f"kinetic: {out['kinetic']}")
+    if args.forces == "on":
+        out["forces"] = forces_sample(W, cpu=args.cpu_baseline == "on")
+        log(f"forces: {out['forces']}")
     W.free()
 
     if args.uneven == "on":

@@ -1,0 +1,135 @@
+"""The k-point contribution to the basis-set-correction forces, forcek.f90
+(no spin, first-variational states, tevecsv false), on the GPU.
+
+For atom alpha and direction l the reference forms dlh = i (g_i - g_j) M_H
+and dlo = -i (g_i - g_j) M_O over the basis (g the l-th component of G+k on
+the APW rows, 0 on the local-orbital rows, the G-G block through the G
+difference), with M_H = H_alpha + dpij zt1 and M_O = O_alpha + zt1 (the
+atom's muffin-tin and lo-border parts, and zt1 = -ffacg conj(sfacg) of the
+difference G, its step-function term), then ffv = X^H (dlh X + dlo X e)
+and adds wkpt sum_j occ_j Re ffv(j, j) (forcek.f90:259-335).
+
+Because the derivative matrices are commutators with diag(g), the diagonal
+needs only Y = M X:
+
+    Re ffv(j, j) = -2 sum_i g_i Im(conj(X_ij) Y_H,ij) + 2 e_j sum_i g_i Im(conj(X_ij) Y_O,ij)
+
+summed over the APW rows i.  Y is built without any n x n matrix:
+  * muffin-tin and lo-border parts per atom from B = A X (A the atom's
+    matching coefficients, l <= lmaxmat): A^H (aa B + loa^H x_lo) and
+    A^H B + O_GL x_lo, as batched library GEMMs over all atoms;
+  * the step-function term per species: zt1(G_i - G_j) =
+    conj(p_i) F_s(i, j) p_j with p = exp(i G.tau) and the real matrix
+    F_s(i, j) = -ffacg(|G_i - G_j|) (dpij F_s for H), so
+    Y += conj(p) * (F_s (p * X)) for all atoms of a species in one GEMM.
+The G differences must lie inside the FFT box (gmaxvr >= 2 gkmax, as
+exciting's defaults make it); the reference folds them back into the box
+otherwise, which this path refuses.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _rows(s):
+    """Per species: indices into apwalm's (lm, io) rows of the atom's APW
+    functions with l <= lmaxmat (forcek.f90:71-82 order), and per LO column
+    (ilo, m) its lm and ilo."""
+    p = s.apwordmax
+    out = []
+    for is_ in range(s.nspecies):
+        r = [(l * (l + 1) + m) * p + io for l in range(s.lmaxmat + 1) for m in range(-l, l + 1)
+             for io in range(s.apword[l, is_])]
+        lo = [(l * (l + 1) + m, ilo) for ilo in range(s.nlorb[is_])
+              for l in [s.lorbl[ilo, is_]] for m in range(-l, l + 1)]
+        out.append((np.array(r, np.int64), lo))
+    return out
+
+
+def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg):
+    """forcek.f90 for one k-point: a (3, natmtot) float64 CUDA tensor to add
+    to forceibs.  d: lapw.DeviceKPoint (match is run here, as the reference
+    does); evec (nstfv, nmat) complex128 (row = state); evalfv, occ (nstfv)
+    float64; ffacg (ngvec, nspecies) float64 (genffacg); sfacg
+    (ngvec, natmtot) complex128, exp(i G.tau).  CUDA tensors throughout."""
+    import torch
+    s = ctx.sys
+    dev = evec.device
+    ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, d.apwalm)
+    ngk, natm, p = d.ngk, s.natmtot, s.apwordmax
+    lmm = (s.lmaxapw + 1) ** 2
+    igp = d.igpig.cpu().numpy()[:ngk].astype(np.int64) - 1              # G index of each row
+    # index of G_i - G_j for every pair of rows, on the device (ngk^2 entries)
+    ivk = torch.as_tensor(np.asarray(s.ivg)[:, igp], device=dev)        # (3, ngk)
+    lo = [int(x) for x in np.asarray(s.intgv)[:, 0]]
+    hi = [int(x) for x in np.asarray(s.intgv)[:, 1]]
+    nb = [h - l + 1 for l, h in zip(lo, hi)]
+    box = torch.zeros((), dtype=torch.int64, device=dev)
+    for a in (2, 1, 0):                      # ivgig(i1,i2,i3): first index fastest
+        da = ivk[a][:, None] - ivk[a][None, :]
+        if bool(((da < lo[a]) | (da > hi[a])).any()):
+            raise NotImplementedError("G+k differences outside the FFT box (gmaxvr < 2 gkmax)")
+        box = box * nb[a] + (da - lo[a])
+    ivgig = torch.as_tensor(np.asarray(s.ivgig).ravel(order="F"), device=dev)
+    ig_t = ivgig[box].long() - 1                                          # (ngk, ngk)
+    if int(ig_t.max()) >= s.ngvec:
+        raise NotImplementedError("G differences beyond ngvec")
+    XG = evec[:, :ngk].transpose(0, 1).contiguous()                     # (ngk, nst)
+    nst = XG.shape[1]
+    vg = d.vgpc.view(-1, 3)[:ngk]                                        # (ngk, 3) G+k
+    dp = 0.5 * (vg @ vg.transpose(0, 1))                                 # dpij
+    A_all = d.apwalm.view(natm, lmm * p, s.ngkmax)
+    rows = _rows(s)
+    aa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.aa, 2, 0)), device=dev)
+    loa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.loa, 2, 0)), device=dev)
+    salo = np.asarray(s.oalo) + (np.asarray(s.h1loa) if s.h1loa is not None else 0.0)
+    YH = torch.zeros((natm, ngk, nst), dtype=torch.complex128, device=dev)
+    YO = torch.zeros_like(YH)
+    species = np.repeat(np.arange(s.nspecies), s.natoms)
+    lobase = np.concatenate([[0], np.cumsum([s.losize[i] for i in species])])[:-1]
+    ngk_lo = ngk
+    for is_ in range(s.nspecies):
+        at = np.nonzero(species == is_)[0]
+        r_idx, lo_cols = rows[is_]
+        at_t = torch.as_tensor(at, device=dev)
+        A = A_all[at_t][:, torch.as_tensor(r_idx, device=dev), :ngk]     # (na, nr, ngk)
+        B = torch.matmul(A, XG)                                          # (na, nr, nst)
+        Ah = A.conj().transpose(1, 2)
+        nr = len(r_idx)
+        YH[at_t] = torch.matmul(Ah, torch.matmul(aa[at_t][:, :nr, :nr], B))
+        YO[at_t] = torch.matmul(Ah, B)
+        los = int(s.losize[is_])
+        if los:
+            XL = torch.stack([evec[:, ngk_lo + int(lobase[a]):ngk_lo + int(lobase[a]) + los]
+                              .transpose(0, 1) for a in at])             # (na, los, nst)
+            # H(G, lo) = conj(loa A)^T: Y_H += A^H loa^H x_lo (forcek.f90:112-126)
+            YH[at_t] += torch.matmul(Ah, torch.matmul(loa[at_t][:, :los, :nr].conj()
+                                                      .transpose(1, 2), XL))
+            # O(G, lo) = conj(sum_io apwalm(G, io, lm, alpha) s(io, ilo)) (:170-183)
+            OGL = torch.zeros((len(at), ngk, los), dtype=torch.complex128, device=dev)
+            for c, (lm, ilo) in enumerate(lo_cols):
+                l = s.lorbl[ilo, is_]
+                for io in range(s.apword[l, is_]):
+                    sv = torch.as_tensor(salo[io, ilo, at], device=dev)
+                    OGL[:, :, c] += A_all[at_t, lm * p + io, :ngk] * sv[:, None]
+            YO[at_t] += torch.matmul(OGL.conj(), XL)
+        # step-function term of the species: F(i, j) = -ffacg(G_i - G_j)
+        F = -ffacg[ig_t, is_]                                           # (ngk, ngk) real
+        ph = sfacg[torch.as_tensor(igp, device=dev)][:, at_t].transpose(0, 1)   # (na, ngk)
+        PX = ph[:, :, None] * XG[None]                                   # p * X
+        W = PX.transpose(0, 1).reshape(ngk, -1)                          # (ngk, na nst)
+        # one real GEMM: [F; dpij F] (2 ngk x ngk) times [Re W, Im W]
+        FF = torch.cat([F, dp * F], dim=0)
+        R = FF @ torch.cat([W.real, W.imag], dim=1)                     # (2 ngk, 2 na nst)
+        m = W.shape[1]
+        FW = torch.complex(R[:ngk, :m], R[:ngk, m:]).reshape(ngk, len(at), nst).transpose(0, 1)
+        FHW = torch.complex(R[ngk:, :m], R[ngk:, m:]).reshape(ngk, len(at), nst).transpose(0, 1)
+        YO[at_t] += ph.conj()[:, :, None] * FW
+        YH[at_t] += ph.conj()[:, :, None] * FHW
+    # Re ffv(j, j) per atom and direction
+    im_h = (XG.conj()[None] * YH).imag                                   # (natm, ngk, nst)
+    im_o = (XG.conj()[None] * YO).imag
+    fh = -2.0 * torch.einsum("il,aij->alj", vg, im_h)                    # (natm, 3, nst)
+    fo = 2.0 * torch.einsum("il,aij->alj", vg, im_o)
+    per_state = fh + fo * evalfv[None, None, :]
+    return wkpt * (per_state * occ[None, None, :]).sum(dim=2).transpose(0, 1)

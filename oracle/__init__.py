@@ -503,3 +503,102 @@ def kinetic_energy(sys_, ik, evecfv, apwalm, cfunir, lofr, veffmt, valence_rel_n
             zvec = z[idx] * vg[ix]
             e[ist] += 0.5 * np.vdot(evecfv[:ngp, ist], zvec).real
     return e
+
+
+def genffacg(s, is_, cfdamp=0.0, gmaxvr=12.0, epslat=1e-6):
+    """genffacg.f90:13-40: step-function form factors of species is_ for the
+    first ngvec G vectors."""
+    gc = g_lengths(s.ivg[:, :s.ngvec], s.bvec)
+    t1 = 4 * np.pi / s.omega
+    t2 = cfdamp / gmaxvr
+    out = np.empty(s.ngvec)
+    fin = gc > epslat
+    t3 = np.exp(-(t2 * gc[fin]) ** 2) if cfdamp != 0.0 else 1.0
+    t4 = gc[fin] * s.rmt[is_]
+    out[fin] = t1 * t3 * (np.sin(t4) - t4 * np.cos(t4)) / gc[fin] ** 3
+    out[~fin] = (t1 / 3.0) * s.rmt[is_] ** 3
+    return out
+
+
+def sfacg_of(s):
+    """sfacg(ngvec, natmtot) = exp(i G.tau) (gensfacgp.f90 for the G set)."""
+    v = np.asarray(s.bvec) @ np.asarray(s.ivg[:, :s.ngvec], dtype=float)
+    return np.exp(1j * (v.T @ np.asarray(s.atposc).T))
+
+
+def forcek(sys_, ik, evecfv, evalfv, occ, wkpt, ffacg, sfacg):
+    """forcek.f90:1-404 for one k-point, no spin, first-variational states
+    only (tevecsv false): the contribution wkpt * sum_j occ_j Re ffv(j, j) to
+    forceibs(3, natmtot).  Per atom H and O of the muffin-tin and lo border
+    (:60-230), dlh/dlo with the folded G differences (:91-98, :259-285) and
+    the three zgemm (:286-335), restated with NumPy.  ffacg(ngvec, nspecies),
+    sfacg(ngvec, natmtot), evecfv(nmat, nstfv)."""
+    s = sys_.s
+    k = s.kpts[ik]
+    ngk = k.ngk
+    n = ngk + s.nlotot
+    p = s.apwordmax
+    apwalm = match(sys_, ik)
+    ngrid = np.asarray(ngrid_of(s.intgv))
+    lo = np.asarray(s.intgv)[:, 0]
+    ivk = np.asarray(s.ivg)[:, k.igpig[:ngk] - 1]                        # (3, ngk)
+    iv = ivk[:, :, None] - ivk[:, None, :]
+    iv = np.mod(iv - lo[:, None, None], ngrid[:, None, None]) + lo[:, None, None]
+    ijgij = s.ivgig[iv[0] - lo[0], iv[1] - lo[1], iv[2] - lo[2]]       # 1-based
+    assert ijgij.max() <= s.ngvec, "G differences beyond ngvec"
+    vgk = np.asarray(k.vgpc)[:, :ngk]
+    dpij = 0.5 * (vgk.T @ vgk)
+    vgc = np.asarray(s.vgc)
+    X = evecfv[:n]
+    X2 = X * evalfv[None, :]
+    out = np.zeros((3, s.natmtot))
+    ias = 0
+    lob = 0
+    for is_ in range(s.nspecies):
+        for _ in range(s.natoms[is_]):
+            rows = [(l, m, io) for l in range(s.lmaxmat + 1) for m in range(-l, l + 1)
+                    for io in range(s.apword[l, is_])]
+            apwi = np.array([apwalm[:ngk, io, l * (l + 1) + m, ias] for (l, m, io) in rows])
+            maxaa = s.maxaa
+            apwi_full = np.zeros((maxaa, ngk), dtype=complex)
+            apwi_full[:len(rows)] = apwi
+            H = np.zeros((n, n), dtype=complex)
+            O = np.zeros((n, n), dtype=complex)
+            zm = s.aa[:, :, ias] @ apwi_full
+            H[:ngk, :ngk] += apwi_full.conj().T @ zm
+            los = int(s.losize[is_])
+            if s.nlorb[is_]:
+                r0 = ngk + lob
+                H[r0:r0 + los, :ngk] += s.loa[:los, :, ias] @ apwi_full
+                H[:ngk, r0:r0 + los] = H[r0:r0 + los, :ngk].conj().T
+            O[:ngk, :ngk] += apwi_full.conj().T @ apwi_full
+            if s.nlorb[is_]:
+                c = ngk + lob
+                for ilo in range(s.nlorb[is_]):
+                    l = s.lorbl[ilo, is_]
+                    lm1, lm2 = l * (l + 1) - l, l * (l + 1) + l
+                    for io in range(s.apword[l, is_]):
+                        sv = s.oalo[io, ilo, ias] + (s.h1loa[io, ilo, ias]
+                                                     if s.h1loa is not None else 0.0)
+                        O[:ngk, c:c + 2 * l + 1] += np.conj(apwalm[:ngk, io, lm1:lm2 + 1, ias] * sv)
+                    O[c:c + 2 * l + 1, :ngk] = O[:ngk, c:c + 2 * l + 1].conj().T
+                    c += 2 * l + 1
+            for l in range(3):
+                ig = ijgij - 1
+                t1 = vgc[l, ig]
+                zt1 = -ffacg[ig, is_] * np.conj(sfacg[ig, ias])
+                dlh = np.zeros((n, n), dtype=complex)
+                dlo = np.zeros((n, n), dtype=complex)
+                dlh[:ngk, :ngk] = (dpij * zt1 + H[:ngk, :ngk]) * t1 * 1j
+                dlo[:ngk, :ngk] = -(zt1 + O[:ngk, :ngk]) * t1 * 1j
+                tk = vgk[l][:, None]
+                dlh[:ngk, ngk:] = H[:ngk, ngk:] * tk * 1j
+                dlh[ngk:, :ngk] = dlh[:ngk, ngk:].conj().T
+                dlo[:ngk, ngk:] = -O[:ngk, ngk:] * tk * 1j
+                dlo[ngk:, :ngk] = dlo[:ngk, ngk:].conj().T
+                zmf = dlh @ X + dlo @ X2
+                ffv = X.conj().T @ zmf
+                out[l, ias] += wkpt * np.sum(occ * np.real(np.diag(ffv)))
+            lob += los
+            ias += 1
+    return out
