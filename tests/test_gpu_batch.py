@@ -223,3 +223,42 @@ def test_batch_rejects_bad_arguments(cases):
         c.ctx.setup_batch(kp, flags=lapw.SPLIT(3))
     with pytest.raises(lapw.ExhError):
         c.ctx.batch_plan = lapw.plan_count()
+
+
+@pytest.mark.gpu
+def test_graph_replays_follow_device_made_tables(cases):
+    """A captured batch keeps replaying correctly when the next SCF
+    iteration's tables are made and installed on the device (hmlint +
+    exh_set_mt_device, gencfun/genveffig + exh_set_interstitial_device):
+    same sizes, rewritten in place, so the graph reads the new values."""
+    import dataclasses
+    from exciting_amd import synth
+    c = cases("tiny_multi")
+    t = c.torch
+    s = synth.add_mt_potential(dataclasses.replace(c.s), lmaxvr=4, seed=11)
+    dev = lambda a: t.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).cuda()
+    bufs = c.fresh()
+    c.batch(lapw.GRAPH, bufs=bufs)                  # capture with the loaded tables
+    mnlo = max(s.maxnlo, 1)
+    outs = [t.zeros(n, dtype=t.complex128, device="cuda")
+            for n in (s.maxaa * s.maxaa * s.natmtot, mnlo * s.maxaa * s.natmtot,
+                      mnlo * mnlo * s.natmtot)]
+    try:
+        c.ctx.hmlint(s.lmaxvr, dev(s.veffmt), dev(s.lofr), True, *outs)
+        c.ctx.set_mt(s.maxaa, s.maxnlo, *outs, dev(s.oalo), dev(s.ololo))
+        cg = t.zeros(s.ngrtot, dtype=t.complex128, device="cuda")
+        cr = t.zeros(s.ngrtot, dtype=t.float64, device="cuda")
+        c.ctx.gencfun(s.bvec, s.atposc, cfunig=cg, cfunir=cr)
+        vg = t.zeros(s.ngvec, dtype=t.complex128, device="cuda")
+        veffir = dev(-0.4 + 0.3 * np.random.default_rng(5).uniform(-1, 1, s.ngrtot))
+        c.ctx.genveffig_device(veffir, cr, veffig=vg)
+        c.ctx.set_interstitial(cg, vg, valence_rel_none=True)
+        eager = c.batch(0)
+        replay = c.batch(lapw.GRAPH, bufs=bufs)
+        same(replay, eager, "replay after device-made tables")
+    finally:
+        c.ctx.set_mt(c.s.maxaa, c.s.maxnlo, c.s.aa, c.s.loa, c.s.lolo, c.s.oalo, c.s.ololo,
+                     c.s.h1aa, c.s.h1loa, c.s.h1lolo)
+        c.ctx.set_interstitial(c.s.cfunig, c.s.veffig, c.s.meffig, c.s.m2effig,
+                               c.s.valence_rel_none, c.s.h1on)
+        c.ref.clear()
