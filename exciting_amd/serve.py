@@ -88,7 +88,9 @@ def context_bytes(sys_, nmod_max: int = 12) -> int:
         scratch += 7 * nmod_max * kpad * gpad    # int8 residues of A and aa.A
         scratch += 6 * nmod_max * nt * (nt + 1) // 2 * 128 * 128   # product residues
     perk = len(sys_.kpts) * sys_.ngkmax * (16 * sys_.natmtot + 8 * 7 + 4)
-    solve = 5 * 16 * n * n                       # Cholesky factor, reduced matrix, workspace
+    # Cholesky factor, the two triangular solves' results, the reduced matrix
+    # and its Hermitian average, and the eigensolver's copy and workspace
+    solve = 8 * 16 * n * n
     return apwalm + ho + scratch + perk + solve
 
 
@@ -301,20 +303,47 @@ def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
                 if d is None:
                     d = lapw.DeviceKPoint(s, ik, device=dev, with_apwalm=False)
                     ctx.kp[ik] = d
-                ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, ctx.apwalm, stream=ctx.stream)
                 nn = d.rank * d.rank
                 sysd = lapw.EvSystem(lapw.HermitianMatrix(d.rank, ctx.H[:nn]),
                                      lapw.HermitianMatrix(d.rank, ctx.O[:nn]))
-                ctx.hamiltonandoverlapsetup(sysd, d.ngk, ctx.apwalm, d.igpig, d.vgpc,
-                                            flags=lapw.SPLIT(req.nmod) if req.nmod else 0,
-                                            stream=ctx.stream)
+
+                def setup():
+                    ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, ctx.apwalm, stream=ctx.stream)
+                    ctx.hamiltonandoverlapsetup(sysd, d.ngk, ctx.apwalm, d.igpig, d.vgpc,
+                                                flags=lapw.SPLIT(req.nmod) if req.nmod else 0,
+                                                stream=ctx.stream)
+                try:
+                    setup()
+                except lapw.ExhError as e:
+                    # The library's scratch grows on first use of a larger
+                    # setup (e.g. the first split one of a context).  Blocks
+                    # PyTorch's caching allocator holds free for the
+                    # eigensolves are not visible to hipMalloc: release them
+                    # and run the setup once more (it overwrites H and O).
+                    if "out of memory" not in str(e):
+                        raise
+                    torch.cuda.synchronize(device)
+                    torch.cuda.empty_cache()
+                    setup()
                 if digests:
                     h = hashlib.sha1(ctx.H[:nn].cpu().numpy().tobytes())
                     h.update(ctx.O[:nn].cpu().numpy().tobytes())
                     dg.append(h.hexdigest())
                 with solve_slots:
-                    w = _solve.geneig(ctx.H, ctx.O, d.rank, min(req.nstfv, d.rank))
+                    try:
+                        w = _solve.geneig(ctx.H, ctx.O, d.rank, min(req.nstfv, d.rank))
+                    except torch.cuda.OutOfMemoryError:
+                        # freed temporaries of other solves may sit in
+                        # PyTorch's cache in pieces too small for this one
+                        torch.cuda.synchronize(device)
+                        torch.cuda.empty_cache()
+                        w = _solve.geneig(ctx.H, ctx.O, d.rank, min(req.nstfv, d.rank))
                     out.append(w.cpu().numpy())
+                    del w
+                    # the solve's n x n temporaries are free now; hand them back
+                    # so that they do not sit in PyTorch's cache in pieces that
+                    # the admission budget cannot see and hipMalloc cannot use
+                    torch.cuda.empty_cache()
         if digests:
             req.digests = dg
         return out
@@ -388,7 +417,9 @@ def main(argv=None) -> int:
     ap.add_argument("--max-inflight", type=int, default=8)
     ap.add_argument("--max-solves", type=int, default=4,
                     help="eigensolves allowed to run at once")
-    ap.add_argument("--reserve-gib", type=float, default=16.0)
+    ap.add_argument("--reserve-gib", type=float, default=48.0,
+                    help="HBM left outside the admission budget (library and "
+                         "allocator working room)")
     ap.add_argument("--cells", default="fcc2,sc8,ortho16,sc32,sc64")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
