@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void hml_contract_kernel(
     int nf, int lmmaxvr, int lmmaxmat, const int *__restrict__ atom_species, RowTable R,
     RowTable C, int ldo, int nco, const int *__restrict__ gstart, const int *__restrict__ glm2,
     const double2 *__restrict__ gval, const double *__restrict__ ipot,
-    const double *__restrict__ ikin, double2 *__restrict__ out, int accumulate) {
+    const double *__restrict__ ikin, double2 *__restrict__ out, int accumulate, int with_kin) {
   const int ias = blockIdx.y;
   const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (e >= (long long)ldo * nco) return;
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kThreads) void hml_contract_kernel(
   if (i < R.count[is] && j < C.count[is]) {
     const int2 ri = R.rows[is * R.ld + i], cj = C.rows[is * C.ld + j];
     const size_t base = ((size_t)ias * nf + ri.y) * nf + cj.y;
-    if (ri.x == cj.x) re += ikin[base];
+    if (with_kin && ri.x == cj.x) re += ikin[base];
     double zr = 0.0, zi = 0.0;
     const int q = ri.x * lmmaxmat + cj.x;
     const double *ip = ipot + base * lmmaxvr;
@@ -460,9 +460,10 @@ using namespace exh;
 
 extern "C" {
 
-int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *veffmt,
-                      const double *lofr, int valence_rel_none, exh_z *aa, exh_z *loa,
-                      exh_z *lolo, int flags) {
+// hmlint (mt_kin + mt_pot) or, with with_kin false, mt_pot alone.
+static int hmlint_impl(exh_context *c, void *stream, int lmaxvr, const double *veffmt,
+                       const double *lofr, int valence_rel_none, exh_z *aa, exh_z *loa,
+                       exh_z *lolo, int flags, bool with_kin) {
   if (!c) return EXH_EINVAL;
   if (int rc = wait_done(c)) return rc;
   if (!c->have_basis || !c->have_radial)
@@ -487,7 +488,7 @@ int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *ve
   const int npair = nf * (nf + 1) / 2;
   const double *lf = lofr ? lofr : (const double *)c->hml_w.p;  // never read when nlomax = 0
   hipError_t e = timed(c, EXH_K_HMLRAD, s, [&] {
-    dim3 grid(natm, nchunk + 1);
+    dim3 grid(natm, nchunk + (with_kin ? 1 : 0));
 #define LAUNCH(P)                                                                               \
   hipLaunchKernelGGL(hml_radial_kernel<P>, grid, dim3(kRadThreads), 0, s, a,                       \
                      (const int *)c->atom_species_d.p, counts + 2 * c->nspecies,                \
@@ -521,13 +522,25 @@ int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *ve
                          (const int *)c->hml_gstart.p, (const int *)c->hml_glm2.p,
                          (const double2 *)c->hml_gval.p, (const double *)c->hml_ipot.p,
                          (const double *)c->hml_ikin.p, (double2 *)P.out,
-                         (flags & EXH_ACCUMULATE) ? 1 : 0);
+                         (flags & EXH_ACCUMULATE) ? 1 : 0, with_kin ? 1 : 0);
       le = hipGetLastError();
     }
     return le;
   });
   if (e == hipSuccess) e = mark_done(c, s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "hmlint contract");
+}
+
+int exh_hmlint_device(exh_context *c, void *stream, int lmaxvr, const double *veffmt,
+                      const double *lofr, int valence_rel_none, exh_z *aa, exh_z *loa,
+                      exh_z *lolo, int flags) {
+  return hmlint_impl(c, stream, lmaxvr, veffmt, lofr, valence_rel_none, aa, loa, lolo, flags,
+                     true);
+}
+
+int exh_mt_pot_device(exh_context *c, void *stream, int lmaxvr, const double *pot,
+                      const double *lofr, exh_z *aa, exh_z *loa, exh_z *lolo, int flags) {
+  return hmlint_impl(c, stream, lmaxvr, pot, lofr, 1, aa, loa, lolo, flags, false);
 }
 
 int exh_mt_kinetic_device(exh_context *c, void *stream, int lmaxvr, const double *veffmt,
@@ -572,8 +585,9 @@ int exh_mt_kinetic_device(exh_context *c, void *stream, int lmaxvr, const double
   return e == hipSuccess ? 0 : hip_fail(c, e, "mt_kinetic");
 }
 
-int exh_hmlint(exh_context *c, int lmaxvr, const double *veffmt, const double *lofr,
-               int valence_rel_none, exh_z *aa, exh_z *loa, exh_z *lolo, int flags) {
+static int hmlint_host(exh_context *c, int lmaxvr, const double *veffmt, const double *lofr,
+                       int valence_rel_none, exh_z *aa, exh_z *loa, exh_z *lolo, int flags,
+                       bool with_kin) {
   if (!c) return EXH_EINVAL;
   if (!c->have_basis || !c->have_radial)
     return fail(c, EXH_ESTATE, "hmlint: basis and radial data not set");
@@ -599,15 +613,24 @@ int exh_hmlint(exh_context *c, int lmaxvr, const double *veffmt, const double *l
                : ensure(c, c->hml_lolo, nlolo * sizeof(double2));
   }
   if (rc) return rc;
-  rc = exh_hmlint_device(c, nullptr, lmaxvr, (const double *)c->hml_vin.p,
-                         c->nlomax > 0 ? (const double *)c->hml_loin.p : nullptr,
-                         valence_rel_none, (exh_z *)c->hml_aa.p,
-                         c->nlomax > 0 ? (exh_z *)c->hml_loa.p : nullptr,
-                         c->nlomax > 0 ? (exh_z *)c->hml_lolo.p : nullptr, flags);
+  rc = hmlint_impl(c, nullptr, lmaxvr, (const double *)c->hml_vin.p,
+                   c->nlomax > 0 ? (const double *)c->hml_loin.p : nullptr, valence_rel_none,
+                   (exh_z *)c->hml_aa.p, c->nlomax > 0 ? (exh_z *)c->hml_loa.p : nullptr,
+                   c->nlomax > 0 ? (exh_z *)c->hml_lolo.p : nullptr, flags, with_kin);
   if (!rc) rc = copy_out(c, aa, c->hml_aa, naa * sizeof(double2));
   if (!rc && c->nlomax > 0) rc = copy_out(c, loa, c->hml_loa, nloa * sizeof(double2));
   if (!rc && c->nlomax > 0) rc = copy_out(c, lolo, c->hml_lolo, nlolo * sizeof(double2));
   return rc;
+}
+
+int exh_hmlint(exh_context *c, int lmaxvr, const double *veffmt, const double *lofr,
+               int valence_rel_none, exh_z *aa, exh_z *loa, exh_z *lolo, int flags) {
+  return hmlint_host(c, lmaxvr, veffmt, lofr, valence_rel_none, aa, loa, lolo, flags, true);
+}
+
+int exh_mt_pot(exh_context *c, int lmaxvr, const double *pot, const double *lofr, exh_z *aa,
+               exh_z *loa, exh_z *lolo, int flags) {
+  return hmlint_host(c, lmaxvr, pot, lofr, 1, aa, loa, lolo, flags, false);
 }
 
 int exh_set_mt_device(exh_context *c, void *stream, int maxaa, int maxnlo, const exh_z *aa,

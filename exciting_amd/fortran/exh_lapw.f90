@@ -22,6 +22,9 @@
 !                                          (src/src_hybrids/calc_vnlmat.f90:52-53)
 !   exh_hmlint                   replaces  hmlint's spinless part
 !                                          (src/src_eigensystem/hmlint.f90:53-55)
+!   exh_mt_pot                   replaces  mt_pot (src/src_eigensystem/mt_pot.f90:14),
+!                                          as the spin-polarised blocks of hmlint
+!                                          call it (hmlint.f90:58-173)
 module exh_lapw
   use, intrinsic :: iso_c_binding
   implicit none
@@ -50,7 +53,8 @@ module exh_lapw
             exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace, &
             exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
-            exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device
+            exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device, &
+            exh_mt_pot, exh_mt_pot_device
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -199,6 +203,22 @@ module exh_lapw
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, stream, veffmt, lofr, aa, loa, lolo
       integer(c_int), value :: lmaxvr, valence_rel_none, flags
+    end function
+
+    integer(c_int) function exh_mt_pot(ctx, lmaxvr, pot, lofr, aa, loa, lolo, flags) &
+        bind(C, name="exh_mt_pot")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lmaxvr, flags
+      real(c_double), intent(in) :: pot(*), lofr(*)
+      complex(c_double_complex), intent(inout) :: aa(*), loa(*), lolo(*)
+    end function
+
+    integer(c_int) function exh_mt_pot_device(ctx, stream, lmaxvr, pot, lofr, aa, loa, lolo, &
+        flags) bind(C, name="exh_mt_pot_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, pot, lofr, aa, loa, lolo
+      integer(c_int), value :: lmaxvr, flags
     end function
 
     ! The kinetic radial integrals alone (mt_kin, kinetic_energy): tkin(nf,nf,natmtot).

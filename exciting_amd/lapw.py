@@ -35,6 +35,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libexh.so")
 
 ACCUMULATE = 1
+Y00 = 0.28209479177387814       # 1 / sqrt(4 pi)
+# ge alpha / 4 of hmlint.f90:45, with physical_constants.f90:53,56
+GA4 = 2.00231930436256 * (1.0 / 137.035999084) / 4.0
 
 _p = C.c_void_p
 _i = C.c_int
@@ -95,6 +98,8 @@ def lib():
     L.exh_overlapsetup.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
     L.exh_overlapsetup_device.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
     L.exh_mt_kinetic_device.argtypes = [_p, _p, _i, _p, _p, _i, _p]
+    L.exh_mt_pot.argtypes = [_p, _i, _p, _p, _p, _p, _p, _i]
+    L.exh_mt_pot_device.argtypes = [_p, _p, _i, _p, _p, _p, _p, _p, _i]
     _lib = L
     return L
 
@@ -116,7 +121,7 @@ EXPORTS = (
     "exh_set_interstitial_device", "exh_gencfun", "exh_gencfun_device", "exh_genveffig",
     "exh_genmeffig", "exh_genveffig_device", "exh_set_mt_device", "exh_hmlint",
     "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
-    "exh_mt_kinetic_device",
+    "exh_mt_kinetic_device", "exh_mt_pot", "exh_mt_pot_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
@@ -348,11 +353,80 @@ class Context:
         outputs, Fortran layouts flattened) the device entry point fills the
         outputs; otherwise NumPy arrays are returned (accumulated into the
         given ones with flags=ACCUMULATE)."""
+        return self._mt_tables(True, lmaxvr, veffmt, lofr, valence_rel_none, aa, loa, lolo,
+                               flags, stream)
+
+    def mt_pot(self, lmaxvr, pot, lofr, aa=None, loa=None, lolo=None, flags: int = 0,
+               stream=None):
+        """mt_pot.f90:14-323 alone: the same tables from any real muffin-tin
+        function pot (lmmaxvr, nrmtmax, natmtot), without the kinetic part
+        (the spin-polarised blocks of hmlint.f90:58-173).  Same calling
+        convention as hmlint."""
+        return self._mt_tables(False, lmaxvr, pot, lofr, True, aa, loa, lolo, flags, stream)
+
+    def hmlint_spin(self, lmaxvr, bxcmt, lofr, ncmag=False, bext=None):
+        """The spin-polarised blocks of hmlint (hmlint.f90:58-173, without
+        mt_so) from CUDA float64 tensors: bxcmt(lmmaxvr, nrmtmax, natmtot,
+        ndmag) flattened in Fortran order (ndmag 1 collinear, 3 non-collinear),
+        lofr as for hmlint, and bext (natmtot, 3) = bfcmt of the atom +
+        bfieldc, the external fields added to the l = 0 channel as
+        ga4 b / y00 (ga4 = ge alpha / 4).  Returns {"alpha", "beta"[, "ab"]},
+        each a tuple (aa, loa, lolo) of CUDA complex128 tensors with the
+        shapes of the spinless tables:
+          collinear:      alpha = T(bxc_1 + f_z), beta = -alpha;
+          non-collinear:  alpha = T(bxc_3 + f_z), beta = -alpha,
+                          ab = -i T(bxc_2 + f_y) + T(bxc_1 + f_x),
+        T being mt_pot.  bxcmt is not modified."""
+        import torch
+        na, nr = sum(self.natoms), self.sys.nrmtmax
+        lmm = (lmaxvr + 1) ** 2
+        ndmag = 3 if ncmag else 1
+        plane = lmm * nr * na
+        if not _is_torch(bxcmt) or bxcmt.numel() != plane * ndmag \
+                or bxcmt.dtype != torch.float64:
+            raise ExhError(f"bxcmt must be a CUDA float64 tensor of {plane * ndmag} elements "
+                           f"(lmmaxvr {lmm}, nrmtmax {nr}, natmtot {na}, ndmag {ndmag})")
+        b = np.zeros((na, 3)) if bext is None else np.asarray(bext, dtype=np.float64)
+        if b.shape != (na, 3):
+            raise ExhError(f"bext must be ({na}, 3)")
+        mnlo = max(self.maxnlo, 1)
+        sizes = (self.maxaa * self.maxaa * na, mnlo * self.maxaa * na, mnlo * mnlo * na)
+        dev = bxcmt.device
+
+        def component(c, axis):
+            v = bxcmt[c * plane:(c + 1) * plane].clone()
+            shift = torch.from_numpy(GA4 * b[:, axis] / Y00).to(dev)
+            v.view(na, nr, lmm)[:, :, 0] += shift[:, None]
+            return v
+
+        def tables():
+            return tuple(torch.zeros(n, dtype=torch.complex128, device=dev) for n in sizes)
+
+        alpha = tables()
+        self.mt_pot(lmaxvr, component(2 if ncmag else 0, 2), lofr, *alpha)
+        out = {"alpha": alpha, "beta": tuple(-t for t in alpha)}
+        if ncmag:
+            ab = tables()
+            self.mt_pot(lmaxvr, component(1, 1), lofr, *ab)
+            for t in ab:
+                t.mul_(-1j)
+            self.mt_pot(lmaxvr, component(0, 0), lofr, *ab, flags=ACCUMULATE)
+            out["ab"] = ab
+        return out
+
+    def _mt_tables(self, kin, lmaxvr, veffmt, lofr, valence_rel_none, aa, loa, lolo, flags,
+                   stream):
+        name = "exh_hmlint" if kin else "exh_mt_pot"
         if _is_torch(veffmt):
-            rc = self._device_call(stream, lambda st: lib().exh_hmlint_device(
-                self._h, st, lmaxvr, _ptr(veffmt), _ptr(lofr), 1 if valence_rel_none else 0,
-                _ptr(aa), _ptr(loa), _ptr(lolo), flags))
-            self._check(rc, "exh_hmlint_device")
+            if kin:
+                fn = lambda st: lib().exh_hmlint_device(
+                    self._h, st, lmaxvr, _ptr(veffmt), _ptr(lofr),
+                    1 if valence_rel_none else 0, _ptr(aa), _ptr(loa), _ptr(lolo), flags)
+            else:
+                fn = lambda st: lib().exh_mt_pot_device(
+                    self._h, st, lmaxvr, _ptr(veffmt), _ptr(lofr), _ptr(aa), _ptr(loa),
+                    _ptr(lolo), flags)
+            self._check(self._device_call(stream, fn), name + "_device")
             return aa, loa, lolo
         v = _host(veffmt, np.float64)
         lf = _host(lofr, np.float64) if lofr is not None else None
@@ -367,9 +441,13 @@ class Context:
                       and x.dtype == np.complex128):
                 raise ExhError("hmlint tables must be Fortran-ordered complex128 arrays")
             outs.append(x)
-        self._check(lib().exh_hmlint(self._h, lmaxvr, _ptr(v), _ptr(lf),
-                                     1 if valence_rel_none else 0, *[_ptr(x) for x in outs],
-                                     flags), "exh_hmlint")
+        if kin:
+            rc = lib().exh_hmlint(self._h, lmaxvr, _ptr(v), _ptr(lf),
+                                  1 if valence_rel_none else 0, *[_ptr(x) for x in outs], flags)
+        else:
+            rc = lib().exh_mt_pot(self._h, lmaxvr, _ptr(v), _ptr(lf),
+                                  *[_ptr(x) for x in outs], flags)
+        self._check(rc, name)
         return tuple(outs)
 
     def mt_kinetic(self, lofr, mass_flags=0, veffmt=None, lmaxvr=0, stream=None):

@@ -158,6 +158,15 @@ int exh_hmlint(exh_context *ctx, int lmaxvr, const double *veffmt, const double 
 int exh_hmlint_device(exh_context *ctx, void *stream, int lmaxvr, const double *veffmt,
                       const double *lofr, int valence_rel_none, exh_z *aa, exh_z *loa,
                       exh_z *lolo, int flags);
+/* mt_pot.f90:14-323 alone, for any real muffin-tin function pot(lmmaxvr,
+ * nrmtmax, natmtot): the spin-polarised part of hmlint (hmlint.f90:58-173)
+ * calls it with the B_xc components (plus the external fields) into the
+ * alpha / ab tables, and sets beta = -alpha and ab = -i ab itself.  Same
+ * tables, flags and limits as exh_hmlint. */
+int exh_mt_pot(exh_context *ctx, int lmaxvr, const double *pot, const double *lofr,
+               exh_z *aa, exh_z *loa, exh_z *lolo, int flags);
+int exh_mt_pot_device(exh_context *ctx, void *stream, int lmaxvr, const double *pot,
+                      const double *lofr, exh_z *aa, exh_z *loa, exh_z *lolo, int flags);
 
 /* The kinetic radial integrals alone, as mt_kin.f90:117-170 and
  * kinetic_energy.f90:50-183 (t_aa, t_alo, t_lolo) take them: for the radial

@@ -81,6 +81,7 @@ def lib():
         L.orc_gaunt_yyy.restype = _d
         L.orc_gaunt_yry.argtypes = [_i] * 6 + [_p]
         L.orc_hmlint.argtypes = [_p, _i, _p, _p, _i, _p, _p, _p]
+        L.orc_hmlint_parts.argtypes = [_p, _i, _p, _p, _i, _p, _p, _p, _i]
         L.orc_nlotot.argtypes = [_p]
         L.orc_maxapw.argtypes = [_p]
         _lib = L
@@ -341,9 +342,10 @@ def gaunt_yry(l1, l2, l3, m1, m2, m3) -> complex:
     return complex(out[0], out[1])
 
 
-def hmlint(sys_, lmaxvr=None, veffmt=None, lofr=None, valence_rel_none=None):
+def hmlint(sys_, lmaxvr=None, veffmt=None, lofr=None, valence_rel_none=None, parts=3):
     """mt_kin + mt_pot (hmlint.f90:53-55) from zero tables: (aa, loa, lolo)
-    with the shapes of mt_hscf%main."""
+    with the shapes of mt_hscf%main.  parts: 1 mt_kin only, 2 mt_pot only
+    (with veffmt as the potential), 3 both."""
     s = sys_.s
     lmaxvr = s.lmaxvr if lmaxvr is None else lmaxvr
     veffmt = s.veffmt if veffmt is None else veffmt
@@ -355,8 +357,9 @@ def hmlint(sys_, lmaxvr=None, veffmt=None, lofr=None, valence_rel_none=None):
     lolo = np.zeros((mnlo, mnlo, s.natmtot), dtype=np.complex128, order="F")
     v = np.asfortranarray(veffmt, dtype=np.float64)
     lf = np.asfortranarray(lofr, dtype=np.float64)
-    rc = lib().orc_hmlint(sys_.ptr, lmaxvr, v.ctypes.data, lf.ctypes.data, 1 if relnone else 0,
-                          aa.ctypes.data, loa.ctypes.data, lolo.ctypes.data)
+    rc = lib().orc_hmlint_parts(sys_.ptr, lmaxvr, v.ctypes.data, lf.ctypes.data,
+                                1 if relnone else 0, aa.ctypes.data, loa.ctypes.data,
+                                lolo.ctypes.data, parts)
     if rc != 0:
         raise RuntimeError(f"orc_hmlint failed: {rc}")
     return aa, loa, lolo

@@ -1099,6 +1099,13 @@ void orc_gaunt_yry(int l1, int l2, int l3, int m1, int m2, int m3, double *out) 
 /* ------------------------------------------------------------------ */
 int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const double *lofr,
                int valence_rel_none, orc_z *aa, orc_z *loa, orc_z *lolo) {
+  return orc_hmlint_parts(s, lmaxvr, veffmt, lofr, valence_rel_none, aa, loa, lolo, 3);
+}
+
+/* parts: 1 = mt_kin, 2 = mt_pot (with pot = veffmt), 3 = both (hmlint). */
+int orc_hmlint_parts(const orc_system *s, int lmaxvr, const double *veffmt, const double *lofr,
+                     int valence_rel_none, orc_z *aa, orc_z *loa, orc_z *lolo, int parts) {
+  const int want_kin = parts & 1, want_pot = parts & 2;
   const double y00 = 0.28209479177387814347; /* constants.f90:90 */
   const double alpha = 1.0 / 137.03599911;
   const int lmmaxvr = (lmaxvr + 1) * (lmaxvr + 1);
@@ -1207,7 +1214,7 @@ int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const doub
                 for (int io2 = 0; io2 < apword_of(s, l3, is); io2++, if3++) {
                   const int fa = l1 * p + io1, fb = l3 * p + io2;
                   orc_z *e = aa + if1 + (size_t)maxaa * (if3 + (size_t)maxaa * ias);
-                  if (l1 == l3 && m1 == m3) /* kinetic: haaintegrals(1,io2,l1,io1,l1)*y00 */
+                  if (want_kin && l1 == l3 && m1 == m3) /* kinetic: haaintegrals(1,io2,l1,io1,l1)*y00 */
                     *e = zadd(*e, zmk(Ikin[fb * nf + fa] * y00, 0.0));
                 }
           }
@@ -1225,7 +1232,7 @@ int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const doub
                   for (int t = start[q]; t < endp[q]; t++) /* haaintegrals(lm2,io2,l3,io1,l1) */
                     zsum = zadd(zsum, zmul(gval[t], zmk(Ipot[((size_t)fb * nf + fa) * lmmaxvr + glm2[t]], 0.0)));
                   orc_z *e = aa + if1 + (size_t)maxaa * (if3 + (size_t)maxaa * ias);
-                  *e = zadd(*e, zsum);
+                  if (want_pot) *e = zadd(*e, zsum);
                 }
           }
       /* loa (mt_kin.f90:189-206, mt_pot.f90:260-287) and lolo (mt_kin.f90:208-226,
@@ -1241,7 +1248,7 @@ int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const doub
                 for (int io = 0; io < apword_of(s, l3, is); io++, if3++) {
                   const int fl = nfa + ilo, fb = l3 * p + io;
                   orc_z *e = loa + r1 + (size_t)mnlo * (if3 + (size_t)maxaa * ias);
-                  if (idxlm(l1, m1) == idxlm(l3, m3))
+                  if (want_kin && idxlm(l1, m1) == idxlm(l3, m3))
                     *e = zadd(*e, zmk(Ikin[fb * nf + fl] * y00, 0.0));
                 }
           }
@@ -1260,7 +1267,7 @@ int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const doub
                   for (int t = start[q]; t < endp[q]; t++) /* halointegrals(lm2,io,l3,ilo) */
                     zsum = zadd(zsum, zmul(gval[t], zmk(Ipot[((size_t)fl * nf + fb) * lmmaxvr + glm2[t]], 0.0)));
                   orc_z *e = loa + r1 + (size_t)mnlo * (if3 + (size_t)maxaa * ias);
-                  *e = zadd(*e, zsum);
+                  if (want_pot) *e = zadd(*e, zsum);
                 }
           }
         }
@@ -1273,7 +1280,7 @@ int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const doub
               const int l3 = lorbl_of(s, ilo2, is);
               for (int m3 = -l3; m3 <= l3; m3++, r3++) {
                 orc_z *e = lolo + r1 + (size_t)mnlo * (r3 + (size_t)mnlo * ias);
-                if (idxlm(l1, m1) == idxlm(l3, m3))
+                if (want_kin && idxlm(l1, m1) == idxlm(l3, m3))
                   *e = zadd(*e, zmk(Ikin[(nfa + ilo1) * nf + nfa + ilo2] * y00, 0.0));
               }
             }
@@ -1292,7 +1299,7 @@ int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const doub
                 for (int t = start[q]; t < endp[q]; t++) /* hlolointegrals(lm2,ilo1,ilo2) */
                   zsum = zadd(zsum, zmul(gval[t], zmk(Ipot[((size_t)(nfa + ilo1) * nf + nfa + ilo2) * lmmaxvr + glm2[t]], 0.0)));
                 orc_z *e = lolo + r1 + (size_t)mnlo * (r3 + (size_t)mnlo * ias);
-                *e = zadd(*e, zsum);
+                if (want_pot) *e = zadd(*e, zsum);
               }
             }
           }

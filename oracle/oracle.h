@@ -160,6 +160,11 @@ void orc_gaunt_yry(int l1, int l2, int l3, int m1, int m2, int m3, double *out);
  * list is init1.f90:563-609. */
 int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const double *lofr,
                int valence_rel_none, orc_z *aa, orc_z *loa, orc_z *lolo);
+/* parts: 1 = mt_kin only, 2 = mt_pot only (the potential veffmt is any real
+ * muffin-tin function, e.g. a B_xc component as hmlint.f90:58-173 passes),
+ * 3 = both. */
+int orc_hmlint_parts(const orc_system *s, int lmaxvr, const double *veffmt, const double *lofr,
+                     int valence_rel_none, orc_z *aa, orc_z *loa, orc_z *lolo, int parts);
 
 /* Planted-mistake switch for tests: 0 = faithful.  Nonzero values inject one
  * deliberate error so tests can show the checks catch it. */

@@ -124,6 +124,92 @@ def test_setup_from_device_made_tables(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name,lmaxvr", [("tiny_zora", 6), ("tiny_nolo", 4), ("sc8_k0", 8)])
+def test_mt_pot_matches_the_oracle(name, lmaxvr):
+    """mt_pot alone on a function other than veffmt (the B_xc components of
+    the spin-polarised blocks): host and device entry points against the
+    oracle's parts=2, and hmlint = mt_kin (oracle parts=1) + mt_pot(veffmt)."""
+    import torch
+    s = with_potential(name, lmaxvr)
+    bxc = synth.add_mt_potential(dataclasses.replace(_cases.system(name)), lmaxvr=lmaxvr,
+                                 seed=33).veffmt
+    print(f"[gpu-mt_pot] case {name} lmaxvr {lmaxvr} natmtot {s.natmtot}", flush=True)
+    S = oracle.System(s)
+    want = oracle.hmlint(S, veffmt=bxc, parts=2)
+    kin = oracle.hmlint(S, parts=1)
+    ctx = lapw.Context(0).load(s)
+    try:
+        got = ctx.mt_pot(lmaxvr, bxc, s.lofr)
+        for g, w, what in zip(got, want, ("aa", "loa", "lolo")):
+            e = err(g, w)
+            print(f"  {what}: {e:.1e}")
+            assert e <= TOL, what
+        outs = [torch.zeros(w.size, dtype=torch.complex128, device="cuda") for w in want]
+        ctx.mt_pot(lmaxvr, dev(bxc), dev(s.lofr), *outs)
+        torch.cuda.synchronize()
+        for t, g in zip(outs, got):
+            assert np.array_equal(host(t, g), g)
+        # hmlint minus mt_pot(veffmt) is the kinetic part
+        full = ctx.hmlint(lmaxvr, s.veffmt, s.lofr, s.valence_rel_none)
+        pot_v = ctx.mt_pot(lmaxvr, s.veffmt, s.lofr)
+        for f, p, k, what in zip(full, pot_v, kin, ("aa", "loa", "lolo")):
+            assert np.abs(f - p - k).max() <= TOL * max(np.abs(f).max(), 1e-300), what
+        # accumulate onto existing tables
+        acc = ctx.mt_pot(lmaxvr, bxc, s.lofr, *[x.copy(order="F") for x in full],
+                         flags=lapw.ACCUMULATE)
+        for a_, f, g in zip(acc, full, got):
+            assert np.abs(a_ - (f + g)).max() <= 1e-14 * max(1.0, np.abs(f).max())
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,ncmag", [("tiny", False), ("tiny", True), ("sc8_k0", True)])
+def test_hmlint_spin_blocks(name, ncmag):
+    """hmlint.f90:58-173: alpha, beta (and ab) from B_xc plus the external
+    fields, against the oracle's mt_pot on the shifted components composed
+    as the reference does (sign of beta, -i on the y part, x accumulated)."""
+    import torch
+    lmaxvr = 4 if name == "tiny" else 8
+    s = with_potential(name, lmaxvr)
+    ndmag = 3 if ncmag else 1
+    comps = [0.05 * synth.add_mt_potential(dataclasses.replace(_cases.system(name)),
+                                           lmaxvr=lmaxvr, seed=40 + c).veffmt
+             for c in range(ndmag)]
+    rng = np.random.default_rng(3)
+    bext = rng.uniform(-0.2, 0.2, (s.natmtot, 3))
+    print(f"[gpu-hmlint-spin] case {name} ncmag {ncmag}", flush=True)
+    S = oracle.System(s)
+
+    def T(c, axis):
+        v = comps[c].copy(order="F")
+        v[0, :, :] += lapw.GA4 * bext[:, axis][None, :] / lapw.Y00
+        return oracle.hmlint(S, veffmt=v, parts=2)
+
+    want = {"alpha": T(2 if ncmag else 0, 2)}
+    want["beta"] = tuple(-x for x in want["alpha"])
+    if ncmag:
+        want["ab"] = tuple(-1j * y + x for y, x in zip(T(1, 1), T(0, 0)))
+    flat = np.concatenate([c.ravel(order="F") for c in comps])
+    bx = torch.from_numpy(flat).cuda()
+    ctx = lapw.Context(0).load(s)
+    try:
+        got = ctx.hmlint_spin(lmaxvr, bx, dev(s.lofr), ncmag=ncmag, bext=bext)
+        torch.cuda.synchronize()
+        assert sorted(got) == sorted(want)
+        for key in want:
+            for g, w, what in zip(got[key], want[key], ("aa", "loa", "lolo")):
+                e = err(host(g, w), w)
+                print(f"  {key} {what}: {e:.1e}")
+                assert e <= TOL, (key, what)
+        assert np.array_equal(bx.cpu().numpy(), flat)         # input untouched
+        with pytest.raises(lapw.ExhError, match="bxcmt"):
+            ctx.hmlint_spin(lmaxvr, bx[:-1], dev(s.lofr), ncmag=ncmag)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
 def test_hmlint_rejects_bad_arguments():
     s = with_potential("tiny", 4)
     ctx = lapw.Context(0).load(s)

@@ -8,8 +8,11 @@ mt_pot.f90, Gaunt list of init1.f90:563-609) pinned without itself:
 * the whole table set against an independent NumPy statement: dense Gaunt
   tensor from that quadrature, radial integrals as weighted sums, the
   reference's row layouts built separately, contraction by einsum;
-* a spherical potential gives tables diagonal in (l, m) (closed form).
+* a spherical potential gives tables diagonal in (l, m) (closed form);
+* mt_pot alone and mt_kin alone (parts 2 and 1) match the NumPy
+  statement with the other term off and add up to the whole.
 """
+import dataclasses
 import os
 
 import numpy as np
@@ -126,8 +129,9 @@ def spline_weights(x):
     return np.array([oracle.fderiv_int(x, np.eye(n)[j])[-1] for j in range(n)])
 
 
-def numpy_hmlint(s):
-    """Independent NumPy statement of mt_kin + mt_pot for every atom."""
+def numpy_hmlint(s, parts=3):
+    """Independent NumPy statement of mt_kin (parts & 1) + mt_pot (parts & 2)
+    for every atom."""
     lmaxmat, p, lmaxvr = s.lmaxmat, s.apwordmax, s.lmaxvr
     lmm = (lmaxmat + 1) ** 2
     Q = gaunt_by_quadrature(lmaxmat, lmaxvr)              # [lm1, lm2, lm3]
@@ -165,8 +169,8 @@ def numpy_hmlint(s):
             def block(rows, cols, out, rowfun, colfun):
                 for i, (lm1, f1, d1, l1) in enumerate(rows):
                     for j, (lm3, f3, d3, l3) in enumerate(cols):
-                        v = Q[lm1, :, lm3] @ pot(f1, f3)
-                        if lm1 == lm3:
+                        v = Q[lm1, :, lm3] @ pot(f1, f3) if parts & 2 else 0.0
+                        if parts & 1 and lm1 == lm3:
                             v += kin(f1, f3, d1, d3, l1)
                         out[i, j] += v
 
@@ -210,3 +214,29 @@ def test_spherical_potential_gives_lm_diagonal_tables():
     off = lm[:, None] != lm[None, :]
     assert np.abs(aa[:len(lm), :len(lm)][off]).max() == 0.0
     assert np.abs(aa).max() > 0.1
+
+
+@pytest.mark.parametrize("relnone", [True, False])
+def test_hmlint_parts_split_and_match_numpy_statement(relnone):
+    """mt_pot alone (parts=2) on a non-spherical function other than veffmt,
+    as the spin-polarised blocks call it, and mt_kin alone (parts=1): each
+    matches the NumPy statement, and the two add up to parts=3."""
+    s = synth.add_mt_potential(synth.build(synth.tiny(seed=9)), lmaxvr=4)
+    s.valence_rel_none = relnone
+    sy = oracle.System(s)
+    kin = oracle.hmlint(sy, parts=1)
+    both = oracle.hmlint(sy, parts=3)
+    bxc = synth.add_mt_potential(synth.build(synth.tiny(seed=9)), lmaxvr=4, seed=33).veffmt
+    pot = oracle.hmlint(sy, veffmt=bxc, parts=2)
+    s_b = dataclasses.replace(s, veffmt=bxc)
+    for g, w_ in zip(pot, numpy_hmlint(s_b, parts=2)):
+        assert np.abs(g - w_).max() <= 1e-12 * np.abs(w_).max()
+    for g, w_ in zip(kin, numpy_hmlint(s, parts=1)):
+        assert np.abs(g - w_).max() <= 1e-12 * np.abs(w_).max()
+    pot_v = oracle.hmlint(sy, parts=2)
+    for k_, p_, b_ in zip(kin, pot_v, both):
+        assert np.abs(k_ + p_ - b_).max() <= 1e-13 * np.abs(b_).max()
+    # mt_pot does not see the relativity flag
+    other = oracle.hmlint(sy, veffmt=bxc, valence_rel_none=not relnone, parts=2)
+    for a_, b_ in zip(pot, other):
+        assert np.array_equal(a_, b_)
