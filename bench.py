@@ -427,8 +427,23 @@ def hmlint_sample(ctx, s, lmaxvr=8, reps=10, cpu=True):
         run()
         ctx.set_mt(s.maxaa, s.maxnlo, *outs, oalo, ololo)
 
+    # spin: the non-collinear alpha/beta/ab blocks (three mt_pot on B_xc
+    # components) and the spin-orbit terms added to them
+    bx = dev(np.concatenate([0.05 * sp.veffmt.ravel(order="F")] * 3))
+    t4 = (s.maxaa * s.maxaa * s.natmtot, mnlo * s.maxaa * s.natmtot, mnlo * mnlo * s.natmtot,
+          s.maxaa * mnlo * s.natmtot)
+    so_tables = {k: tuple(torch.zeros(n, dtype=torch.complex128, device="cuda") for n in t4)
+                 for k in ("alpha", "beta", "ab")}
+
+    def run_spin():
+        ctx.hmlint_spin(lmaxvr, bx, lf, ncmag=True)
+
+    def run_so():
+        ctx.mt_so(lmaxvr, v, lf, so_tables)
+
     res = {}
-    for label, fn in (("hmlint", run), ("hmlint_and_set_mt_device", run_install)):
+    for label, fn in (("hmlint", run), ("hmlint_and_set_mt_device", run_install),
+                      ("spin_blocks_noncollinear", run_spin), ("mt_so", run_so)):
         fn()
         torch.cuda.synchronize()
         ctx.profile(True)
@@ -472,6 +487,12 @@ def hmlint_sample(ctx, s, lmaxvr=8, reps=10, cpu=True):
                                "cores": 1, "kind": "port",
                                "sample": (f"oracle hmlint (C restatement of mt_kin + mt_pot, "
                                           f"one thread) on all {s.natmtot} atoms")}
+        t0 = time.perf_counter()
+        oracle.mt_so(oracle.System(sp), pot=sp.veffmt)
+        t = time.perf_counter() - t0
+        res["mt_so_cpu_baseline"] = {"value": 1e3 * t, "unit": "ms per mt_so of the cell",
+                                     "cores": 1, "kind": "port",
+                                     "sample": f"oracle mt_so (C, one thread) on all {s.natmtot} atoms"}
     return res
 
 

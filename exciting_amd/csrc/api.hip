@@ -288,7 +288,8 @@ void exh_destroy(exh_context *c) {
                     &c->i_in1, &c->i_out0, &c->i_out1, &c->i_out2, &c->apwfr_d, &c->hml_fl,
                     &c->hml_arow, &c->hml_lrow, &c->hml_count, &c->hml_gstart, &c->hml_glm2,
                     &c->hml_gval, &c->hml_w, &c->hml_spr, &c->hml_ipot, &c->hml_ikin,
-                    &c->hml_vin, &c->hml_loin, &c->hml_aa, &c->hml_loa, &c->hml_lolo};
+                    &c->hml_vin, &c->hml_loin, &c->hml_aa, &c->hml_loa, &c->hml_lolo,
+                    &c->so_dt, &c->so_rowb, &c->so_pairs, &c->so_pstart, &c->so_host};
   for (DevBuf *b : bufs) release(*b);
   for (auto &r : c->recs) {
     (void)hipEventDestroy(r.a);

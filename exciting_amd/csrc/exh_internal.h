@@ -127,6 +127,8 @@ struct exh_context {
   exh::DevBuf apwfr_d, hml_fl, hml_arow, hml_lrow, hml_count, hml_gstart, hml_glm2, hml_gval;
   exh::DevBuf hml_w, hml_spr, hml_ipot, hml_ikin, hml_vin, hml_loin, hml_aa, hml_loa, hml_lolo;
   int hml_lmaxvr = -1;
+  exh::DevBuf so_dt, so_rowb, so_pairs, so_pstart, so_host;   // mt_so
+  long long so_radial_epoch = -1, so_basis_epoch = -1;
   long long radial_epoch = 0, basis_epoch = 0, hml_radial_epoch = -1, hml_basis_epoch = -1;
 
   // instrumentation: (kernel id, start, stop) per recorded launch

@@ -232,50 +232,60 @@ __global__ __launch_bounds__(kThreads) void hml_contract_kernel(
 
 // ---- host-side tables --------------------------------------------------
 
+// Cubic spline coefficients cf(3, n) of f on the grid x, with the steps of
+// spline.f90 (natural-like end conditions, cf(1,n) and cf(3,n) extrapolated
+// from the last interval): the same rule has to give the same numbers.  iw
+// holds n scratch values.
+void spline_coefficients(int n, const double *x, const double *f, double *cf, double *iw) {
+  for (int i = 0; i + 1 < n; i++) {
+    iw[i] = 1.0 / (x[i + 1] - x[i]);
+    cf[3 * i] = iw[i] * (f[i + 1] - f[i]);
+  }
+  cf[1] = 1.0;
+  cf[2] = (cf[3] - cf[0]) / (x[2] - x[0]);
+  double t1 = (x[1] - x[0]) * iw[1];
+  double t2 = t1 * cf[1];
+  double t3 = 1.0 / (2.0 * (t1 + 1.0));
+  cf[3 + 1] = t3;
+  double t4 = 3.0 * (cf[3] - cf[0]) * iw[1] - t2 * cf[2];
+  cf[3 + 2] = t4;
+  for (int i = 2; i + 1 < n; i++) {
+    t1 = (x[i] - x[i - 1]) * iw[i];
+    t2 = t1 * t3;
+    t3 = 1.0 / (2.0 * t1 + 2.0 - t2);
+    cf[3 * i + 1] = t3;
+    t4 = 3.0 * (cf[3 * i] - cf[3 * (i - 1)]) * iw[i] - t2 * t4;
+    cf[3 * i + 2] = t4;
+  }
+  cf[3 * (n - 1) + 1] = 1.0;
+  t1 = (cf[3 * (n - 2)] - cf[3 * (n - 3)]) / (x[n - 1] - x[n - 3]);
+  cf[3 * (n - 1) + 2] = t1;
+  for (int i = n - 2; i >= 1; i--) {
+    t1 = cf[3 * i + 2] - t1 * cf[3 * (i + 1) + 1];
+    cf[3 * i + 2] = t1;
+  }
+  for (int i = 0; i < n; i++) cf[3 * i + 1] *= cf[3 * i + 2];
+  for (int i = 0; i + 1 < n; i++) {
+    t1 = (1.0 / 3.0) * (cf[3 * (i + 1) + 1] - cf[3 * i + 1]) * iw[i];
+    cf[3 * i + 2] = t1;
+    t2 = x[i + 1] - x[i];
+    cf[3 * i] = cf[3 * i] - (cf[3 * i + 1] + t1 * t2) * t2;
+  }
+  t1 = x[n - 1] - x[n - 2];
+  cf[3 * (n - 1)] = cf[3 * (n - 2)] + (2.0 * cf[3 * (n - 2) + 1] + 3.0 * cf[3 * (n - 2) + 2] * t1) * t1;
+  cf[3 * (n - 1) + 2] = cf[3 * (n - 2) + 2];
+}
+
 // Weights w with sum_i w_i f_i equal to the integral over the whole grid that
 // fderiv.f90 (m = -1) returns: its spline (spline.f90) and the per-interval
 // cubic integral are linear in f, so w_j is that rule applied to the j-th
-// unit vector.  The steps are the reference's (the rule has to be the same
-// for the same numbers).
+// unit vector.
 void spline_integral_weights(int n, const double *x, double *w) {
   std::vector<double> cf(3 * (size_t)n), iw(n), f(n);
   for (int j = 0; j < n; j++) {
     std::fill(f.begin(), f.end(), 0.0);
     f[j] = 1.0;
-    for (int i = 0; i + 1 < n; i++) {
-      iw[i] = 1.0 / (x[i + 1] - x[i]);
-      cf[3 * i] = iw[i] * (f[i + 1] - f[i]);
-    }
-    cf[1] = 1.0;
-    cf[2] = (cf[3] - cf[0]) / (x[2] - x[0]);
-    double t1 = (x[1] - x[0]) * iw[1];
-    double t2 = t1 * cf[1];
-    double t3 = 1.0 / (2.0 * (t1 + 1.0));
-    cf[3 + 1] = t3;
-    double t4 = 3.0 * (cf[3] - cf[0]) * iw[1] - t2 * cf[2];
-    cf[3 + 2] = t4;
-    for (int i = 2; i + 1 < n; i++) {
-      t1 = (x[i] - x[i - 1]) * iw[i];
-      t2 = t1 * t3;
-      t3 = 1.0 / (2.0 * t1 + 2.0 - t2);
-      cf[3 * i + 1] = t3;
-      t4 = 3.0 * (cf[3 * i] - cf[3 * (i - 1)]) * iw[i] - t2 * t4;
-      cf[3 * i + 2] = t4;
-    }
-    cf[3 * (n - 1) + 1] = 1.0;
-    t1 = (cf[3 * (n - 2)] - cf[3 * (n - 3)]) / (x[n - 1] - x[n - 3]);
-    cf[3 * (n - 1) + 2] = t1;
-    for (int i = n - 2; i >= 1; i--) {
-      t1 = cf[3 * i + 2] - t1 * cf[3 * (i + 1) + 1];
-      cf[3 * i + 2] = t1;
-    }
-    for (int i = 0; i < n; i++) cf[3 * i + 1] *= cf[3 * i + 2];
-    for (int i = 0; i + 1 < n; i++) {
-      t1 = (1.0 / 3.0) * (cf[3 * (i + 1) + 1] - cf[3 * i + 1]) * iw[i];
-      cf[3 * i + 2] = t1;
-      t2 = x[i + 1] - x[i];
-      cf[3 * i] = cf[3 * i] - (cf[3 * i + 1] + t1 * t2) * t2;
-    }
+    spline_coefficients(n, x, f.data(), cf.data(), iw.data());
     double g = 0.0;
     for (int i = 0; i + 1 < n; i++) {
       const double dx = x[i + 1] - x[i];
@@ -283,6 +293,20 @@ void spline_integral_weights(int n, const double *x, double *w) {
                 dx + f[i]) * dx;
     }
     w[j] = g;
+  }
+}
+
+// The first derivative at the grid points that fderiv.f90 (m = 1) returns,
+// cf(1, :) of the spline, as a matrix: dt[j * ld + i] is the derivative at
+// point i of the j-th unit vector (stored so that a workgroup reading one j
+// reads consecutive i).
+void spline_derivative_matrix(int n, const double *x, int ld, double *dt) {
+  std::vector<double> cf(3 * (size_t)n), iw(n), f(n);
+  for (int j = 0; j < n; j++) {
+    std::fill(f.begin(), f.end(), 0.0);
+    f[j] = 1.0;
+    spline_coefficients(n, x, f.data(), cf.data(), iw.data());
+    for (int i = 0; i < n; i++) dt[(size_t)j * ld + i] = cf[3 * i];
   }
 }
 
@@ -451,6 +475,166 @@ __global__ void add_tables_kernel(size_t n, const double *__restrict__ a,
                                   const double *__restrict__ b, double *__restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a ? a[i] + (b ? b[i] : 0.0) : 0.0;
+}
+
+// ---- spin-orbit terms (mt_so.f90) ----------------------------------------
+
+// The twelve tables of one call: alpha (aa, loa, lolo, alo), beta, ab.
+struct SoTables {
+  double2 *t[12];
+};
+
+struct SoArgs {
+  int nfa, p, lmaxapw, nrmtmax, nlomax, lmmaxvr, maxaa, mnlo, rel;
+  double a;   // alpha^2/2 (mt_so.f90:38,75)
+};
+
+constexpr int kSoThreads = 256;
+
+// One workgroup per atom.  LDS: rm (the relativistic factor of mt_so.f90:81-93
+// on the grid) and q = w r^2 (1/r) d rm/dr, w the spline integration weights,
+// so that the radial integral of a pair is sum_r q u1 u2.  Each thread then
+// takes same-l pairs of radial functions (f1, f2, l >= 1) and adds, for every
+// m, the L.S elements: alpha += m R/2, beta -= m R/2 on (m, m), ab +=
+// sqrt(l(l+1) - m(m+1)) R/2 on (m, m+1).  Every table element is written by
+// one (pair, m) only.
+__global__ __launch_bounds__(kSoThreads) void so_kernel(
+    SoArgs A, const int *__restrict__ atom_species, const int *__restrict__ nrmt,
+    const double *__restrict__ w, const double *__restrict__ spr, const double *__restrict__ dt,
+    const int4 *__restrict__ pairs, const int *__restrict__ pair_start,
+    const int2 *__restrict__ rowb, const double *__restrict__ apwfr,
+    const double *__restrict__ lofr, const double *__restrict__ pot, SoTables T) {
+  extern __shared__ double so_lds[];
+  double *rm = so_lds, *q = so_lds + A.nrmtmax;
+  const int ias = blockIdx.x, is = atom_species[ias], nr = nrmt[is];
+  const double *x = spr + (size_t)A.nrmtmax * is;
+  constexpr double kY00 = 0.28209479177387814347;
+  for (int ir = threadIdx.x; ir < nr; ir += blockDim.x) {
+    const double v = A.a * pot[(size_t)A.lmmaxvr * (ir + (size_t)A.nrmtmax * ias)] * kY00;
+    rm[ir] = A.rel == 0 ? v : A.rel == 1 ? 1.0 / (1.0 - v) : A.a / ((1.0 - v) * (1.0 - v));
+  }
+  __syncthreads();
+  const double *D = dt + (size_t)A.nrmtmax * A.nrmtmax * is;
+  const double *ws = w + (size_t)A.nrmtmax * is;
+  // The spline maps a constant to a zero derivative, so D is applied to
+  // rm - rm(nr): rm is near 1 for zora and the entries of D reach 1/dx, and
+  // without the shift the sum would cancel to a few digits near the origin.
+  const double c0 = rm[nr - 1];
+  for (int ir = threadIdx.x; ir < nr; ir += blockDim.x) {
+    double d = 0.0;
+    for (int j = 0; j < nr; j++) d += D[(size_t)j * A.nrmtmax + ir] * (rm[j] - c0);
+    q[ir] = ws[ir] * (d / x[ir]) * x[ir] * x[ir];
+  }
+  __syncthreads();
+  for (int t = pair_start[is] + threadIdx.x; t < pair_start[is + 1]; t += blockDim.x) {
+    const int4 P = pairs[t];   // f1, f2, l, kind (1: f1 is an LO, 2: f2 is an LO)
+    double R = 0.0;
+    for (int ir = 0; ir < nr; ir++) {
+      const double u1 = P.x < A.nfa
+          ? apwfr[ir + (size_t)A.nrmtmax * 2 * (P.x % A.p + (size_t)A.p * (P.x / A.p + (size_t)(A.lmaxapw + 1) * ias))]
+          : lofr[ir + (size_t)A.nrmtmax * 2 * ((P.x - A.nfa) + (size_t)A.nlomax * ias)];
+      const double u2 = P.y < A.nfa
+          ? apwfr[ir + (size_t)A.nrmtmax * 2 * (P.y % A.p + (size_t)A.p * (P.y / A.p + (size_t)(A.lmaxapw + 1) * ias))]
+          : lofr[ir + (size_t)A.nrmtmax * 2 * ((P.y - A.nfa) + (size_t)A.nlomax * ias)];
+      R += q[ir] * u1 * u2;
+    }
+    const int l = P.z, kind = P.w;
+    const int2 b1 = rowb[is * kNfMax + P.x], b2 = rowb[is * kNfMax + P.y];
+    const int ld = (kind & 1) ? A.mnlo : A.maxaa, nc = (kind & 2) ? A.mnlo : A.maxaa;
+    // alpha/beta/ab table of this kind: aa (0), loa (1), lolo (3); ab also alo (2)
+    const int slot = kind == 0 ? 0 : kind == 1 ? 1 : kind == 3 ? 2 : 3;
+    double2 *al = slot < 3 ? T.t[slot] : nullptr, *be = slot < 3 ? T.t[4 + slot] : nullptr;
+    double2 *ab = T.t[8 + slot];
+    for (int m = -l; m <= l; m++) {
+      const size_t r = b1.x + (size_t)(m + l) * b1.y;
+      if (al) {
+        const size_t e = r + (size_t)ld * (b2.x + (size_t)(m + l) * b2.y + (size_t)nc * ias);
+        al[e].x += 0.5 * m * R;
+        be[e].x -= 0.5 * m * R;
+      }
+      if (m < l) {
+        const size_t e = r + (size_t)ld * (b2.x + (size_t)(m + l + 1) * b2.y + (size_t)nc * ias);
+        ab[e].x += 0.5 * sqrt((double)(l * (l + 1) - m * (m + 1))) * R;
+      }
+    }
+  }
+}
+
+// alo(i, j, ias) = conjg(loa(j, i, ias)) for alpha and beta (mt_so.f90:361-366).
+__global__ void so_alo_kernel(int maxaa, int maxnlo, int mnlo, int natm, SoTables T) {
+  const long long n = (long long)maxaa * maxnlo * natm;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = (int)(k % maxaa), j = (int)((k / maxaa) % maxnlo), ias = (int)(k / ((long long)maxaa * maxnlo));
+  const size_t alo = i + (size_t)maxaa * (j + (size_t)mnlo * ias);
+  const size_t loa = j + (size_t)mnlo * (i + (size_t)maxaa * ias);
+  const double2 a = T.t[1][loa], b = T.t[5][loa];
+  T.t[3][alo] = make_double2(a.x, -a.y);
+  T.t[7][alo] = make_double2(b.x, -b.y);
+}
+
+__global__ void zero_kernel(size_t n, double2 *__restrict__ p) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+
+// Per species: the derivative matrices, the same-l pair list and the row base
+// and m stride of every radial function (APW (l, io): off(l) + io, stride
+// apword(l); LO ilo: its first row, stride 1).  Rebuilt when the basis or
+// the radial grid changed.
+int so_prepare(exh_context *c) {
+  if (int rc = prepare(c, c->hml_lmaxvr >= 0 ? c->hml_lmaxvr : 0)) return rc;
+  if (c->so_radial_epoch == c->radial_epoch && c->so_basis_epoch == c->basis_epoch) return 0;
+  const int ns = c->nspecies, p = c->apwordmax, lmaxmat = c->lmaxmat, nrm = c->nrmtmax;
+  const int nfa = (lmaxmat + 1) * p;
+  std::vector<double> dt((size_t)ns * nrm * nrm, 0.0);
+  for (int is = 0; is < ns; is++)
+    spline_derivative_matrix(c->nrmt_h[is], c->spr_h.data() + (size_t)nrm * is, nrm,
+                             dt.data() + (size_t)nrm * nrm * is);
+  std::vector<int2> rowb((size_t)ns * kNfMax, make_int2(0, 0));
+  std::vector<int> fl((size_t)ns * kNfMax, -1);
+  std::vector<int4> pairs;
+  std::vector<int> start(ns + 1, 0);
+  for (int is = 0; is < ns; is++) {
+    const int *aw = c->apword.data() + (c->lmaxapw + 1) * is;
+    int off = 0;
+    for (int l = 0; l <= lmaxmat; l++) {
+      for (int io = 0; io < aw[l]; io++) {
+        rowb[is * kNfMax + l * p + io] = make_int2(off + io, aw[l]);
+        fl[is * kNfMax + l * p + io] = l;
+      }
+      off += aw[l] * (2 * l + 1);
+    }
+    int lo0 = 0;
+    for (int ilo = 0; ilo < c->nlorb[is]; ilo++) {
+      const int l = c->lorbl[ilo + c->nlomax * is];
+      rowb[is * kNfMax + nfa + ilo] = make_int2(lo0, 1);
+      fl[is * kNfMax + nfa + ilo] = l;
+      lo0 += 2 * l + 1;
+    }
+    start[is] = (int)pairs.size();
+    for (int f1 = 0; f1 < kNfMax; f1++)
+      for (int f2 = 0; f2 < kNfMax; f2++) {
+        const int l1 = fl[is * kNfMax + f1], l2 = fl[is * kNfMax + f2];
+        if (l1 < 1 || l1 != l2) continue;
+        pairs.push_back(make_int4(f1, f2, l1, (f1 >= nfa ? 1 : 0) | (f2 >= nfa ? 2 : 0)));
+      }
+  }
+  start[ns] = (int)pairs.size();
+  auto put = [&](DevBuf &b, const void *src, size_t bytes) {
+    int rc = ensure(c, b, std::max<size_t>(bytes, 16));
+    if (rc || bytes == 0) return rc;
+    hipError_t e = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "mt_so tables");
+  };
+  int rc = put(c->so_dt, dt.data(), dt.size() * sizeof(double));
+  if (!rc) rc = put(c->so_rowb, rowb.data(), rowb.size() * sizeof(int2));
+  if (!rc) rc = put(c->so_pairs, pairs.data(), pairs.size() * sizeof(int4));
+  if (!rc) rc = put(c->so_pstart, start.data(), start.size() * sizeof(int));
+  if (rc) return rc;
+  c->so_radial_epoch = c->radial_epoch;
+  c->so_basis_epoch = c->basis_epoch;
+  return 0;
 }
 
 }  // namespace
@@ -682,6 +866,121 @@ int exh_set_mt_device(exh_context *c, void *stream, int maxaa, int maxnlo, const
   c->have_h1aa = h1aa != nullptr;
   c->have_mt = true;
   return 0;
+}
+
+
+int exh_mt_so_device(exh_context *c, void *stream, int lmaxvr, const double *pot,
+                     const double *lofr, int rel_level, exh_z *alpha_aa, exh_z *alpha_loa,
+                     exh_z *alpha_lolo, exh_z *alpha_alo, exh_z *beta_aa, exh_z *beta_loa,
+                     exh_z *beta_lolo, exh_z *beta_alo, exh_z *ab_aa, exh_z *ab_loa,
+                     exh_z *ab_lolo, exh_z *ab_alo, int flags) {
+  if (!c) return EXH_EINVAL;
+  if (int rc = wait_done(c)) return rc;
+  if (!c->have_basis || !c->have_radial)
+    return fail(c, EXH_ESTATE, "mt_so: basis and radial data not set");
+  if (lmaxvr < 0 || lmaxvr > 50) return fail(c, EXH_EINVAL, "mt_so: lmaxvr outside 0..50");
+  if (rel_level < 0 || rel_level > 2) return fail(c, EXH_EINVAL, "mt_so: rel_level outside 0..2");
+  if (flags & ~EXH_ACCUMULATE) return fail(c, EXH_EINVAL, "mt_so: unknown flags");
+  exh_z *tab[12] = {alpha_aa, alpha_loa, alpha_lolo, alpha_alo, beta_aa, beta_loa,
+                    beta_lolo, beta_alo, ab_aa, ab_loa, ab_lolo, ab_alo};
+  const bool lo = c->nlomax > 0;
+  for (int k = 0; k < 12; k++)
+    if (!tab[k] && (k % 4 == 0 || lo)) return fail(c, EXH_EINVAL, "mt_so: null array");
+  if (!pot || (lo && !lofr)) return fail(c, EXH_EINVAL, "mt_so: null array");
+  if ((size_t)2 * c->nrmtmax * sizeof(double) > 64 * 1024)
+    return fail(c, EXH_ERANGE, "mt_so: nrmtmax above 4096");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  if (int rc = so_prepare(c)) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int natm = c->natmtot, maxaa = c->maxaa, mnlo = std::max(c->maxnlo, 1);
+  const size_t sizes[4] = {(size_t)maxaa * maxaa * natm, (size_t)mnlo * maxaa * natm,
+                           (size_t)mnlo * mnlo * natm, (size_t)maxaa * mnlo * natm};
+  SoTables T;
+  for (int k = 0; k < 12; k++) T.t[k] = (double2 *)tab[k];
+  const double alpha = 1.0 / 137.03599911;
+  SoArgs A{(c->lmaxmat + 1) * c->apwordmax, c->apwordmax, c->lmaxapw, c->nrmtmax,
+           std::max(c->nlomax, 1), (lmaxvr + 1) * (lmaxvr + 1), maxaa, mnlo, rel_level,
+           0.5 * alpha * alpha};
+  const int *counts = (const int *)c->hml_count.p;
+  const double *lf = lofr ? lofr : (const double *)c->hml_w.p;  // never read when nlomax = 0
+  hipError_t e = timed(c, EXH_K_SO, s, [&] {
+    hipError_t le = hipSuccess;
+    if (!(flags & EXH_ACCUMULATE))
+      for (int k = 0; k < 12 && le == hipSuccess; k++) {
+        if (!tab[k]) continue;
+        hipLaunchKernelGGL(zero_kernel, dim3(cdiv_ll((long long)sizes[k % 4], 256)), dim3(256), 0,
+                           s, sizes[k % 4], T.t[k]);
+        le = hipGetLastError();
+      }
+    if (le != hipSuccess) return le;
+    hipLaunchKernelGGL(so_kernel, dim3(natm), dim3(kSoThreads),
+                       2 * c->nrmtmax * sizeof(double), s, A, (const int *)c->atom_species_d.p,
+                       counts + 2 * c->nspecies, (const double *)c->hml_w.p,
+                       (const double *)c->hml_spr.p, (const double *)c->so_dt.p,
+                       (const int4 *)c->so_pairs.p, (const int *)c->so_pstart.p,
+                       (const int2 *)c->so_rowb.p, (const double *)c->apwfr_d.p, lf, pot, T);
+    le = hipGetLastError();
+    if (le == hipSuccess && lo && c->maxnlo > 0) {
+      hipLaunchKernelGGL(so_alo_kernel,
+                         dim3(cdiv_ll((long long)maxaa * c->maxnlo * natm, 256)), dim3(256), 0, s,
+                         maxaa, c->maxnlo, mnlo, natm, T);
+      le = hipGetLastError();
+    }
+    return le;
+  });
+  if (e == hipSuccess) e = mark_done(c, s);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "mt_so");
+}
+
+int exh_mt_so(exh_context *c, int lmaxvr, const double *pot, const double *lofr, int rel_level,
+              exh_z *alpha_aa, exh_z *alpha_loa, exh_z *alpha_lolo, exh_z *alpha_alo,
+              exh_z *beta_aa, exh_z *beta_loa, exh_z *beta_lolo, exh_z *beta_alo, exh_z *ab_aa,
+              exh_z *ab_loa, exh_z *ab_lolo, exh_z *ab_alo, int flags) {
+  if (!c) return EXH_EINVAL;
+  if (int rc = wait_done(c)) return rc;
+  if (!c->have_basis || !c->have_radial)
+    return fail(c, EXH_ESTATE, "mt_so: basis and radial data not set");
+  if (lmaxvr < 0 || lmaxvr > 50) return fail(c, EXH_EINVAL, "mt_so: lmaxvr outside 0..50");
+  exh_z *tab[12] = {alpha_aa, alpha_loa, alpha_lolo, alpha_alo, beta_aa, beta_loa,
+                    beta_lolo, beta_alo, ab_aa, ab_loa, ab_lolo, ab_alo};
+  const bool lo = c->nlomax > 0;
+  for (int k = 0; k < 12; k++)
+    if (!tab[k] && (k % 4 == 0 || lo)) return fail(c, EXH_EINVAL, "mt_so: null array");
+  if (!pot || (lo && !lofr)) return fail(c, EXH_EINVAL, "mt_so: null array");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  const int natm = c->natmtot, maxaa = c->maxaa, mnlo = std::max(c->maxnlo, 1);
+  const size_t sizes[4] = {(size_t)maxaa * maxaa * natm, (size_t)mnlo * maxaa * natm,
+                           (size_t)mnlo * mnlo * natm, (size_t)maxaa * mnlo * natm};
+  size_t off[13] = {0};
+  for (int k = 0; k < 12; k++) off[k + 1] = off[k] + (tab[k] ? sizes[k % 4] : 0);
+  const size_t lmm = (size_t)(lmaxvr + 1) * (lmaxvr + 1);
+  int rc = copy_in(c, c->hml_vin, pot, lmm * c->nrmtmax * natm);
+  if (!rc && lo) rc = copy_in(c, c->hml_loin, lofr, (size_t)c->nrmtmax * 2 * c->nlomax * natm);
+  if (!rc) rc = ensure(c, c->so_host, off[12] * sizeof(double2));
+  if (rc) return rc;
+  double2 *base = (double2 *)c->so_host.p;
+  exh_z *dev[12];
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 12; k++) {
+    dev[k] = tab[k] ? (exh_z *)(base + off[k]) : nullptr;
+    if (tab[k] && (flags & EXH_ACCUMULATE) && e == hipSuccess)
+      e = hipMemcpyAsync(dev[k], tab[k], sizes[k % 4] * sizeof(double2), hipMemcpyHostToDevice,
+                         c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "upload");
+  rc = exh_mt_so_device(c, nullptr, lmaxvr, (const double *)c->hml_vin.p,
+                        lo ? (const double *)c->hml_loin.p : nullptr, rel_level, dev[0], dev[1],
+                        dev[2], dev[3], dev[4], dev[5], dev[6], dev[7], dev[8], dev[9], dev[10],
+                        dev[11], flags);
+  if (rc) return rc;
+  for (int k = 0; k < 12; k++) {
+    if (!tab[k]) continue;
+    e = hipMemcpyAsync(tab[k], dev[k], sizes[k % 4] * sizeof(double2), hipMemcpyDeviceToHost,
+                       c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "download");
+  }
+  e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "download");
 }
 
 }  // extern "C"

@@ -25,6 +25,8 @@
 !   exh_mt_pot                   replaces  mt_pot (src/src_eigensystem/mt_pot.f90:14),
 !                                          as the spin-polarised blocks of hmlint
 !                                          call it (hmlint.f90:58-173)
+!   exh_mt_so                    replaces  mt_so (src/src_eigensystem/mt_so.f90:11),
+!                                          called from hmlint.f90:172
 module exh_lapw
   use, intrinsic :: iso_c_binding
   implicit none
@@ -54,7 +56,7 @@ module exh_lapw
             exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
             exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device, &
-            exh_mt_pot, exh_mt_pot_device
+            exh_mt_pot, exh_mt_pot_device, exh_mt_so, exh_mt_so_device
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -219,6 +221,28 @@ module exh_lapw
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, stream, pot, lofr, aa, loa, lolo
       integer(c_int), value :: lmaxvr, flags
+    end function
+
+    ! alpha, beta and ab tables (aa, loa, lolo, alo each); rel_level 1 = level_zora
+    integer(c_int) function exh_mt_so(ctx, lmaxvr, pot, lofr, rel_level, alpha_aa, alpha_loa, &
+        alpha_lolo, alpha_alo, beta_aa, beta_loa, beta_lolo, beta_alo, ab_aa, ab_loa, ab_lolo, &
+        ab_alo, flags) bind(C, name="exh_mt_so")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lmaxvr, rel_level, flags
+      real(c_double), intent(in) :: pot(*), lofr(*)
+      complex(c_double_complex), intent(inout) :: alpha_aa(*), alpha_loa(*), alpha_lolo(*), &
+          alpha_alo(*), beta_aa(*), beta_loa(*), beta_lolo(*), beta_alo(*), ab_aa(*), &
+          ab_loa(*), ab_lolo(*), ab_alo(*)
+    end function
+
+    integer(c_int) function exh_mt_so_device(ctx, stream, lmaxvr, pot, lofr, rel_level, &
+        alpha_aa, alpha_loa, alpha_lolo, alpha_alo, beta_aa, beta_loa, beta_lolo, beta_alo, &
+        ab_aa, ab_loa, ab_lolo, ab_alo, flags) bind(C, name="exh_mt_so_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, pot, lofr, alpha_aa, alpha_loa, alpha_lolo, alpha_alo, &
+          beta_aa, beta_loa, beta_lolo, beta_alo, ab_aa, ab_loa, ab_lolo, ab_alo
+      integer(c_int), value :: lmaxvr, rel_level, flags
     end function
 
     ! The kinetic radial integrals alone (mt_kin, kinetic_energy): tkin(nf,nf,natmtot).

@@ -100,6 +100,8 @@ def lib():
     L.exh_mt_kinetic_device.argtypes = [_p, _p, _i, _p, _p, _i, _p]
     L.exh_mt_pot.argtypes = [_p, _i, _p, _p, _p, _p, _p, _i]
     L.exh_mt_pot_device.argtypes = [_p, _p, _i, _p, _p, _p, _p, _p, _i]
+    L.exh_mt_so.argtypes = [_p, _i, _p, _p, _i] + [_p] * 12 + [_i]
+    L.exh_mt_so_device.argtypes = [_p, _p, _i, _p, _p, _i] + [_p] * 12 + [_i]
     _lib = L
     return L
 
@@ -121,12 +123,13 @@ EXPORTS = (
     "exh_set_interstitial_device", "exh_gencfun", "exh_gencfun_device", "exh_genveffig",
     "exh_genmeffig", "exh_genveffig_device", "exh_set_mt_device", "exh_hmlint",
     "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
-    "exh_mt_kinetic_device", "exh_mt_pot", "exh_mt_pot_device",
+    "exh_mt_kinetic_device", "exh_mt_pot", "exh_mt_pot_device", "exh_mt_so",
+    "exh_mt_so_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
            "oz_gemm", "oz_combine", "ho_grouped", "batch_graph", "cfunig", "ifft",
-           "iproducts", "igather", "hml_radial", "hml_contract")
+           "iproducts", "igather", "hml_radial", "hml_contract", "mt_so")
 
 MATCH = 2   # batch: run match first
 GRAPH = 4   # batch: capture in a HIP graph, replay on identical calls
@@ -412,6 +415,45 @@ class Context:
                 t.mul_(-1j)
             self.mt_pot(lmaxvr, component(0, 0), lofr, *ab, flags=ACCUMULATE)
             out["ab"] = ab
+        return out
+
+    def mt_so(self, lmaxvr, pot, lofr, tables=None, rel_level=1, flags: int = ACCUMULATE,
+              stream=None):
+        """mt_so.f90:11-397: the spin-orbit terms added to the alpha, beta and
+        ab blocks, tables = {"alpha": (aa, loa, lolo, alo), "beta": ...,
+        "ab": ...}.  With CUDA tensors (pot, lofr and every table, Fortran
+        layouts flattened) the device entry point updates the tables in
+        place; otherwise NumPy arrays are returned (the given ones updated,
+        missing ones created as zeros).  rel_level 1 is level_zora, the
+        level hmlint.f90:172 uses.  flags=0 zeroes the tables first."""
+        keys = ("alpha", "beta", "ab")
+        if _is_torch(pot):
+            ptrs = [_ptr(x) for k in keys for x in tables[k]]
+            rc = self._device_call(stream, lambda st: lib().exh_mt_so_device(
+                self._h, st, lmaxvr, _ptr(pot), _ptr(lofr), rel_level, *ptrs, flags))
+            self._check(rc, "exh_mt_so_device")
+            return tables
+        na, mnlo, maxaa = sum(self.natoms), max(self.maxnlo, 1), self.maxaa
+        shapes = ((maxaa, maxaa, na), (mnlo, maxaa, na), (mnlo, mnlo, na), (maxaa, mnlo, na))
+        out = {}
+        for k in keys:
+            given = None if tables is None else tables.get(k)
+            arrs = []
+            for i, shape in enumerate(shapes):
+                x = None if given is None else given[i]
+                if x is None:
+                    x = np.zeros(shape, dtype=np.complex128, order="F")
+                elif not (isinstance(x, np.ndarray) and x.flags.f_contiguous
+                          and x.dtype == np.complex128 and x.shape == shape):
+                    raise ExhError(f"mt_so {k} tables must be Fortran-ordered complex128 "
+                                   f"arrays of shapes {shapes}")
+                arrs.append(x)
+            out[k] = tuple(arrs)
+        v = _host(pot, np.float64)
+        lf = _host(lofr, np.float64) if lofr is not None else None
+        self._check(lib().exh_mt_so(self._h, lmaxvr, _ptr(v), _ptr(lf), rel_level,
+                                    *[_ptr(x) for k in keys for x in out[k]], flags),
+                    "exh_mt_so")
         return out
 
     def _mt_tables(self, kin, lmaxvr, veffmt, lofr, valence_rel_none, aa, loa, lolo, flags,

@@ -168,6 +168,31 @@ int exh_mt_pot(exh_context *ctx, int lmaxvr, const double *pot, const double *lo
 int exh_mt_pot_device(exh_context *ctx, void *stream, int lmaxvr, const double *pot,
                       const double *lofr, exh_z *aa, exh_z *loa, exh_z *lolo, int flags);
 
+/* mt_so.f90:11-397, as hmlint.f90:172 calls it (mt_so(veffmt, mt_basis,
+ * mt_basis, mt_h, level_zora)) when spin-orbit coupling is on: for every
+ * pair of radial functions of equal l >= 1, R = integral of u1 u2 r^2
+ * (1/r) d/dr rm(r), with rm = a V00 y00 (rel_level 0), 1/(1 - a V00 y00)
+ * (1, zora) or a/(1 - a V00 y00)^2 (2), a = alpha^2/2 and V00 = pot(1,:,ias),
+ * the derivative being fderiv.f90's (m = 1).  Per m, alpha += m R/2 and
+ * beta -= m R/2 on (m, m); ab += sqrt(l(l+1) - m(m+1)) R/2 on (m, m+1), in
+ * aa, loa, lolo and (APW row, LO column) alo.  alpha%alo and beta%alo are
+ * then set to conjg(transpose(loa)) over the whole table (:361-366).
+ * Tables: aa(maxaa,maxaa,natmtot), loa(maxnlo,maxaa,natmtot),
+ * lolo(maxnlo,maxnlo,natmtot), alo(maxaa,maxnlo,natmtot); the LO tables may
+ * be null without local orbitals.  EXH_ACCUMULATE adds to them as the
+ * reference does (after hmlint's mt_pot calls); 0 zeroes them first.
+ * nrmtmax <= 4096. */
+int exh_mt_so(exh_context *ctx, int lmaxvr, const double *pot, const double *lofr,
+              int rel_level, exh_z *alpha_aa, exh_z *alpha_loa, exh_z *alpha_lolo,
+              exh_z *alpha_alo, exh_z *beta_aa, exh_z *beta_loa, exh_z *beta_lolo,
+              exh_z *beta_alo, exh_z *ab_aa, exh_z *ab_loa, exh_z *ab_lolo, exh_z *ab_alo,
+              int flags);
+int exh_mt_so_device(exh_context *ctx, void *stream, int lmaxvr, const double *pot,
+                     const double *lofr, int rel_level, exh_z *alpha_aa, exh_z *alpha_loa,
+                     exh_z *alpha_lolo, exh_z *alpha_alo, exh_z *beta_aa, exh_z *beta_loa,
+                     exh_z *beta_lolo, exh_z *beta_alo, exh_z *ab_aa, exh_z *ab_loa,
+                     exh_z *ab_lolo, exh_z *ab_alo, int flags);
+
 /* The kinetic radial integrals alone, as mt_kin.f90:117-170 and
  * kinetic_energy.f90:50-183 (t_aa, t_alo, t_lolo) take them: for the radial
  * functions f of each atom -- APW (l, io) as f = l*apwordmax + io for
@@ -351,7 +376,8 @@ enum {
   EXH_K_IGATHER = 16, /* grid -> G gathers (cfunir, veffig, meffig...)  */
   EXH_K_HMLRAD = 17,  /* hmlint: radial integrals (potential, kinetic)  */
   EXH_K_HMLCON = 18,  /* hmlint: Gaunt contraction into aa/loa/lolo     */
-  EXH_K_COUNT = 19
+  EXH_K_SO = 19,      /* mt_so: spin-orbit terms of alpha/beta/ab       */
+  EXH_K_COUNT = 20
 };
 int exh_profile_enable(exh_context *ctx, int on);
 int exh_profile_read(exh_context *ctx, int kernel, double *total_ms, int *launches);

@@ -82,6 +82,7 @@ def lib():
         L.orc_gaunt_yry.argtypes = [_i] * 6 + [_p]
         L.orc_hmlint.argtypes = [_p, _i, _p, _p, _i, _p, _p, _p]
         L.orc_hmlint_parts.argtypes = [_p, _i, _p, _p, _i, _p, _p, _p, _i]
+        L.orc_mt_so.argtypes = [_p, _i, _p, _p, _i, _p]
         L.orc_nlotot.argtypes = [_p]
         L.orc_maxapw.argtypes = [_p]
         _lib = L
@@ -363,6 +364,34 @@ def hmlint(sys_, lmaxvr=None, veffmt=None, lofr=None, valence_rel_none=None, par
     if rc != 0:
         raise RuntimeError(f"orc_hmlint failed: {rc}")
     return aa, loa, lolo
+
+
+def mt_so(sys_, lmaxvr=None, pot=None, lofr=None, rel_level=1, tables=None):
+    """mt_so.f90:11-397 (hmlint.f90:172 calls it with veffmt and level_zora
+    = 1): the spin-orbit terms accumulated onto tables = {"alpha", "beta",
+    "ab"}, each (aa, loa, lolo, alo), zero when not given.  Returns new
+    arrays; alpha/beta alo are reset to conj(loa^T) as the reference does."""
+    s = sys_.s
+    lmaxvr = s.lmaxvr if lmaxvr is None else lmaxvr
+    pot = s.veffmt if pot is None else pot
+    lofr = s.lofr if lofr is None else lofr
+    mnlo = max(s.maxnlo, 1)
+    shapes = ((s.maxaa, s.maxaa, s.natmtot), (mnlo, s.maxaa, s.natmtot),
+              (mnlo, mnlo, s.natmtot), (s.maxaa, mnlo, s.natmtot))
+    out = {}
+    for key in ("alpha", "beta", "ab"):
+        src = None if tables is None else tables.get(key)
+        out[key] = tuple(np.zeros(sh, dtype=np.complex128, order="F") if src is None
+                         else np.array(src[i], dtype=np.complex128, order="F", copy=True)
+                         for i, sh in enumerate(shapes))
+    ptrs = (C.c_void_p * 12)(*[x.ctypes.data for k in ("alpha", "beta", "ab")
+                                    for x in out[k]])
+    v = np.asfortranarray(pot, dtype=np.float64)
+    lf = np.asfortranarray(lofr, dtype=np.float64)
+    rc = lib().orc_mt_so(sys_.ptr, lmaxvr, v.ctypes.data, lf.ctypes.data, rel_level, ptrs)
+    if rc != 0:
+        raise RuntimeError(f"orc_mt_so failed: {rc}")
+    return out
 
 
 def overlapsetup(sys_, ik: int, apwalm: np.ndarray, h: np.ndarray = None, fast: bool = True):

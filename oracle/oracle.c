@@ -1317,3 +1317,165 @@ int orc_hmlint_parts(const orc_system *s, int lmaxvr, const double *veffmt, cons
   free(fr); free(gr); free(Ipot); free(Ikin);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* mt_so.f90:11-397 (the branch with InterstitialSO false, the only one */
+/* the reference runs): the spin-orbit terms of the alpha, beta and ab  */
+/* blocks, accumulated.                                                 */
+/* ------------------------------------------------------------------ */
+int orc_mt_so(const orc_system *s, int lmaxvr, const double *pot, const double *lofr,
+              int rel_level, orc_z *const *t) {
+  const double y00 = 0.28209479177387814347;
+  const double alpha = 1.0 / 137.03599911, a = 0.5 * alpha * alpha; /* mt_so.f90:38,75 */
+  const int lmmaxvr = (lmaxvr + 1) * (lmaxvr + 1);
+  const int lmaxmat = s->lmaxmat, nrmtmax = s->nrmtmax, p = s->apwordmax;
+  const int nlomax = s->nlomax, maxaa = s->maxaa, mnlo = s->maxnlo > 1 ? s->maxnlo : 1;
+  if (lmaxmat > 62) return 1;
+  orc_z *al_aa = t[0], *al_loa = t[1], *al_lolo = t[2], *al_alo = t[3];
+  orc_z *be_aa = t[4], *be_loa = t[5], *be_lolo = t[6], *be_alo = t[7];
+  orc_z *ab_aa = t[8], *ab_loa = t[9], *ab_lolo = t[10], *ab_alo = t[11];
+  double *rmtable = (double *)malloc(sizeof(double) * nrmtmax);
+  double *rm2 = (double *)malloc(sizeof(double) * nrmtmax);
+  double *fr = (double *)malloc(sizeof(double) * nrmtmax);
+  double *gr = (double *)malloc(sizeof(double) * nrmtmax);
+  double *cf = (double *)malloc(sizeof(double) * 3 * nrmtmax);
+#define APWFR(ir, k, io, l, ias) \
+  s->apwfr[(ir) + (size_t)nrmtmax * ((k) + 2 * ((io) + (size_t)p * ((l) + (size_t)(s->lmaxapw + 1) * (ias))))]
+#define LOFR(ir, ilo, ias) lofr[(ir) + (size_t)nrmtmax * (2 * ((ilo) + (size_t)nlomax * (ias)))]
+#define POT00(ir, ias) pot[(size_t)lmmaxvr * ((ir) + (size_t)nrmtmax * (ias))]
+#define AA(x, i, j) (x)[(i) + (size_t)maxaa * ((j) + (size_t)maxaa * ias)]
+#define LOA(x, i, j) (x)[(i) + (size_t)mnlo * ((j) + (size_t)maxaa * ias)]
+#define LOLO(x, i, j) (x)[(i) + (size_t)mnlo * ((j) + (size_t)mnlo * ias)]
+#define ALO(x, i, j) (x)[(i) + (size_t)maxaa * ((j) + (size_t)mnlo * ias)]
+  int ias = 0;
+  for (int is = 0; is < s->nspecies; is++) {
+    const int nr = s->nrmt[is];
+    const double *spr = s->spr + (size_t)nrmtmax * is;
+    for (int ia = 0; ia < s->natoms[is]; ia++, ias++) {
+      for (int ir = 0; ir < nr; ir++) { /* :81-93 */
+        const double v = POT00(ir, ias) * y00;
+        if (rel_level == 0) rmtable[ir] = a * v;
+        else if (rel_level == 1) rmtable[ir] = 1.0 / (1.0 - a * v);
+        else rmtable[ir] = a / ((1.0 - a * v) * (1.0 - a * v));
+      }
+      /* :97-101: (1/r) d/dr of rmtable, the derivative being fderiv m = 1,
+       * i.e. the first spline coefficient */
+      orc_spline(nr, spr, rmtable, cf);
+      for (int ir = 0; ir < nr; ir++) rm2[ir] = cf[3 * ir] / spr[ir];
+      /* APW row offset of l: sum over l' < l of apword(l') (2l'+1) */
+      int off[64];
+      off[0] = 0;
+      for (int l = 1; l <= lmaxmat + 1; l++)
+        off[l] = off[l - 1] + apword_of(s, l - 1, is) * (2 * l - 1);
+#define RADINT(u1, u2) \
+  (__extension__({ \
+    for (int ir = 0; ir < nr; ir++) fr[ir] = (u1) * (u2) * rm2[ir] * spr[ir] * spr[ir]; \
+    orc_fderiv_int(nr, spr, fr, gr); \
+    gr[nr - 1]; \
+  }))
+      /* APW-APW (:108-165) */
+      for (int l1 = 1; l1 <= lmaxmat; l1++) {
+        const int na = apword_of(s, l1, is);
+        for (int io1 = 0; io1 < na; io1++)
+          for (int io2 = 0; io2 < na; io2++) {
+            const double R = RADINT(APWFR(ir, 0, io1, l1, ias), APWFR(ir, 0, io2, l1, ias));
+            for (int m1 = -l1; m1 <= l1; m1++) {
+              const int r = off[l1] + (m1 + l1) * na + io1;
+              if (m1 != l1) {
+                int c = off[l1] + (m1 + l1 + 1) * na + io2, rr = r;
+                if (g_fault == 8) { rr = c; c = r; } /* planted mistake: L+ in place of L- */
+                AA(ab_aa, rr, c) = zadd(AA(ab_aa, rr, c),
+                                        zmk(0.5 * sqrt((double)(l1 * (l1 + 1) - m1 * (m1 + 1))) * R, 0.0));
+              }
+              const int c = off[l1] + (m1 + l1) * na + io2;
+              AA(al_aa, r, c) = zadd(AA(al_aa, r, c), zmk(0.5 * m1 * R, 0.0));
+              AA(be_aa, r, c) = zadd(AA(be_aa, r, c), zmk(-0.5 * m1 * R, 0.0));
+            }
+          }
+      }
+      if (s->maxnlo > 0) {
+        /* APW row, LO column: ab%alo (:174-221) */
+        int lo0 = 0;
+        for (int ilo = 0; ilo < s->nlorb[is]; ilo++) {
+          const int l1 = lorbl_of(s, ilo, is);
+          if (l1 != 0) {
+            const int na = apword_of(s, l1, is);
+            for (int io = 0; io < na; io++) {
+              const double R = RADINT(APWFR(ir, 0, io, l1, ias), LOFR(ir, ilo, ias));
+              for (int m1 = -l1; m1 < l1; m1++) {
+                const int r = off[l1] + (m1 + l1) * na + io, c = lo0 + m1 + l1 + 1;
+                ALO(ab_alo, r, c) = zadd(ALO(ab_alo, r, c),
+                                         zmk(0.5 * sqrt((double)(l1 * (l1 + 1) - m1 * (m1 + 1))) * R, 0.0));
+              }
+            }
+          }
+          lo0 += 2 * l1 + 1;
+        }
+        /* LO row, APW column: ab%loa, alpha%loa, beta%loa (:223-284) */
+        lo0 = 0;
+        for (int ilo = 0; ilo < s->nlorb[is]; ilo++) {
+          const int l1 = lorbl_of(s, ilo, is);
+          if (l1 != 0) {
+            const int na = apword_of(s, l1, is);
+            for (int io = 0; io < na; io++) {
+              const double R = RADINT(LOFR(ir, ilo, ias), APWFR(ir, 0, io, l1, ias));
+              for (int m1 = -l1; m1 <= l1; m1++) {
+                const int r = lo0 + m1 + l1;
+                if (m1 != l1) {
+                  const int c = off[l1] + (m1 + l1 + 1) * na + io;
+                  LOA(ab_loa, r, c) = zadd(LOA(ab_loa, r, c),
+                                           zmk(0.5 * sqrt((double)(l1 * (l1 + 1) - m1 * (m1 + 1))) * R, 0.0));
+                }
+                const int c = off[l1] + (m1 + l1) * na + io;
+                LOA(al_loa, r, c) = zadd(LOA(al_loa, r, c), zmk(0.5 * m1 * R, 0.0));
+                LOA(be_loa, r, c) = zadd(LOA(be_loa, r, c), zmk(-0.5 * m1 * R, 0.0));
+              }
+            }
+          }
+          lo0 += 2 * l1 + 1;
+        }
+        /* LO-LO (:291-357) */
+        int r0 = 0;
+        for (int ilo1 = 0; ilo1 < s->nlorb[is]; ilo1++) {
+          const int l1 = lorbl_of(s, ilo1, is);
+          int c0 = 0;
+          for (int ilo2 = 0; ilo2 < s->nlorb[is]; ilo2++) {
+            const int l3 = lorbl_of(s, ilo2, is);
+            if (l1 == l3 && l1 != 0) {
+              const double R = RADINT(LOFR(ir, ilo1, ias), LOFR(ir, ilo2, ias));
+              for (int m1 = -l1; m1 <= l1; m1++) {
+                const int r = r0 + m1 + l1;
+                if (m1 != l1) {
+                  const int c = c0 + m1 + l1 + 1;
+                  LOLO(ab_lolo, r, c) = zadd(LOLO(ab_lolo, r, c),
+                                             zmk(0.5 * sqrt((double)(l1 * (l1 + 1) - m1 * (m1 + 1))) * R, 0.0));
+                }
+                const int c = c0 + m1 + l1;
+                LOLO(al_lolo, r, c) = zadd(LOLO(al_lolo, r, c), zmk(0.5 * m1 * R, 0.0));
+                LOLO(be_lolo, r, c) = zadd(LOLO(be_lolo, r, c), zmk(-0.5 * m1 * R, 0.0));
+              }
+            }
+            c0 += 2 * l3 + 1;
+          }
+          r0 += 2 * l1 + 1;
+        }
+        /* :361-366: alo = conjg(transpose(loa)) for alpha and beta, whole table */
+        for (int i = 0; i < maxaa; i++)
+          for (int j = 0; j < s->maxnlo; j++) {
+            ALO(al_alo, i, j) = conj(LOA(al_loa, j, i));
+            ALO(be_alo, i, j) = conj(LOA(be_loa, j, i));
+          }
+      }
+#undef RADINT
+    }
+  }
+#undef APWFR
+#undef LOFR
+#undef POT00
+#undef AA
+#undef LOA
+#undef LOLO
+#undef ALO
+  free(rmtable); free(rm2); free(fr); free(gr); free(cf);
+  return 0;
+}

@@ -166,6 +166,17 @@ int orc_hmlint(const orc_system *s, int lmaxvr, const double *veffmt, const doub
 int orc_hmlint_parts(const orc_system *s, int lmaxvr, const double *veffmt, const double *lofr,
                      int valence_rel_none, orc_z *aa, orc_z *loa, orc_z *lolo, int parts);
 
+/* mt_so.f90:11-397 as the reference runs it (InterstitialSO false): the
+ * spin-orbit terms (1/r) d/dr[rmtable] L.S of every same-l pair of radial
+ * functions with l >= 1, accumulated into the alpha, beta and ab blocks.
+ * rel_level 0 (nr): rmtable = a V00 y00; 1 (zora, as hmlint.f90:172 calls
+ * it): 1/(1 - a V00 y00); 2: a/(1 - a V00 y00)^2, a = alpha^2/2, V00 =
+ * pot(1,:,ias).  t[12] = alpha (aa, loa, lolo, alo), beta (same), ab (same);
+ * alo(maxaa, maxnlo, natmtot).  alpha%alo and beta%alo are then overwritten
+ * with conjg(transpose(loa)) (:361-366); ab%alo only accumulates. */
+int orc_mt_so(const orc_system *s, int lmaxvr, const double *pot, const double *lofr,
+              int rel_level, orc_z *const *t);
+
 /* Planted-mistake switch for tests: 0 = faithful.  Nonzero values inject one
  * deliberate error so tests can show the checks catch it. */
 void orc_set_fault(int code);

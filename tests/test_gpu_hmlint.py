@@ -210,6 +210,58 @@ def test_hmlint_spin_blocks(name, ncmag):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name,rel_level", [("tiny", 1), ("tiny_zora", 0), ("tiny_nolo", 2),
+                                            ("sc8_k0", 1)])
+def test_mt_so_matches_the_oracle(name, rel_level):
+    """mt_so onto existing alpha/beta/ab tables (random, as hmlint's mt_pot
+    calls would leave them): host and device entry points against the
+    oracle's mt_so, and flags=0 equal to accumulating onto zeros."""
+    import torch
+    lmaxvr = 4 if name.startswith("tiny") else 8
+    s = with_potential(name, lmaxvr)
+    print(f"[gpu-mt_so] case {name} rel_level {rel_level} natmtot {s.natmtot}", flush=True)
+    S = oracle.System(s)
+    mnlo = max(s.maxnlo, 1)
+    shapes = ((s.maxaa, s.maxaa, s.natmtot), (mnlo, s.maxaa, s.natmtot),
+              (mnlo, mnlo, s.natmtot), (s.maxaa, mnlo, s.natmtot))
+    rng = np.random.default_rng(4)
+    # base tables of the size of the spin-orbit terms, so that the relative
+    # comparison below sees the terms themselves
+    pre = {k: tuple(np.asfortranarray(3e-5 * (rng.normal(size=sh) + 1j * rng.normal(size=sh)))
+                    for sh in shapes) for k in ("alpha", "beta", "ab")}
+    want = oracle.mt_so(S, pot=s.veffmt, rel_level=rel_level, tables=pre)
+    ctx = lapw.Context(0).load(s)
+    try:
+        got = ctx.mt_so(lmaxvr, s.veffmt, s.lofr,
+                        {k: tuple(x.copy(order="F") for x in v) for k, v in pre.items()},
+                        rel_level=rel_level)
+        for key in want:
+            for g, w, what in zip(got[key], want[key], ("aa", "loa", "lolo", "alo")):
+                e = err(g, w)
+                print(f"  {key} {what}: {e:.1e}")
+                assert e <= TOL, (key, what)
+        # device: bitwise the host entry point
+        dev_t = {k: tuple(dev(x) for x in v) for k, v in pre.items()}
+        ctx.mt_so(lmaxvr, dev(s.veffmt), dev(s.lofr), dev_t, rel_level=rel_level)
+        torch.cuda.synchronize()
+        for key in want:
+            for t, g in zip(dev_t[key], got[key]):
+                assert np.array_equal(host(t, g), g)
+        # flags=0 overwrites: the same as accumulating onto zeros
+        fresh = ctx.mt_so(lmaxvr, s.veffmt, s.lofr, {k: tuple(x.copy(order="F") for x in v)
+                                                      for k, v in pre.items()},
+                          rel_level=rel_level, flags=0)
+        zero = oracle.mt_so(S, pot=s.veffmt, rel_level=rel_level)
+        for key in want:
+            for g, w in zip(fresh[key], zero[key]):
+                assert np.abs(g - w).max() <= TOL * max(np.abs(w).max(), 1e-300)
+        with pytest.raises(lapw.ExhError, match="rel_level"):
+            ctx.mt_so(lmaxvr, s.veffmt, s.lofr, rel_level=3)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
 def test_hmlint_rejects_bad_arguments():
     s = with_potential("tiny", 4)
     ctx = lapw.Context(0).load(s)
