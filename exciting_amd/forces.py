@@ -1,5 +1,7 @@
-"""The k-point contribution to the basis-set-correction forces, forcek.f90
-(no spin, first-variational states, tevecsv false), on the GPU.
+"""The k-point contribution to the basis-set-correction forces, forcek.f90,
+on the GPU: first-variational states, or second-variational ones through
+their density matrix in the first-variational basis (tevecsv, normal
+spin-polarised or spin-spiral).
 
 For atom alpha and direction l the reference forms dlh = i (g_i - g_j) M_H
 and dlo = -i (g_i - g_j) M_O over the basis (g the l-th component of G+k on
@@ -46,12 +48,33 @@ def _rows(s):
     return out
 
 
-def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg):
+def density_matrix(evecsv, occsv, nstfv, nspinor=2, spiral_ispn=None):
+    """The state weights of the second-variational force sum
+    (forcek.f90:341-366): dm = sum_j occsv_j sum_b v_bj v_bj^H over the
+    nspinor blocks b of nstfv rows of evecsv(nstsv, nstsv), or over the one
+    block of first-variational spin spiral_ispn (0-based) for a spin spiral.
+    CUDA tensors; returns (nstfv, nstfv) complex128."""
+    blocks = [spiral_ispn] if spiral_ispn is not None else range(nspinor)
+    dm = None
+    for b in blocks:
+        v = evecsv[b * nstfv:(b + 1) * nstfv]
+        t = (v * occsv[None, :].to(v.dtype)) @ v.conj().transpose(0, 1)
+        dm = t if dm is None else dm + t
+    return dm
+
+
+def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None):
     """forcek.f90 for one k-point: a (3, natmtot) float64 CUDA tensor to add
     to forceibs.  d: lapw.DeviceKPoint (match is run here, as the reference
     does); evec (nstfv, nmat) complex128 (row = state); evalfv, occ (nstfv)
     float64; ffacg (ngvec, nspecies) float64 (genffacg); sfacg
-    (ngvec, natmtot) complex128, exp(i G.tau).  CUDA tensors throughout."""
+    (ngvec, natmtot) complex128, exp(i G.tau).  CUDA tensors throughout.
+
+    With second-variational states (tevecsv) pass dm = density_matrix(...)
+    (occ is then not read): the state sum sum_ab Re ffv_ab dm_ba becomes
+    -2 sum_i g_i Im(conj(Z_H) Y_H) + sum_i g_i Im(conj(Z_O) Y_O) summed over
+    states, with Z_H = X dm and Z_O = X (dm E + E dm), E = diag(evalfv),
+    which for dm = diag(occ) is the first-variational sum."""
     import torch
     s = ctx.sys
     dev = evec.device
@@ -126,10 +149,20 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg):
         FHW = torch.complex(R[ngk:, :m], R[ngk:, m:]).reshape(ngk, len(at), nst).transpose(0, 1)
         YO[at_t] += ph.conj()[:, :, None] * FW
         YH[at_t] += ph.conj()[:, :, None] * FHW
-    # Re ffv(j, j) per atom and direction
-    im_h = (XG.conj()[None] * YH).imag                                   # (natm, ngk, nst)
-    im_o = (XG.conj()[None] * YO).imag
-    fh = -2.0 * torch.einsum("il,aij->alj", vg, im_h)                    # (natm, 3, nst)
-    fo = 2.0 * torch.einsum("il,aij->alj", vg, im_o)
-    per_state = fh + fo * evalfv[None, None, :]
-    return wkpt * (per_state * occ[None, None, :]).sum(dim=2).transpose(0, 1)
+    if dm is None:
+        # Re ffv(j, j) per atom and direction
+        im_h = (XG.conj()[None] * YH).imag                               # (natm, ngk, nst)
+        im_o = (XG.conj()[None] * YO).imag
+        fh = -2.0 * torch.einsum("il,aij->alj", vg, im_h)                # (natm, 3, nst)
+        fo = 2.0 * torch.einsum("il,aij->alj", vg, im_o)
+        per_state = fh + fo * evalfv[None, None, :]
+        return wkpt * (per_state * occ[None, None, :]).sum(dim=2).transpose(0, 1)
+    # sum_ab Re ffv_ab dm_ba
+    dmc = dm.to(XG.dtype)
+    ev = evalfv.to(XG.dtype)
+    ZH = XG @ dmc
+    ZO = XG @ (dmc * ev[None, :] + ev[:, None] * dmc)
+    im_h = (ZH.conj()[None] * YH).imag.sum(dim=2)                         # (natm, ngk)
+    im_o = (ZO.conj()[None] * YO).imag.sum(dim=2)
+    f = -2.0 * (im_h @ vg) + im_o @ vg                                    # (natm, 3)
+    return wkpt * f.transpose(0, 1)

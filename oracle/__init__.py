@@ -558,13 +558,18 @@ def sfacg_of(s):
     return np.exp(1j * (v.T @ np.asarray(s.atposc).T))
 
 
-def forcek(sys_, ik, evecfv, evalfv, occ, wkpt, ffacg, sfacg):
-    """forcek.f90:1-404 for one k-point, no spin, first-variational states
-    only (tevecsv false): the contribution wkpt * sum_j occ_j Re ffv(j, j) to
-    forceibs(3, natmtot).  Per atom H and O of the muffin-tin and lo border
-    (:60-230), dlh/dlo with the folded G differences (:91-98, :259-285) and
-    the three zgemm (:286-335), restated with NumPy.  ffacg(ngvec, nspecies),
-    sfacg(ngvec, natmtot), evecfv(nmat, nstfv)."""
+def forcek(sys_, ik, evecfv, evalfv, occ, wkpt, ffacg, sfacg, evecsv=None, nspinor=2,
+           spiral_ispn=None):
+    """forcek.f90:1-404 for one k-point and one first-variational spin: the
+    contribution to forceibs(3, natmtot).  Per atom H and O of the
+    muffin-tin and lo border (:60-230), dlh/dlo with the folded G
+    differences (:91-98, :259-285) and the three zgemm (:286-335), restated
+    with NumPy.  ffacg(ngvec, nspecies), sfacg(ngvec, natmtot),
+    evecfv(nmat, nstfv), occ = occsv(nstsv).  The state sum (:340-372):
+    without evecsv (tevecsv false) wkpt sum_j occ_j Re ffv(j, j); with
+    evecsv(nstsv, nstsv) the second-variational sum over nspinor blocks of
+    nstfv rows, or over the block of first-variational spin spiral_ispn
+    (0-based) for a spin spiral."""
     s = sys_.s
     k = s.kpts[ik]
     ngk = k.ngk
@@ -630,7 +635,17 @@ def forcek(sys_, ik, evecfv, evalfv, occ, wkpt, ffacg, sfacg):
                 dlo[ngk:, :ngk] = dlo[:ngk, ngk:].conj().T
                 zmf = dlh @ X + dlo @ X2
                 ffv = X.conj().T @ zmf
-                out[l, ias] += wkpt * np.sum(occ * np.real(np.diag(ffv)))
+                nstfv = ffv.shape[0]
+                if evecsv is None:
+                    tot = np.sum(occ * np.real(np.diag(ffv)))
+                else:
+                    tot = 0.0
+                    blocks = [spiral_ispn] if spiral_ispn is not None else range(nspinor)
+                    for j in range(evecsv.shape[1]):
+                        for b in blocks:
+                            v = evecsv[b * nstfv:(b + 1) * nstfv, j]
+                            tot += occ[j] * np.real(v.conj() @ (ffv @ v))
+                out[l, ias] += wkpt * tot
             lob += los
             ias += 1
     return out

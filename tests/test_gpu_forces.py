@@ -1,7 +1,8 @@
 """The basis-set-correction force contribution of a k-point (forcek.f90) on
 the GPU (exciting_amd.forces: commutator form, batched GEMMs, per-species
 step-function matrices) against the oracle's literal restatement, itself
-pinned by finite differences (test_oracle_forcek.py).  Tolerance: 1e-10 of
+pinned by finite differences (test_oracle_forcek.py), for first-variational
+states and for second-variational ones through their density matrix.  Tolerance: 1e-10 of
 the largest force component (the GPU path sums the same terms in another
 order, through Y = M X instead of the derivative matrices)."""
 import dataclasses
@@ -38,5 +39,46 @@ def test_forcek_matches_the_oracle(name, nst):
         err = np.abs(got - want).max() / np.abs(want).max()
         print(f"  {name}: max |dF| / max |F| = {err:.1e}")
         assert err <= 1e-10
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,nst,spiral", [("tiny", 5, None), ("tiny", 5, 1),
+                                             ("sc8_k0", 10, None)])
+def test_second_variational_forcek_matches_the_oracle(name, nst, spiral):
+    """tevecsv: the density matrix of random unitary second-variational
+    vectors (two spinor blocks, or one block of a spin spiral) against the
+    oracle's literal state loop of forcek.f90:341-366."""
+    import torch
+    s = _cases.system(name)
+    s = dataclasses.replace(s, kpts=[s.kpts[0]])
+    print(f"[gpu-forces-sv] case {name} spiral {spiral}", flush=True)
+    ctx = lapw.Context(0).load(s)
+    try:
+        d = lapw.DeviceKPoint(s, 0)
+        n = d.rank
+        rng = np.random.default_rng(10)
+        X = rng.normal(size=(n, nst)) + 1j * rng.normal(size=(n, nst))
+        e = rng.uniform(-1, 1, nst)
+        nsv = 2 * nst
+        V = np.linalg.qr(rng.normal(size=(nsv, nsv)) + 1j * rng.normal(size=(nsv, nsv)))[0]
+        occ = rng.uniform(0, 1, nsv)
+        ff = np.stack([oracle.genffacg(s, i) for i in range(s.nspecies)], axis=1)
+        sg = oracle.sfacg_of(s)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        dm = forces.density_matrix(t(V), t(occ), nst, spiral_ispn=spiral)
+        got = forces.forcek(ctx, d, t(X.T), t(e), None, 0.5, t(ff), t(sg), dm=dm).cpu().numpy()
+        want = oracle.forcek(oracle.System(s), 0, X, e, occ, 0.5, ff, sg, evecsv=V,
+                             spiral_ispn=spiral)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print(f"  {name}: max |dF| / max |F| = {err:.1e}")
+        assert err <= 1e-10
+        # identity vectors with one spinor block: the first-variational sum
+        occ1 = rng.uniform(0, 2, nst)
+        dm1 = forces.density_matrix(t(np.eye(nst, dtype=complex)), t(occ1), nst, nspinor=1)
+        a = forces.forcek(ctx, d, t(X.T), t(e), None, 0.5, t(ff), t(sg), dm=dm1).cpu().numpy()
+        b = forces.forcek(ctx, d, t(X.T), t(e), t(occ1), 0.5, t(ff), t(sg)).cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
     finally:
         ctx.close()
