@@ -403,14 +403,17 @@ class Context:
             return v
 
         def tables():
-            return tuple(torch.zeros(n, dtype=torch.complex128, device=dev) for n in sizes)
+            # mt_pot with flags 0 writes every element of aa (and of loa and
+            # lolo when there are LOs; without them those stay as allocated)
+            new = torch.empty if self.maxnlo > 0 else torch.zeros
+            return tuple(new(n, dtype=torch.complex128, device=dev) for n in sizes)
 
         alpha = tables()
-        self.mt_pot(lmaxvr, component(2 if ncmag else 0, 2), lofr, *alpha)
+        self.mt_pot(lmaxvr, component(2 if ncmag else 0, 2), lofr, *alpha, flags=0)
         out = {"alpha": alpha, "beta": tuple(-t for t in alpha)}
         if ncmag:
             ab = tables()
-            self.mt_pot(lmaxvr, component(1, 1), lofr, *ab)
+            self.mt_pot(lmaxvr, component(1, 1), lofr, *ab, flags=0)
             for t in ab:
                 t.mul_(-1j)
             self.mt_pot(lmaxvr, component(0, 0), lofr, *ab, flags=ACCUMULATE)
