@@ -485,8 +485,8 @@ __device__ __forceinline__ void ho_tile(
 }
 
 // Single k-point: one workgroup per upper 64 x 64 tile pair I <= J, dealt to
-// the 8 XCDs in contiguous ranges, J-major within each range.  WH = false:
-// O alone.
+// the 8 XCDs in contiguous ranges, J-major within each range.  O alone runs
+// on o_kernel's larger tiles instead.
 template <bool WH>
 __global__ __launch_bounds__(NTH) void ho_kernel(
     int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
@@ -506,6 +506,28 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   IstBox box{ivgk, vgpc, T, lo0, lo1, lo2, n0, n1, n2};
   ho_tile<TM, TM, TK, const double2 *, double2 *, IstBox, WH>(
       ngp, Ktot, X, ldx, YB, ldb, YA, lda, box, H, O, ldh, accumulate, I * TM, J * TM);
+}
+
+// overlapsetup alone (O without the Hamiltonian's products): per operand
+// load it does half the MFMA work of the fused H + O tile, so it takes
+// TO x TO tiles (upper tile pairs I <= J, XCD-contiguous like ho_kernel) to
+// raise the flops per loaded byte.
+template <int TO, int TKO>
+__global__ __launch_bounds__(64 * (TO / 32) * (TO / 32)) void o_kernel(
+    int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
+    const double2 *__restrict__ YA, long long lda, const int4 *__restrict__ ivgk,
+    const double *__restrict__ vgpc, const IstEntry *__restrict__ T, int lo0, int lo1, int lo2,
+    int n0, int n1, int n2, double2 *__restrict__ O, long long ldh, int accumulate) {
+  const int bid = blockIdx.x, nb = gridDim.x;
+  const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
+  const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
+  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((J + 1) * (J + 2) / 2 <= t) J++;
+  while (J * (J + 1) / 2 > t) J--;
+  const int I = t - J * (J + 1) / 2;
+  IstBox box{ivgk, vgpc, T, lo0, lo1, lo2, n0, n1, n2};
+  ho_tile<TO, TO, TKO, const double2 *, double2 *, IstBox, false>(
+      ngp, Ktot, X, ldx, X, ldx, YA, lda, box, nullptr, O, ldh, accumulate, I * TO, J * TO);
 }
 
 // Grouped over k-points: a list of (k, I0, J0) tiles (k < 0 = padding) is
@@ -820,10 +842,14 @@ hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
                        X, ldx, bt, (long long)ldb, YA, lda, ivgk, vgpc,
                        (const IstEntry *)c->ist.p, lo0, lo1, lo2, n0, n1, n2, h, o,
                        (long long)ldh, flags & EXH_ACCUMULATE);
-  else  // overlapsetup alone: bt and H are not read
-    hipLaunchKernelGGL(ho_kernel<false>, dim3((unsigned)npairs), dim3(NTH), 0, s, ngp, c->Ktot,
-                       X, ldx, X, ldx, YA, lda, ivgk, vgpc, (const IstEntry *)c->ist.p, lo0,
-                       lo1, lo2, n0, n1, n2, h, o, (long long)ldh, flags & EXH_ACCUMULATE);
+  else {  // overlapsetup alone: bt and H are not read
+    constexpr int TO = 128, TKO = 8;
+    const long long ntO = (ngp + TO - 1) / TO;
+    hipLaunchKernelGGL((o_kernel<TO, TKO>), dim3((unsigned)(ntO * (ntO + 1) / 2)),
+                       dim3(HoShape<TO, TO, TKO>::NTH), 0, s, ngp, c->Ktot, X, ldx, YA, lda,
+                       ivgk, vgpc, (const IstEntry *)c->ist.p, lo0, lo1, lo2, n0, n1, n2, o,
+                       (long long)ldh, flags & EXH_ACCUMULATE);
+  }
   return hipGetLastError();
 }
 
