@@ -550,8 +550,9 @@ def vnlmat_sample(W, nstfv=400, reps=3, cpu=True):
 def kinetic_sample(W, nstfv=400, reps=3, cpu=True, cpu_states=4):
     """kinetic_energy.f90 on k-point 0 of the workload ('none' relativity):
     muffin-tin part (kinetic radial integrals, A^T x per atom, quadratic
-    forms) and interstitial part (3 directions x a batched FFT pair over all
-    states), inputs resident in HBM; random states."""
+    forms) and interstitial part (the cyclic convolution with the step
+    function as one dense G+k matrix from one FFT, one GEMM over the three
+    directions), inputs resident in HBM; random states."""
     import dataclasses
     import torch
     from exciting_amd import kinetic, synth
@@ -574,8 +575,10 @@ def kinetic_sample(W, nstfv=400, reps=3, cpu=True, cpu_states=4):
         kinetic.kinetic_energy(W.ctx, d, X, cfunir, lofr, True, veffmt, 8, plan=plan)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / reps * 1e3
+    ngk = d.ngk
     res = {"ms": ms, "nstfv": nstfv, "n": n,
-           "ffts": 6 * nstfv, "grid": [int(x) for x in s.intgv[:, 1] - s.intgv[:, 0] + 1],
+           "interstitial_gemm_flops": 8.0 * ngk * ngk * 3 * nstfv,
+           "grid": [int(x) for x in s.intgv[:, 1] - s.intgv[:, 0] + 1],
            "workload": f"k-point 0 of {W.spec.name}: n = {n}, nstfv = {nstfv}"}
     if cpu:
         import oracle

@@ -16,22 +16,17 @@ Per k-point and state x (evecfv(:, ist)) the reference adds
 Here the integrals come from libexh (exh_mt_kinetic_device, the same
 kernel as hmlint's kinetic block), zm is one library GEMM per atom batch,
 the per-atom quadratic forms are batched matrix products, and the
-interstitial part is one batched 3-D FFT pair over all states per direction
-(torch.fft, i.e. hipFFT), all in FP64 with the states resident in HBM.
+interstitial part is the same cyclic convolution written as one dense
+matrix F(i, j) = c((G_i - G_j) mod box) over the G+k set (c the forward
+transform of the grid function, one FFT), applied to the three directions'
+weighted states in one library GEMM; all in FP64 with the states resident
+in HBM.
 """
 from __future__ import annotations
 
 import numpy as np
 
 ALPHA = 1.0 / 137.03599911
-
-
-def _grid_index(ivg_cols, ngrid):
-    """genigfft (gengvec.f90:129-167), 0-based, for integer G columns."""
-    n = np.asarray(ngrid, dtype=np.int64)[:, None]
-    iv = np.asarray(ivg_cols, dtype=np.int64)
-    j = np.where(iv >= 0, iv, n + iv)
-    return j[2] * n[0, 0] * n[1, 0] + j[1] * n[0, 0] + j[0]
 
 
 class KineticPlan:
@@ -69,6 +64,29 @@ class KineticPlan:
             self.rowlm.append(lmr)
         self.species = np.repeat(np.arange(s.nspecies), s.natoms)
         self.lobase = np.concatenate([[0], np.cumsum([s.losize[i] for i in self.species])])[:-1]
+        self.losize = [int(s.losize[i]) for i in range(s.nspecies)]
+        self._dev = {}
+
+    def on(self, dev):
+        """Per species, on device dev (built once): the atoms, the f-index of
+        every row (clamped), the mask of row pairs with equal (l, m), and the
+        rows of x_lo of each atom's local orbitals."""
+        import torch
+        key = str(dev)
+        if key not in self._dev:
+            out = []
+            for is_ in range(len(self.rowf)):
+                at = np.nonzero(self.species == is_)[0]
+                f = torch.as_tensor(self.rowf[is_], device=dev)
+                lmr = torch.as_tensor(self.rowlm[is_], device=dev)
+                ok = f >= 0
+                same = ok[:, None] & ok[None, :] & (lmr[:, None] == lmr[None, :])
+                los = self.losize[is_]
+                lorows = (self.lobase[at][:, None] + np.arange(los)[None, :]).astype(np.int64)
+                out.append((torch.as_tensor(at, device=dev), f.clamp(min=0), same, los,
+                            torch.as_tensor(lorows, device=dev)))
+            self._dev[key] = out
+        return self._dev[key]
 
 
 def kinetic_energy(ctx, d, evec, cfunir, lofr, valence_rel_none=True, veffmt=None, lmaxvr=0,
@@ -100,39 +118,37 @@ def kinetic_energy(ctx, d, evec, cfunir, lofr, valence_rel_none=True, veffmt=Non
     u[:, :lmm * p] = zm
     xlo = evec[:, ngp:].transpose(0, 1)                                # (nlotot, nst)
     M = torch.zeros((natm, plan.dim, plan.dim), dtype=torch.float64, device=dev)
-    for ias in range(natm):
-        is_ = plan.species[ias]
-        los = int(s.losize[is_])
+    zero = torch.zeros((), device=dev, dtype=torch.float64)
+    for at, fi, same, los, lorows in plan.on(dev):     # one pass per species
         if los:
-            b = int(plan.lobase[ias])
-            u[ias, lmm * p:lmm * p + los] = xlo[b:b + los]
-        f = torch.as_tensor(plan.rowf[is_], device=dev)
-        lmr = torch.as_tensor(plan.rowlm[is_], device=dev)
-        ok = f >= 0
-        same = ok[:, None] & ok[None, :] & (lmr[:, None] == lmr[None, :])
-        fi = f.clamp(min=0)
-        M[ias] = torch.where(same, T[ias][fi[:, None], fi[None, :]], torch.zeros((), device=dev,
-                                                                               dtype=torch.float64))
+            u[at, lmm * p:lmm * p + los] = xlo[lorows]
+        M[at] = torch.where(same[None], T[at][:, fi[:, None], fi[None, :]], zero)
     Mu = torch.matmul(M.to(torch.complex128), u)
     e_mt = (u.conj() * Mu).sum(dim=(0, 1)).real
     # interstitial part
     ngrid = tuple(int(x) for x in s.intgv[:, 1] - s.intgv[:, 0] + 1)
     igp = d.igpig.cpu().numpy()[:ngp].astype(np.int64) - 1
-    grid_idx = torch.as_tensor(_grid_index(s.ivg[:, igp], ngrid), device=dev)
     cfun = cfunir
     if not valence_rel_none:
         cfun = cfunir / (1.0 - 0.5 * ALPHA * ALPHA * veffir)
     cf = cfun.view(ngrid[2], ngrid[1], ngrid[0])
     vg = d.vgpc.view(-1, 3)[:ngp]                                      # (ngp, 3)
-    ntot = ngrid[0] * ngrid[1] * ngrid[2]
-    e_ist = torch.zeros(nst, dtype=torch.float64, device=dev)
-    for dxyz in range(3):
-        z = torch.zeros((nst, ntot), dtype=torch.complex128, device=dev)
-        z[:, grid_idx] = (X * vg[:, dxyz:dxyz + 1]).transpose(0, 1)
-        z = z.view(nst, ngrid[2], ngrid[1], ngrid[0])
-        z = torch.fft.ifftn(z, dim=(1, 2, 3), norm="forward")         # unscaled backward
-        z = z * cf
-        z = torch.fft.fftn(z, dim=(1, 2, 3), norm="forward")          # forward with 1/N
-        y = z.reshape(nst, ntot)[:, grid_idx] * vg[:, dxyz]
-        e_ist += 0.5 * (evec[:, :ngp].conj() * y).sum(dim=1).real
+    # The reference's backward FFT, product with cf on the grid and forward
+    # FFT (scaled 1/N) is a cyclic convolution: at G_i it gives
+    # sum_j c((G_i - G_j) mod box) z(G_j), with c the forward transform of
+    # cf.  On the G+k set that is the dense matrix F(i, j) = c((G_i - G_j)
+    # mod box), exact whatever the G differences, so one transform of cf and
+    # one matrix product over the three directions replace 2 x 3 x nstfv
+    # grid transforms.
+    c = torch.fft.fftn(cf.to(torch.complex128), norm="forward").reshape(-1)
+    ivk = torch.as_tensor(np.asarray(s.ivg)[:, igp], device=dev)          # (3, ngp)
+    box = torch.zeros((ngp, ngp), dtype=torch.int64, device=dev)
+    for a in (2, 1, 0):                       # first grid index fastest
+        da = torch.remainder(ivk[a][:, None] - ivk[a][None, :], ngrid[a])
+        box = box * ngrid[a] + da
+    F = c[box]                                                          # (ngp, ngp)
+    del box
+    W = torch.cat([X * vg[:, dd:dd + 1] for dd in range(3)], dim=1)     # (ngp, 3 nst)
+    Y = F @ W
+    e_ist = 0.5 * (W.conj() * Y).real.reshape(ngp, 3, nst).sum(dim=(0, 1))
     return e_mt + e_ist
