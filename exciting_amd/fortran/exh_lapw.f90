@@ -56,7 +56,9 @@ module exh_lapw
             exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
             exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device, &
-            exh_mt_pot, exh_mt_pot_device, exh_mt_so, exh_mt_so_device
+            exh_mt_pot, exh_mt_pot_device, exh_mt_so, exh_mt_so_device, exh_stream, &
+            exh_match_device, exh_setup_device, exh_batch_plan_count, exh_batch_plan_info, &
+            exh_profile_enable, exh_profile_read, exh_profile_reset
 
   interface
     integer(c_int) function exh_create(device, ctx) bind(C, name="exh_create")
@@ -221,6 +223,61 @@ module exh_lapw
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, stream, pot, lofr, aa, loa, lolo
       integer(c_int), value :: lmaxvr, flags
+    end function
+
+    ! The context's HIP stream (hipStream_t), for ordering device-pointer calls.
+    type(c_ptr) function exh_stream(ctx) bind(C, name="exh_stream")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    ! match and the setup on device pointers (c_loc of device arrays), enqueued
+    ! on stream (c_null_ptr = the context's stream).
+    integer(c_int) function exh_match_device(ctx, stream, ngp, gpc, tpgpc, sfacgp, apwalm) &
+        bind(C, name="exh_match_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, gpc, tpgpc, sfacgp, apwalm
+      integer(c_int), value :: ngp
+    end function
+
+    integer(c_int) function exh_setup_device(ctx, stream, ngp, rank, apwalm, igpig, vgpc, h, o, &
+        flags) bind(C, name="exh_setup_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, apwalm, igpig, vgpc, h, o
+      integer(c_int), value :: ngp, rank, flags
+    end function
+
+    ! Batch plans: their number and each one's shape.
+    integer(c_int) function exh_batch_plan_count() bind(C, name="exh_batch_plan_count")
+      import :: c_int
+    end function
+
+    integer(c_int) function exh_batch_plan_info(plan, tile_rows, tile_cols, k_slab, persistent, &
+        xcd_order) bind(C, name="exh_batch_plan_info")
+      import :: c_int
+      integer(c_int), value :: plan
+      integer(c_int), intent(out) :: tile_rows, tile_cols, k_slab, persistent, xcd_order
+    end function
+
+    ! Per-kernel timing with HIP events (kernel ids EXH_K_* of include/exh.h).
+    integer(c_int) function exh_profile_enable(ctx, on) bind(C, name="exh_profile_enable")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+
+    integer(c_int) function exh_profile_read(ctx, kernel, total_ms, launches) &
+        bind(C, name="exh_profile_read")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kernel
+      real(c_double), intent(out) :: total_ms
+      integer(c_int), intent(out) :: launches
+    end function
+
+    integer(c_int) function exh_profile_reset(ctx) bind(C, name="exh_profile_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
 
     ! alpha, beta and ab tables (aa, loa, lolo, alo each); rel_level 1 = level_zora

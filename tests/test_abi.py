@@ -30,6 +30,20 @@ def test_library_exports_all_symbols():
         assert hasattr(L, n), n
 
 
+def test_fortran_module_binds_every_entry_point():
+    """The ISO_C_BINDING module declares an interface for every C entry
+    point, under the same name, and makes it public."""
+    text = open(os.path.join(ROOT, "exciting_amd", "fortran", "exh_lapw.f90")).read()
+    bound = set(re.findall(r'bind\(C, name="(exh_[a-z_0-9]+)"\)', text))
+    assert bound == set(declared())
+    public = re.search(r"^\s*public\s*::(.*?)\n\s*\n", text, flags=re.S | re.M).group(1)
+    public = set(re.findall(r"exh_[a-z_0-9]+", public))
+    # exh_error returns a C string: the module keeps that interface private
+    # and exports exh_error_message, which copies it into a Fortran string
+    wrapped = {"exh_error": "exh_error_message"}
+    assert {wrapped.get(n, n) for n in declared()} <= public
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(lapw, "_lib", None)
     monkeypatch.setattr(lapw, "LIB_PATH", str(tmp_path / "nope.so"))
