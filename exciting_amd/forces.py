@@ -63,7 +63,7 @@ def density_matrix(evecsv, occsv, nstfv, nspinor=2, spiral_ispn=None):
     return dm
 
 
-def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None):
+def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None, tables=None):
     """forcek.f90 for one k-point: a (3, natmtot) float64 CUDA tensor to add
     to forceibs.  d: lapw.DeviceKPoint (match is run here, as the reference
     does); evec (nstfv, nmat) complex128 (row = state); evalfv, occ (nstfv)
@@ -74,7 +74,13 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None):
     (occ is then not read): the state sum sum_ab Re ffv_ab dm_ba becomes
     -2 sum_i g_i Im(conj(Z_H) Y_H) + sum_i g_i Im(conj(Z_O) Y_O) summed over
     states, with Z_H = X dm and Z_O = X (dm E + E dm), E = diag(evalfv),
-    which for dm = diag(occ) is the first-variational sum."""
+    which for dm = diag(occ) is the first-variational sum.
+
+    tables: optional (aa, loa, salo) CUDA tensors holding the muffin-tin
+    tables in the reference's Fortran layouts, flattened (aa(maxaa, maxaa,
+    natmtot), loa(maxnlo, maxaa, natmtot), salo = oalo + h1loa
+    (apwordmax, nlomax, natmtot)), e.g. the outputs of hmlint on the device;
+    without them the context's loaded host tables are uploaded."""
     import torch
     s = ctx.sys
     dev = evec.device
@@ -103,9 +109,18 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None):
     dp = 0.5 * (vg @ vg.transpose(0, 1))                                 # dpij
     A_all = d.apwalm.view(natm, lmm * p, s.ngkmax)
     rows = _rows(s)
-    aa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.aa, 2, 0)), device=dev)
-    loa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.loa, 2, 0)), device=dev)
-    salo = np.asarray(s.oalo) + (np.asarray(s.h1loa) if s.h1loa is not None else 0.0)
+    mnlo = max(s.maxnlo, 1)
+    if tables is not None:
+        aa_t, loa_t, salo_t = tables
+        aa = aa_t.view(natm, s.maxaa, s.maxaa).transpose(1, 2)         # [ias, i, j]
+        loa = loa_t.view(natm, s.maxaa, mnlo).transpose(1, 2)          # [ias, ilo row, j]
+        salo_d = salo_t.view(natm, max(s.nlomax, 1), p)                  # [ias, ilo, io]
+    else:
+        aa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.aa, 2, 0)), device=dev)
+        loa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.loa, 2, 0)), device=dev)
+        salo = np.asarray(s.oalo) + (np.asarray(s.h1loa) if s.h1loa is not None else 0.0)
+        salo_d = torch.as_tensor(np.ascontiguousarray(np.moveaxis(salo, (0, 1, 2), (2, 1, 0))),
+                                 device=dev)                            # [ias, ilo, io]
     YH = torch.zeros((natm, ngk, nst), dtype=torch.complex128, device=dev)
     YO = torch.zeros_like(YH)
     species = np.repeat(np.arange(s.nspecies), s.natoms)
@@ -133,7 +148,7 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None):
             for c, (lm, ilo) in enumerate(lo_cols):
                 l = s.lorbl[ilo, is_]
                 for io in range(s.apword[l, is_]):
-                    sv = torch.as_tensor(salo[io, ilo, at], device=dev)
+                    sv = salo_d[at_t, ilo, io]
                     OGL[:, :, c] += A_all[at_t, lm * p + io, :ngk] * sv[:, None]
             YO[at_t] += torch.matmul(OGL.conj(), XL)
         # step-function term of the species: F(i, j) = -ffacg(G_i - G_j)

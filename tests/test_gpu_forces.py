@@ -39,6 +39,12 @@ def test_forcek_matches_the_oracle(name, nst):
         err = np.abs(got - want).max() / np.abs(want).max()
         print(f"  {name}: max |dF| / max |F| = {err:.1e}")
         assert err <= 1e-10
+        # the same tables handed over as device tensors (Fortran layouts)
+        flat = lambda a: t(np.asfortranarray(a).ravel(order="F"))
+        salo = np.asarray(s.oalo) + (np.asarray(s.h1loa) if s.h1loa is not None else 0.0)
+        got_d = forces.forcek(ctx, d, t(X.T), t(e), t(occ), 0.5, t(ff), t(sg),
+                              tables=(flat(s.aa), flat(s.loa), flat(salo))).cpu().numpy()
+        assert np.abs(got_d - got).max() <= 1e-13 * np.abs(got).max()
     finally:
         ctx.close()
 
