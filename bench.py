@@ -289,7 +289,9 @@ def serving_sample(args):
     specs = synth.serving_cells(args.serve_kmax)
     cells = {name: synth.build(spec, kpoints=ks) for name, (spec, ks) in specs.items()}
     make, close, execute = serve.gpu_executor(0, max_solves=args.serve_solves)
-    budget = serve.free_hbm(0) - (48 << 30)
+    from exciting_amd import solve as _solve
+    budget = (serve.free_hbm(0) - (48 << 30)
+              - args.serve_solves * _solve.WORKSPACE_BYTES)   # the solvers' hipBLAS workspaces
     srv = serve.Server(cells, execute, make, close, budget, max_inflight=8,
                        on_finish=lambda r: log(f"serve: request {r.rid} {r.cell} "
                                                f"nk={len(r.kpoints)} nmod={r.nmod} {r.status} "

@@ -25,9 +25,11 @@ Scheduling:
     scratch) in its own host thread, so requests that arrive while others are
     on the GPU start at once and their kernels overlap on the device.  The
     eigensolves (PyTorch's ROCm solver, whose tridiagonal reduction is a long
-    chain of small kernels) also run concurrently, on the requests' own
-    streams, up to a bound; PyTorch keeps its solver and BLAS handles per
-    thread and stream.
+    chain of small kernels) run concurrently on a fixed set of solver
+    threads, each with its own stream: PyTorch keeps a 2 GiB hipBLAS
+    workspace for every (thread's handle, stream) pair that ever runs a BLAS
+    call, so solving on per-request threads and per-context streams would
+    add one such workspace per new pair for the life of the process.
   * Results do not depend on what else runs: every setup kernel has a fixed
     reduction order and owns its output tiles (no atomics on H or O), and
     the eigensolve sees bitwise identical inputs.
@@ -41,6 +43,7 @@ returns.
 from __future__ import annotations
 
 import argparse
+import concurrent.futures
 import dataclasses
 import hashlib
 import json
@@ -278,7 +281,33 @@ def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
     import torch
     from . import lapw
     dev = f"cuda:{device}"
-    solve_slots = threading.BoundedSemaphore(max(1, max_solves))
+    # long-lived solver threads, one stream each: at most max_solves hipBLAS
+    # workspaces, however many contexts and request threads come and go
+    solvers = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, max_solves),
+                                                    thread_name_prefix="exh-solve")
+    local = threading.local()
+
+    def solve_on_worker(ready, H, O, n, nst):
+        st = getattr(local, "stream", None)
+        if st is None:
+            st = local.stream = torch.cuda.Stream(device=dev)
+        st.wait_event(ready)                   # the setup of H and O is done
+        with torch.cuda.stream(st):
+            try:
+                w = _solve.geneig(H, O, n, nst)
+            except torch.cuda.OutOfMemoryError:
+                # freed temporaries of other solves may sit in PyTorch's
+                # cache in pieces too small for this one
+                torch.cuda.synchronize(device)
+                torch.cuda.empty_cache()
+                w = _solve.geneig(H, O, n, nst)
+            out = w.cpu().numpy()              # waits for the solve
+            del w
+        # the solve's n x n temporaries are free now; hand them back so that
+        # they do not sit in PyTorch's cache in pieces that the admission
+        # budget cannot see and hipMalloc cannot use
+        torch.cuda.empty_cache()
+        return out
 
     def make_context(sys_):
         ctx = lapw.Context(device).load(sys_)
@@ -329,21 +358,12 @@ def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
                     h = hashlib.sha1(ctx.H[:nn].cpu().numpy().tobytes())
                     h.update(ctx.O[:nn].cpu().numpy().tobytes())
                     dg.append(h.hexdigest())
-                with solve_slots:
-                    try:
-                        w = _solve.geneig(ctx.H, ctx.O, d.rank, min(req.nstfv, d.rank))
-                    except torch.cuda.OutOfMemoryError:
-                        # freed temporaries of other solves may sit in
-                        # PyTorch's cache in pieces too small for this one
-                        torch.cuda.synchronize(device)
-                        torch.cuda.empty_cache()
-                        w = _solve.geneig(ctx.H, ctx.O, d.rank, min(req.nstfv, d.rank))
-                    out.append(w.cpu().numpy())
-                    del w
-                    # the solve's n x n temporaries are free now; hand them back
-                    # so that they do not sit in PyTorch's cache in pieces that
-                    # the admission budget cannot see and hipMalloc cannot use
-                    torch.cuda.empty_cache()
+                ready = torch.cuda.Event()
+                ready.record(ctx.tstream)
+                # waits for the eigenvalues before the next k-point's setup
+                # overwrites H and O
+                out.append(solvers.submit(solve_on_worker, ready, ctx.H, ctx.O, d.rank,
+                                          min(req.nstfv, d.rank)).result())
         if digests:
             req.digests = dg
         return out
@@ -383,6 +403,14 @@ def poisson_requests(n: int, rate: float, cells: Dict[str, object], seed: int = 
                            nmod=nmod if rng.random() < split_fraction else 0,
                            nstfv=nstfv, arrival=t))
     return out
+
+
+def _torch_allocated(peak: bool = False) -> int:
+    """Bytes PyTorch's allocator holds in live blocks on GPU 0 (now, or the
+    peak so far): contexts' H, O and apwalm, solve temporaries, and the
+    hipBLAS workspaces of the solver threads."""
+    import torch
+    return int(torch.cuda.max_memory_allocated(0) if peak else torch.cuda.memory_allocated(0))
 
 
 def summarize(reqs: List[Request], wall: float) -> dict:
@@ -436,7 +464,8 @@ def main(argv=None) -> int:
           + ", ".join(f"{k} ngk<={c.ngkmax} nk={len(c.kpts)}" for k, c in cells.items()),
           file=sys.stderr, flush=True)
     make, close, execute = gpu_executor(0, max_solves=args.max_solves)
-    budget = free_hbm(0) - int(args.reserve_gib * 2**30)
+    budget = (free_hbm(0) - int(args.reserve_gib * 2**30)
+              - args.max_solves * _solve.WORKSPACE_BYTES)   # the solvers' hipBLAS workspaces
     def report(r):
         print(f"[serve] request {r.rid} {r.cell} nk={len(r.kpoints)} nmod={r.nmod} "
               f"{r.status} latency {r.latency:.3f} s {r.error}", file=sys.stderr, flush=True)
@@ -464,6 +493,8 @@ def main(argv=None) -> int:
                 "split_fraction": args.split_fraction, "nmod": args.nmod,
                 "max_inflight": args.max_inflight, "max_solves": args.max_solves,
                 "budget_gib": budget / 2**30,
+                "torch_allocated_gib_at_end": _torch_allocated() / 2**30,
+                "torch_peak_allocated_gib": _torch_allocated(peak=True) / 2**30,
                 "peak_inflight": srv.peak_inflight, "contexts_created": srv.pool.created,
                 "contexts_evicted": srv.pool.evicted,
                 "peak_context_gib": srv.peak_bytes / 2**30})

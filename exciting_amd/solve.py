@@ -17,6 +17,9 @@ from __future__ import annotations
 import os
 
 WORKSPACE = ":1048576:2"   # 2 hipBLAS workspaces of 1 GiB
+# PyTorch allocates this much per (hipBLAS handle, stream) pair that runs a
+# BLAS call, and keeps it until the process ends
+WORKSPACE_BYTES = 2 * 1048576 * 1024
 
 
 def configure() -> None:
