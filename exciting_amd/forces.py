@@ -114,7 +114,8 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None, tables=None):
         aa_t, loa_t, salo_t = tables
         aa = aa_t.view(natm, s.maxaa, s.maxaa).transpose(1, 2)         # [ias, i, j]
         loa = loa_t.view(natm, s.maxaa, mnlo).transpose(1, 2)          # [ias, ilo row, j]
-        salo_d = salo_t.view(natm, max(s.nlomax, 1), p)                  # [ias, ilo, io]
+        # [ias, ilo, io]; never read without local orbitals
+        salo_d = salo_t.view(natm, s.nlomax, p) if s.nlomax > 0 else None
     else:
         aa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.aa, 2, 0)), device=dev)
         loa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.loa, 2, 0)), device=dev)
