@@ -294,13 +294,13 @@ def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
         st.wait_event(ready)                   # the setup of H and O is done
         with torch.cuda.stream(st):
             try:
-                w = _solve.geneig(H, O, n, nst)
+                w = _solve.geneig(H, O, n, nst, method="auto")
             except torch.cuda.OutOfMemoryError:
                 # freed temporaries of other solves may sit in PyTorch's
                 # cache in pieces too small for this one
                 torch.cuda.synchronize(device)
                 torch.cuda.empty_cache()
-                w = _solve.geneig(H, O, n, nst)
+                w = _solve.geneig(H, O, n, nst, method="auto")
             out = w.cpu().numpy()              # waits for the solve
             del w
         # the solve's n x n temporaries are free now; hand them back so that

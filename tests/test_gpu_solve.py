@@ -105,3 +105,24 @@ def test_iora_renormalisation_of_large_components():
     np.testing.assert_allclose(np.real(np.diag(Y.conj().T @ O0m @ Y)), 1.0, atol=1e-12)
     # the h1 terms make the two overlaps differ, so the rescaling is not trivial
     assert np.abs(np.abs(np.diag(zm2)) - 1).max() > 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,nst", [("sc8_k0", 40), ("sc8_k0", 8), ("fcc2", 20)])
+def test_subspace_eigenvalues_match_zhegvx(name, nst):
+    """The Chebyshev-filtered subspace iteration (method "subspace") against
+    LAPACK's ZHEGVX on the same H and O: the lowest nst eigenvalues within
+    1e-10 Ha, and the same result on a second call (deterministic start)."""
+    import torch
+    s = _cases.system(name)
+    print(f"[gpu-solve-subspace] case {name} nst {nst}", flush=True)
+    H, O, n = device_setup(s)
+    w = solve.geneig(H, O, n, nst, method="subspace").cpu().numpy()
+    w2 = solve.geneig(H, O, n, nst, method="subspace").cpu().numpy()
+    wf = solve.geneig(H, O, n, nst, method="full").cpu().numpy()
+    torch.cuda.synchronize()
+    wl = sl.eigh(host(H, n), host(O, n), lower=False, subset_by_index=[0, nst - 1],
+                 eigvals_only=True, driver="gvx")
+    print(f"  n {n}: max |dE| subspace {np.abs(w - wl).max():.1e}, full {np.abs(wf - wl).max():.1e}")
+    assert np.abs(w - wl).max() <= 1e-10
+    assert np.array_equal(w, w2)
