@@ -709,9 +709,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(k-points would shard across GPUs with no exchange; see DESIGN.md). "
-              "Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(k-points would shard across GPUs with no exchange but a final "
+              "allgather of eigenvalues). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from exciting_amd import solve
