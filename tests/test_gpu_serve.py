@@ -94,3 +94,29 @@ def test_served_eigenvalues_match_the_oracle(served):
             tol = 1e-8 if r.nmod == 0 else 1e-6
             print(f"  request {r.rid} {r.cell} k{ik} nmod={r.nmod}: max|dE| {de:.1e} Ha")
             assert de <= tol
+
+
+@pytest.mark.gpu
+def test_solver_workspaces_do_not_grow_with_contexts():
+    """PyTorch keeps a 2 GiB hipBLAS workspace per (thread, stream) pair
+    that runs a BLAS call.  The executor solves on its own fixed threads and
+    streams, so building, using and closing one context after another (each
+    with its own stream, each request on its own thread) adds no memory
+    once the first solve has made its workspace."""
+    import threading
+    import torch
+    cell = _cases.system("fcc2")
+    make, close, execute = serve.gpu_executor(0, max_solves=2)
+    held = []
+    for i in range(5):
+        print(f"[gpu-serve] context {i}", flush=True)
+        ctx = make(cell)
+        req = serve.Request(rid=i, cell="fcc2", kpoints=[0], nmod=0, nstfv=20)
+        t = threading.Thread(target=lambda: execute(req, ctx))
+        t.start()
+        t.join()
+        close(ctx)
+        torch.cuda.synchronize()
+        held.append(torch.cuda.memory_allocated(0))
+    print(f"  PyTorch allocated after each context: {[h / 2**30 for h in held]} GiB")
+    assert held[-1] <= held[0]
