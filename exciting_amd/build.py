@@ -68,6 +68,10 @@ def build(verbose: bool = False) -> str:
 
 FORTRAN = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(FORTRAN, "exh_driver")
+SHIM_DRIVER = os.path.join(FORTRAN, "exh_shim_driver")
+# the binding module, the caller's matrix types (standalone build only), the
+# integration's state module and the reference-name shims
+MODULES = ("exh_lapw.f90", "fvsystem.f90", "exh_state.f90", "exh_shims.f90")
 
 
 def flang() -> str:
@@ -78,25 +82,30 @@ def flang() -> str:
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """The ISO_C_BINDING module and its driver program, linked to libexh.so.
-    Skipped (returns "") when no Fortran compiler is installed."""
+    """The ISO_C_BINDING module, the drop-in shims (match,
+    hamiltonandoverlapsetup, overlapsetup under the reference's names) and
+    the two driver programs, linked to libexh.so.  Skipped (returns "") when
+    no Fortran compiler is installed."""
     fc = flang()
     if not fc:
         return ""
     bdir = os.path.join(FORTRAN, "_build")
     os.makedirs(bdir, exist_ok=True)
-    objs = []
-    for src in ("exh_lapw.f90", "exh_driver.f90"):
+    objs = {}
+    for src in MODULES + ("exh_driver.f90", "exh_shim_driver.f90"):
         obj = os.path.join(bdir, src.replace(".f90", ".o"))
         cmd = [fc, "-O2", "-c", os.path.join(FORTRAN, src), "-module-dir", bdir, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
-    tmp = DRIVER + ".tmp"
-    cmd = [fc, "-o", tmp, *objs, "-L", HERE, "-lexh", "-Wl,-rpath,$ORIGIN/.."]
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, DRIVER)
+        objs[src] = obj
+    for prog, main, deps in ((DRIVER, "exh_driver.f90", ("exh_lapw.f90",)),
+                             (SHIM_DRIVER, "exh_shim_driver.f90", MODULES)):
+        tmp = prog + ".tmp"
+        cmd = [fc, "-o", tmp, objs[main], *[objs[d] for d in deps], "-L", HERE, "-lexh",
+               "-Wl,-rpath,$ORIGIN/.."]
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, prog)
     shutil.rmtree(bdir, ignore_errors=True)
     return DRIVER
 

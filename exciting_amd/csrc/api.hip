@@ -262,8 +262,12 @@ int exh_create(int device, exh_context **out) {
         ncu > 0)
       c->ncu = ncu;
   }
+  // created here so that no entry point creates it later (mark_done)
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     int rc = (int)e;
+    if (c->done_ev) (void)hipEventDestroy(c->done_ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return rc > 0 ? rc : 1;
   }
@@ -319,6 +323,8 @@ int exh_set_basis(exh_context *c, int nspecies, const int *natoms, int lmaxapw, 
   if (ngkmax <= 0) return fail(c, EXH_EINVAL, "set_basis: ngkmax <= 0");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
   c->have_basis = c->have_radial = c->have_mt = false;
+  c->uniform_apword = true;
+  c->apw_host = nullptr;
   c->nspecies = nspecies;
   c->lmaxapw = lmaxapw;
   c->lmaxmat = lmaxmat;
@@ -431,12 +437,14 @@ int exh_set_radial(exh_context *c, double omega, const double *rmt, const int *n
   if (!c->have_basis) return fail(c, EXH_ESTATE, "set_radial: call exh_set_basis first");
   if (!(omega > 0.0) || !rmt || !nrmt || !spr || !apwfr || nrmtmax <= 0)
     return fail(c, EXH_EINVAL, "set_radial: bad argument");
+  for (int is = 0; is < c->nspecies; is++)
+    if (nrmt[is] > nrmtmax || !(rmt[is] > 0.0)) return fail(c, EXH_EINVAL, "set_radial: nrmt/rmt");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
+  // match and hmlint refuse to run until every radial table below is rebuilt
+  c->have_radial = false;
   c->omega = omega;
   c->rmt.assign(rmt, rmt + c->nspecies);
   c->nrmt_h.assign(nrmt, nrmt + c->nspecies);
-  for (int is = 0; is < c->nspecies; is++)
-    if (nrmt[is] > nrmtmax || !(rmt[is] > 0.0)) return fail(c, EXH_EINVAL, "set_radial: nrmt/rmt");
   c->nrmtmax = nrmtmax;
   c->spr_h.assign(spr, spr + (size_t)nrmtmax * c->nspecies);
   c->apwfr_h.assign(apwfr, apwfr + (size_t)nrmtmax * 2 * c->apwordmax * (c->lmaxapw + 1) * c->natmtot);
@@ -568,12 +576,21 @@ int exh_match_device(exh_context *c, void *stream, int ngp, const double *gpc,
   return e == hipSuccess ? 0 : hip_fail(c, e, "match kernel");
 }
 
+// Scratch of one single-k setup: the K-major operands (aa.A)^T, ((1+h1)A)^T
+// and the packed A^T (Ktot x ldb each, only those the context needs), and the
+// G+k -> box index map (ngkmax entries).
+struct SetupScratch {
+  double2 *bt = nullptr, *a2t = nullptr, *at = nullptr;
+  int4 *ivgk = nullptr;
+};
+
 // The single-k setup.  o_only: overlapsetup.f90 alone -- O accumulated and
 // mirrored, H (may be null) only mirrored (hamiltonoverlapocopy_UL is called
-// from overlapsetup.f90:164-166 for both matrices).
-static int setup_impl(exh_context *c, void *stream, int ngp, int rank, const exh_z *apwalm,
-                      const int *igpig, const double *vgpc, exh_z *h, exh_z *o, int flags,
-                      bool o_only) {
+// from overlapsetup.f90:164-166 for both matrices).  check_setup validates the
+// arguments; setup_launch enqueues the kernels on scratch the caller provides.
+static int check_setup(exh_context *c, int ngp, int rank, const exh_z *apwalm, const int *igpig,
+                       const double *vgpc, const exh_z *h, const exh_z *o, int flags,
+                       bool o_only) {
   EXH_CHECK_CTX(c);
   if (!c->have_basis || !c->have_mt || !c->have_gvec || !c->have_ist)
     return fail(c, EXH_ESTATE, "setup: basis, muffin-tin tables and interstitial data must be set");
@@ -589,58 +606,88 @@ static int setup_impl(exh_context *c, void *stream, int ngp, int rank, const exh
     return fail(c, EXH_ENOTSUP, "setup: split precision with h1aa (iora*) is not implemented");
   if (nmod && o_only) return fail(c, EXH_ENOTSUP, "overlapsetup: split precision is not implemented");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, EXH_ESTATE, "hipSetDevice");
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return 0;
+}
+
+// Enqueue the FP64 kernels (and, with EXH_SPLIT, the split-precision stages,
+// whose scratch the caller has prepared with oz_prepare).  `record` brackets
+// each launch with the profiling events.
+static int setup_launch(exh_context *c, hipStream_t s, int ngp, int rank, const exh_z *apwalm,
+                        const int *igpig, const double *vgpc, exh_z *h, exh_z *o, int flags,
+                        bool o_only, const SetupScratch &w, bool record) {
+  const int nmod = (flags >> 8) & 0xff;
   const int ldb = (c->ngkmax + 63) / 64 * 64;
   exh_z *hw = o_only ? nullptr : h;  // H the kernels write
-  int rc = o_only ? 0 : ensure(c, c->bt, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
-  if (!rc && c->have_h1aa) rc = ensure(c, c->a2t, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
-  if (!rc && !c->contig) rc = ensure(c, c->at, (size_t)std::max(c->Ktot, 1) * ldb * sizeof(double2));
-  if (!rc) rc = ensure(c, c->ivgk, (size_t)std::max(c->ngkmax, 1) * sizeof(int4));
-  if (rc) return rc;
+  auto run = [&](int k, auto &&launch) {
+    return record ? timed(c, k, s, launch) : launch();
+  };
   const double2 *A = (const double2 *)apwalm;
-  hipError_t e = timed(c, EXH_K_IVGK, s, [&] {
-    return launch_ivgk(s, ngp, igpig, (const int *)c->ivg.p, c->ngrtot, (int4 *)c->ivgk.p);
+  hipError_t e = run(EXH_K_IVGK, [&] {
+    return launch_ivgk(s, ngp, igpig, (const int *)c->ivg.p, c->ngrtot, w.ivgk);
   });
   if (e == hipSuccess && !c->contig)
-    e = timed(c, EXH_K_PACK, s, [&] { return launch_pack(s, c, ngp, A); });
+    e = run(EXH_K_PACK, [&] { return launch_pack(s, c, ngp, A, w.at); });
   if (e == hipSuccess && !o_only)
-    e = timed(c, EXH_K_BT, s, [&] { return launch_bt(s, c, ngp, A, (double2 *)c->bt.p, ldb); });
+    e = run(EXH_K_BT, [&] { return launch_bt(s, c, ngp, A, w.at, w.bt, ldb); });
   if (e == hipSuccess && c->have_h1aa)
-    e = timed(c, EXH_K_A2T, s, [&] { return launch_a2t(s, c, ngp, A, (double2 *)c->a2t.p, ldb); });
+    e = run(EXH_K_A2T, [&] { return launch_a2t(s, c, ngp, A, w.a2t, ldb); });
   if (e == hipSuccess && nmod == 0)
-    e = timed(c, EXH_K_HO, s, [&] {
-      return launch_ho(s, c, ngp, A, (const double2 *)c->bt.p,
-                       c->have_h1aa ? (const double2 *)c->a2t.p : nullptr, ldb,
-                       (const int4 *)c->ivgk.p, vgpc, (double2 *)hw, (double2 *)o, rank, flags);
+    e = run(EXH_K_HO, [&] {
+      return launch_ho(s, c, ngp, A, w.at, w.bt, c->have_h1aa ? w.a2t : nullptr, ldb, w.ivgk,
+                       vgpc, (double2 *)hw, (double2 *)o, rank, flags);
     });
   if (e == hipSuccess && nmod > 0) {
-    OzLayout L;
-    int bits = 0;
-    const int rc2 = oz_prepare(c, nmod, L, bits);
-    if (rc2) return rc2;
+    OzLayout L = oz_layout(c->ngkmax, c->Ktot, nmod);
+    const int bits = oz_bits(nmod, c->Ktot);
     long long ldx;
-    const double2 *X = k_major_operand(c, A, &ldx);
+    const double2 *X = k_major_operand(c, A, w.at, &ldx);
     const int lo0 = c->intgv[0], lo1 = c->intgv[1], lo2 = c->intgv[2];
-    IstBox box{(const int4 *)c->ivgk.p, vgpc, (const IstEntry *)c->ist.p, lo0, lo1, lo2,
+    IstBox box{w.ivgk, vgpc, (const IstEntry *)c->ist.p, lo0, lo1, lo2,
                c->intgv[3] - lo0 + 1, c->intgv[4] - lo1 + 1, c->intgv[5] - lo2 + 1};
     const int kid[3] = {EXH_K_OZRES, EXH_K_OZGEMM, EXH_K_OZCOMB};
     for (int st = 0; st < 3 && e == hipSuccess; st++)
-      e = timed(c, kid[st], s, [&] {
-        return launch_ozaki(s, c, ngp, X, ldx, (const double2 *)c->bt.p, ldb, nmod, bits, L,
-                            (int8_t *)c->ozqx.p, (int8_t *)c->ozqb.p, (uint8_t *)c->ozr.p,
-                            (int *)c->ozex.p, (int *)c->ozeb.p, box, (double2 *)h, (double2 *)o,
-                            rank, flags & EXH_ACCUMULATE, 1 << st);
+      e = run(kid[st], [&] {
+        return launch_ozaki(s, c, ngp, X, ldx, w.bt, ldb, nmod, bits, L, (int8_t *)c->ozqx.p,
+                            (int8_t *)c->ozqb.p, (uint8_t *)c->ozr.p, (int *)c->ozex.p,
+                            (int *)c->ozeb.p, box, (double2 *)h, (double2 *)o, rank,
+                            flags & EXH_ACCUMULATE, 1 << st);
       });
   }
   if (e == hipSuccess)
-    e = timed(c, EXH_K_LOBORDER, s, [&] {
+    e = run(EXH_K_LOBORDER, [&] {
       return launch_lo_border(s, c, ngp, A, (double2 *)hw, (double2 *)o, rank, flags);
     });
   if (e == hipSuccess)
-    e = timed(c, EXH_K_LOLO, s, [&] { return launch_lolo(s, c, ngp, (double2 *)hw, (double2 *)o, rank, flags); });
+    e = run(EXH_K_LOLO, [&] { return launch_lolo(s, c, ngp, (double2 *)hw, (double2 *)o, rank, flags); });
   if (e == hipSuccess && o_only && h) e = launch_mirror(s, rank, (double2 *)h, rank);
   if (e == hipSuccess) e = mark_done(c, s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "setup kernels");
+}
+
+static int setup_impl(exh_context *c, void *stream, int ngp, int rank, const exh_z *apwalm,
+                      const int *igpig, const double *vgpc, exh_z *h, exh_z *o, int flags,
+                      bool o_only) {
+  int rc = check_setup(c, ngp, rank, apwalm, igpig, vgpc, h, o, flags, o_only);
+  if (rc) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int nmod = (flags >> 8) & 0xff;
+  const size_t op = (size_t)std::max(c->Ktot, 1) * ((c->ngkmax + 63) / 64 * 64) * sizeof(double2);
+  rc = o_only ? 0 : ensure(c, c->bt, op);
+  if (!rc && c->have_h1aa) rc = ensure(c, c->a2t, op);
+  if (!rc && !c->contig) rc = ensure(c, c->at, op);
+  if (!rc) rc = ensure(c, c->ivgk, (size_t)std::max(c->ngkmax, 1) * sizeof(int4));
+  if (!rc && nmod > 0) {
+    OzLayout L;
+    int bits = 0;
+    rc = oz_prepare(c, nmod, L, bits);
+  }
+  if (rc) return rc;
+  SetupScratch w;
+  w.bt = (double2 *)c->bt.p;
+  w.a2t = (double2 *)c->a2t.p;
+  w.at = (double2 *)c->at.p;
+  w.ivgk = (int4 *)c->ivgk.p;
+  return setup_launch(c, s, ngp, rank, apwalm, igpig, vgpc, h, o, flags, o_only, w, true);
 }
 
 int exh_setup_device(exh_context *c, void *stream, int ngp, int rank, const exh_z *apwalm,
@@ -652,6 +699,32 @@ int exh_overlapsetup_device(exh_context *c, void *stream, int ngp, int rank,
                             const exh_z *apwalm, const int *igpig, const double *vgpc,
                             exh_z *h, exh_z *o, int flags) {
   return setup_impl(c, stream, ngp, rank, apwalm, igpig, vgpc, h, o, flags, true);
+}
+
+// Host setups: bring rows 1..ngp of the host apwalm into s_apwalm, unless
+// reuse is on and s_apwalm already holds this array (same address and ngp
+// as the last host match or setup, same basis).
+static hipError_t stage_apwalm(exh_context *c, hipStream_t s, int ngp, const exh_z *apwalm) {
+  if (ngp <= 0) return hipSuccess;
+  if (c->apw_reuse && apwalm == c->apw_host && ngp == c->apw_ngp &&
+      c->apw_basis_epoch == c->basis_epoch && c->apw_dev == c->s_apwalm.p) {
+    c->apw_reused++;
+    return hipSuccess;
+  }
+  const size_t ld = c->ngkmax, ncol = (size_t)c->apwordmax * c->lmmax * c->natmtot;
+  hipError_t e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm,
+                                  ld * sizeof(double2), ngp * sizeof(double2), ncol,
+                                  hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    c->apw_host = apwalm;
+    c->apw_dev = c->s_apwalm.p;
+    c->apw_ngp = ngp;
+    c->apw_basis_epoch = c->basis_epoch;
+    c->apw_uploads++;
+  } else {
+    c->apw_host = nullptr;
+  }
+  return e;
 }
 
 int exh_overlapsetup(exh_context *c, int ngp, int rank, const exh_z *apwalm, const int *igpig,
@@ -672,10 +745,8 @@ int exh_overlapsetup(exh_context *c, int ngp, int rank, const exh_z *apwalm, con
   if (!rc) rc = ensure(c, c->s_o, std::max<size_t>(nn, 1) * sizeof(double2));
   if (rc) return rc;
   hipStream_t s = c->stream;
-  hipError_t e = hipSuccess;
+  hipError_t e = stage_apwalm(c, s, ngp, apwalm);
   if (ngp > 0) {
-    e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
-                         ngp * sizeof(double2), ncol, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(c->s_igpig.p, igpig, ngp * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -712,6 +783,7 @@ int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
   if (!rc) rc = ensure(c, c->s_apwalm, ld * ncol * sizeof(double2));
   if (rc) return rc;
   hipStream_t s = c->stream;
+  c->apw_host = nullptr;  // s_apwalm is rewritten below
   hipError_t e = hipMemcpyAsync(c->s_gpc.p, gpc, ngp * sizeof(double), hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(c->s_tpgpc.p, tpgpc, 2 * (size_t)ngp * sizeof(double), hipMemcpyHostToDevice, s);
@@ -731,7 +803,16 @@ int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
     e = hipMemcpy2DAsync(apwalm, ld * sizeof(double2), c->s_apwalm.p, ld * sizeof(double2),
                          ngp * sizeof(double2), ncol, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  return e == hipSuccess ? 0 : hip_fail(c, e, "match");
+  if (e != hipSuccess) {
+    c->apw_host = nullptr;
+    return hip_fail(c, e, "match");
+  }
+  // s_apwalm now equals the caller's apwalm (rows 1..ngp)
+  c->apw_host = apwalm;
+  c->apw_dev = c->s_apwalm.p;
+  c->apw_ngp = ngp;
+  c->apw_basis_epoch = c->basis_epoch;
+  return 0;
 }
 
 int exh_hamiltonandoverlapsetup_ex(exh_context *c, int ngp, int rank, const exh_z *apwalm,
@@ -753,10 +834,8 @@ int exh_hamiltonandoverlapsetup_ex(exh_context *c, int ngp, int rank, const exh_
   if (!rc) rc = ensure(c, c->s_o, std::max<size_t>(nn, 1) * sizeof(double2));
   if (rc) return rc;
   hipStream_t s = c->stream;
-  hipError_t e = hipSuccess;
+  hipError_t e = stage_apwalm(c, s, ngp, apwalm);
   if (ngp > 0) {
-    e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
-                         ngp * sizeof(double2), ncol, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(c->s_igpig.p, igpig, ngp * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -774,6 +853,19 @@ int exh_hamiltonandoverlapsetup_ex(exh_context *c, int ngp, int rank, const exh_
   if (e == hipSuccess) e = hipMemcpyAsync(o, c->s_o.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "setup: download");
+}
+
+int exh_set_apwalm_reuse(exh_context *c, int on) {
+  EXH_CHECK_CTX(c);
+  c->apw_reuse = on != 0;
+  return 0;
+}
+
+int exh_apwalm_transfers(exh_context *c, long long *uploads, long long *reused) {
+  EXH_CHECK_CTX(c);
+  if (uploads) *uploads = c->apw_uploads;
+  if (reused) *reused = c->apw_reused;
+  return 0;
 }
 
 int exh_profile_enable(exh_context *c, int on) {

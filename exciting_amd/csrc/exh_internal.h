@@ -165,6 +165,14 @@ struct exh_context {
   bool contig = true;  // colA[k] == k * ngkmax
   // host-API staging
   exh::DevBuf s_apwalm, s_igpig, s_vgpc, s_h, s_o, s_gpc, s_tpgpc, s_sfac;
+  // s_apwalm holds the host array at apw_host (rows 1..apw_ngp) as of the
+  // last host match or setup call; with apw_reuse the host setups then skip
+  // uploading it again (exh_set_apwalm_reuse)
+  bool apw_reuse = false;
+  const void *apw_host = nullptr, *apw_dev = nullptr;
+  int apw_ngp = -1;
+  long long apw_basis_epoch = -1;
+  long long apw_uploads = 0, apw_reused = 0;
 };
 
 namespace exh {
@@ -201,14 +209,17 @@ hipError_t launch_ist_table(hipStream_t s, const exh_context *c,
                             const double2 *cf, const double2 *vf,
                             const double2 *mf, const double2 *m2, int relnone,
                             int h1on);
+// `at` is the Ktot x ldb K-major copy of apwalm that launch_pack writes and
+// the others read when the context's stacked columns are not equally spaced
+// (unused otherwise).
 hipError_t launch_pack(hipStream_t s, const exh_context *c, int ngp,
-                       const double2 *apwalm);
+                       const double2 *apwalm, double2 *at);
 hipError_t launch_bt(hipStream_t s, const exh_context *c, int ngp,
-                     const double2 *apwalm, double2 *bt, int ldb);
+                     const double2 *apwalm, const double2 *at, double2 *bt, int ldb);
 hipError_t launch_a2t(hipStream_t s, const exh_context *c, int ngp,
                       const double2 *apwalm, double2 *a2t, int ldb);
 hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
-                     const double2 *apwalm, const double2 *bt,
+                     const double2 *apwalm, const double2 *at, const double2 *bt,
                      const double2 *a2t, int ldb, const int4 *ivgk,
                      const double *vgpc, double2 *h, double2 *o, int ldh,
                      int flags);
@@ -218,7 +229,8 @@ hipError_t launch_lo_border(hipStream_t s, const exh_context *c, int ngp,
 hipError_t launch_mirror(hipStream_t s, int n, double2 *m, int ld);
 hipError_t launch_lolo(hipStream_t s, const exh_context *c, int ngp,
                        double2 *h, double2 *o, int ldh, int flags);
-const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, long long *ld);
+const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, const double2 *at,
+                               long long *ld);
 int oz_bits(int nmod, int Ktot);
 hipError_t oz_upload_constants(int nmod);
 OzLayout oz_layout(int ngkmax, int Ktot, int nmod);

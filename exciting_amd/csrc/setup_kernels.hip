@@ -760,9 +760,11 @@ hipError_t launch_ist_table(hipStream_t s, const exh_context *c,
 }
 
 // K-major view of A for the MFMA kernels: apwalm itself when its stacked
-// columns are equally spaced (ld = ngkmax), else the packed copy in c->at.
+// columns are equally spaced (ld = ngkmax), else the packed copy `at`
+// (Ktot x ldb scratch).
 static const double2 *k_major_A(hipStream_t s, const exh_context *c, int ngp,
-                                const double2 *apwalm, long long *ld, hipError_t *err) {
+                                const double2 *apwalm, double2 *at, long long *ld,
+                                hipError_t *err) {
   *err = hipSuccess;
   if (c->contig) {
     *ld = c->ngkmax;
@@ -771,39 +773,40 @@ static const double2 *k_major_A(hipStream_t s, const exh_context *c, int ngp,
   const int ldb = (c->ngkmax + 63) / 64 * 64;
   *ld = ldb;
   hipLaunchKernelGGL(pack_kernel, dim3(cdiv(ngp, 256), c->Ktot), dim3(256), 0, s, ngp,
-                     apwalm, (const long long *)c->colA.p, (double2 *)c->at.p, ldb);
+                     apwalm, (const long long *)c->colA.p, at, ldb);
   *err = hipGetLastError();
-  return (const double2 *)c->at.p;
+  return at;
 }
 
 hipError_t launch_pack(hipStream_t s, const exh_context *c, int ngp,
-                       const double2 *apwalm) {
+                       const double2 *apwalm, double2 *at) {
   if (ngp <= 0 || c->contig || c->Ktot == 0) return hipSuccess;
   long long ld;
   hipError_t e;
-  (void)k_major_A(s, c, ngp, apwalm, &ld, &e);
+  (void)k_major_A(s, c, ngp, apwalm, at, &ld, &e);
   return e;
 }
 
 static const double2 *k_major_view(const exh_context *c, const double2 *apwalm,
-                                   long long *ld) {
+                                   const double2 *at, long long *ld) {
   if (c->contig) {
     *ld = c->ngkmax;
     return apwalm;
   }
   *ld = (c->ngkmax + 63) / 64 * 64;
-  return (const double2 *)c->at.p;
+  return at;
 }
 
-const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, long long *ld) {
-  return k_major_view(c, apwalm, ld);
+const double2 *k_major_operand(const exh_context *c, const double2 *apwalm, const double2 *at,
+                               long long *ld) {
+  return k_major_view(c, apwalm, at, ld);
 }
 
 hipError_t launch_bt(hipStream_t s, const exh_context *c, int ngp,
-                     const double2 *apwalm, double2 *bt, int ldb) {
+                     const double2 *apwalm, const double2 *at, double2 *bt, int ldb) {
   if (ngp <= 0 || c->natmtot <= 0) return hipSuccess;
   long long ldx;
-  const double2 *X = k_major_view(c, apwalm, &ldx);
+  const double2 *X = k_major_view(c, apwalm, at, &ldx);
   dim3 grid(cdiv(ngp, TM), cdiv(c->maxaa, TM), c->natmtot);
   hipLaunchKernelGGL(bt_kernel, grid, dim3(NTH), 0, s, ngp, X, ldx,
                      (const int *)c->katom.p, (const int *)c->katom.p + c->natmtot,
@@ -824,7 +827,7 @@ hipError_t launch_a2t(hipStream_t s, const exh_context *c, int ngp,
 }
 
 hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
-                     const double2 *apwalm, const double2 *bt,
+                     const double2 *apwalm, const double2 *at, const double2 *bt,
                      const double2 *a2t, int ldb, const int4 *ivgk,
                      const double *vgpc, double2 *h, double2 *o, int ldh,
                      int flags) {
@@ -834,7 +837,7 @@ hipError_t launch_ho(hipStream_t s, const exh_context *c, int ngp,
   const int lo0 = c->intgv[0], lo1 = c->intgv[1], lo2 = c->intgv[2];
   const int n0 = c->intgv[3] - lo0 + 1, n1 = c->intgv[4] - lo1 + 1, n2 = c->intgv[5] - lo2 + 1;
   long long ldx;
-  const double2 *X = k_major_view(c, apwalm, &ldx);
+  const double2 *X = k_major_view(c, apwalm, at, &ldx);
   const double2 *YA = a2t ? a2t : X;
   const long long lda = a2t ? (long long)ldb : ldx;
   if (h)

@@ -57,7 +57,9 @@ module exh_lapw
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
             exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device, &
             exh_mt_pot, exh_mt_pot_device, exh_mt_so, exh_mt_so_device, exh_stream, &
-            exh_match_device, exh_setup_device, exh_batch_plan_count, exh_batch_plan_info, &
+            exh_match_device, exh_setup_device, &
+            exh_set_apwalm_reuse, exh_apwalm_transfers, &
+            exh_batch_plan_count, exh_batch_plan_info, &
             exh_profile_enable, exh_profile_read, exh_profile_reset
 
   interface
@@ -245,6 +247,21 @@ module exh_lapw
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, stream, apwalm, igpig, vgpc, h, o
       integer(c_int), value :: ngp, rank, flags
+    end function
+
+    ! Host-array setups reuse the apwalm the last host match left in HBM
+    ! (same array, same ngp); the caller guarantees it was not modified.
+    integer(c_int) function exh_set_apwalm_reuse(ctx, on) bind(C, name="exh_set_apwalm_reuse")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+
+    integer(c_int) function exh_apwalm_transfers(ctx, uploads, reused) &
+        bind(C, name="exh_apwalm_transfers")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: uploads, reused
     end function
 
     ! Batch plans: their number and each one's shape.

@@ -76,6 +76,8 @@ def lib():
     L.exh_hamiltonandoverlapsetup.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
     L.exh_hamiltonandoverlapsetup_ex.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p, _i]
     L.exh_setup_device.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
+    L.exh_set_apwalm_reuse.argtypes = [_p, _i]
+    L.exh_apwalm_transfers.argtypes = [_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.exh_profile_enable.argtypes = [_p, _i]
     L.exh_profile_read.argtypes = [_p, _i, C.POINTER(_d), C.POINTER(_i)]
     L.exh_profile_reset.argtypes = [_p]
@@ -124,7 +126,8 @@ EXPORTS = (
     "exh_genmeffig", "exh_genveffig_device", "exh_set_mt_device", "exh_hmlint",
     "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
     "exh_mt_kinetic_device", "exh_mt_pot", "exh_mt_pot_device", "exh_mt_so",
-    "exh_mt_so_device",
+    "exh_mt_so_device", "exh_set_apwalm_reuse",
+    "exh_apwalm_transfers",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
@@ -267,6 +270,19 @@ class Context:
         rc = launch(self.stream)
         cur.wait_stream(self._tstream)
         return rc
+
+    # ---- host-array residency -------------------------------------------
+    def set_apwalm_reuse(self, on: bool = True):
+        """Let host-array setups reuse the apwalm the last host match left in
+        HBM when called with the same array (exh_set_apwalm_reuse)."""
+        self._check(lib().exh_set_apwalm_reuse(self._h, 1 if on else 0), "exh_set_apwalm_reuse")
+
+    def apwalm_transfers(self):
+        """(uploads, reused): host apwalm uploads made and skipped so far."""
+        u, r = C.c_longlong(), C.c_longlong()
+        self._check(lib().exh_apwalm_transfers(self._h, C.byref(u), C.byref(r)),
+                    "exh_apwalm_transfers")
+        return u.value, r.value
 
     # ---- instrumentation ------------------------------------------------
     def profile(self, on: bool = True):

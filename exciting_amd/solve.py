@@ -75,28 +75,38 @@ def _chebyshev_filter(C, X, degree: int, a: float, b: float, a0: float):
     return Y
 
 
-def lowest_eigvalsh(C, nwant: int, degree: int = 16, rtol: float = 1e-11, maxit: int = 40,
-                    seed: int = 1234):
+# Record of the last lowest_eigvalsh call: filter steps, final largest
+# residual norm, the tolerance it met, and whether it fell back to the full
+# eigensolve.
+last_subspace: dict = {}
+
+
+def lowest_eigvalsh(C, nwant: int, degree: int = 16, rtol: float = 1e-11, atol: float = 1e-11,
+                    maxit: int = 40, seed: int = 1234):
     """The nwant lowest eigenvalues of the Hermitian matrix C (n x n CUDA
     tensor), as ZHEGVX's range 'I' asks, by Chebyshev-filtered subspace
     iteration: a block of nwant + 16 (at least 1.25 nwant) vectors, filtered
     with the polynomial that damps everything above the block's largest Ritz
     value, orthonormalised and Rayleigh-Ritz projected until every wanted
-    Ritz pair has residual norm <= rtol * max|spectrum|.  The residual bounds
-    each eigenvalue's error (Weyl).  Small problems, and the rare run that
-    does not converge in maxit filter steps, use the full Hermitian
-    eigensolve instead.  Deterministic: the start block comes from a seeded
-    generator."""
+    Ritz pair has residual norm <= min(rtol * max|spectrum|, atol).  The
+    residual bounds each eigenvalue's error (Weyl), so atol (Ha) caps the
+    error however large the spectrum of C grows with an ill-conditioned
+    overlap.  Small problems, and a run that does not meet the tolerance in
+    maxit filter steps, use the full Hermitian eigensolve instead.
+    Deterministic: the start block comes from a seeded generator."""
     import torch
     n = C.shape[0]
     m = min(n, max(nwant + 16, (5 * nwant + 3) // 4))
+    last_subspace.clear()
     if n < 512 or 3 * m > n:
+        last_subspace.update(steps=0, fallback="small")
         return torch.linalg.eigvalsh(C)[:nwant]
     b = _lanczos_upper_bound(C)
     g = torch.Generator(device=C.device).manual_seed(seed)
     X = torch.randn(n, m, dtype=C.dtype, device=C.device, generator=g)
     X, _ = torch.linalg.qr(X)
-    for _ in range(maxit):
+    res = float("inf")
+    for it in range(maxit):
         CX = C @ X
         Hs = X.conj().transpose(0, 1) @ CX
         Hs = 0.5 * (Hs + Hs.conj().transpose(0, 1))
@@ -105,13 +115,17 @@ def lowest_eigvalsh(C, nwant: int, degree: int = 16, rtol: float = 1e-11, maxit:
         CX = CX @ Q
         R = CX[:, :nwant] - X[:, :nwant] * theta[:nwant].to(X.dtype)[None, :]
         scale = max(abs(b), float(theta.abs().max()))
-        if float(torch.linalg.vector_norm(R, dim=0).max()) <= rtol * scale:
+        tol = min(rtol * scale, atol)
+        res = float(torch.linalg.vector_norm(R, dim=0).max())
+        if res <= tol:
+            last_subspace.update(steps=it, residual=res, tol=tol, scale=scale, fallback=None)
             return theta[:nwant]
         a, a0 = float(theta[-1]), float(theta[0])
         if not a < b:
             break
         X = _chebyshev_filter(C, X, degree, a, b, a0)
         X, _ = torch.linalg.qr(X)
+    last_subspace.update(steps=maxit, residual=res, scale=b, fallback="not converged")
     return torch.linalg.eigvalsh(C)[:nwant]
 
 

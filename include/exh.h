@@ -276,6 +276,21 @@ int exh_setup_device(exh_context *ctx, void *stream, int ngp, int rank,
                      const exh_z *apwalm, const int *igpig, const double *vgpc,
                      exh_z *h, exh_z *o, int flags);
 
+/* No counterpart in the reference (its arrays never leave host memory).
+ * With reuse on, the host-pointer setups (exh_hamiltonandoverlapsetup[_ex],
+ * exh_overlapsetup) skip uploading apwalm when it is the array the last
+ * host-pointer exh_match filled, or the last such setup read, on this
+ * context: same address, same ngp, no exh_set_basis since.  That saves
+ * 16 * ngp * apwordmax * (lmaxapw+1)^2 * natmtot bytes of PCIe per call
+ * (2.3 GB for 64 atoms at ngk 9340).  The caller guarantees that apwalm was
+ * not modified in between, as exciting's k loop does: match fills it and
+ * the setup reads it next (seceqn.f90:57-59, calc_vnlmat.f90:54-60).  Off
+ * by default. */
+int exh_set_apwalm_reuse(exh_context *ctx, int on);
+/* Host-pointer apwalm uploads made and skipped by reuse since the context
+ * was created. */
+int exh_apwalm_transfers(exh_context *ctx, long long *uploads, long long *reused);
+
 /* overlapsetup.f90:8-166 alone: O(rank,rank) accumulated (muffin-tin, lo and
  * interstitial parts, upper triangle) and mirrored; then, as the reference
  * calls hamiltonoverlapocopy_UL for both matrices (:164-166), the lower

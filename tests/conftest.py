@@ -26,3 +26,10 @@ def pytest_collection_modifyitems(config, items):
     for it in items:
         if "gpu" in it.keywords:
             it.add_marker(skip)
+
+
+def pytest_sessionstart(session):
+    # hipBLAS workspaces large enough for n ~ 10^4 triangular solves, set
+    # before any test creates the first hipBLAS handle
+    from exciting_amd import solve
+    solve.configure()

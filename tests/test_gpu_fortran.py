@@ -95,3 +95,44 @@ def test_fortran_driver(tmp_path, name):
     terr = [np.abs(g - w).max() / max(np.abs(w).max(), 1e-300) for g, w in zip(tabs, want)]
     print("  fortran hmlint: " + " ".join(f"{e:.1e}" for e in terr))
     assert max(terr) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tiny", "tiny_lmaxmat", "fcc2"])
+def test_reference_name_shims(tmp_path, name):
+    """exh_shims.f90 called as exciting calls the reference routines:
+    match, then hamiltonandoverlapsetup on a newsystem (seceqn.f90:57,
+    seceqnfv.f90:73-77), then overlapsetup alone (calc_vnlmat.f90:59-60),
+    then hamiltonandoverlapsetup again (accumulating).  Each result is within
+    1e-12 of the oracle, and apwalm crossed PCIe only once (in match's
+    download): the three setups reused the copy in HBM."""
+    if not os.path.exists(exhbuild.SHIM_DRIVER):
+        pytest.fail("shim driver not built (python -m exciting_amd.build)")
+    s = synth.add_mt_potential(dataclasses.replace(_cases.system(name)), lmaxvr=5, seed=3)
+    k = s.kpts[0]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, s)
+    print(f"[fortran-shims] case {name}", flush=True)
+    r = subprocess.run([exhbuild.SHIM_DRIVER, str(fin), str(fout)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    lmm = (s.lmaxapw + 1) ** 2
+    n = k.ngk + s.nlotot
+    na = s.ngkmax * s.apwordmax * lmm * s.natmtot
+    raw = np.fromfile(fout, dtype=np.uint8)
+    z = raw[:16 * (na + 6 * n * n)].view(np.complex128)
+    uploads, reused = raw[16 * (na + 6 * n * n):].view(np.int64)
+    A = z[:na].reshape((s.ngkmax, s.apwordmax, lmm, s.natmtot), order="F")
+    H, O, Oalone, H2, O2 = (z[na + i * n * n:na + (i + 1) * n * n].reshape((n, n), order="F")
+                            for i in range(5))
+    S = oracle.System(s)
+    A0 = oracle.match(S, 0)
+    H0, O0 = oracle.setup(S, 0, A0)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    errs = {"apwalm": rel(A, A0), "H": rel(H, H0), "O": rel(O, O0),
+            "O alone": rel(Oalone, O0), "H twice": rel(H2, 2 * H0), "O twice": rel(O2, 2 * O0)}
+    print(f"  shims {name}: " + " ".join(f"{k} {v:.1e}" for k, v in errs.items())
+          + f"; apwalm uploads {uploads}, reused {reused}")
+    assert max(errs.values()) <= 1e-12
+    assert uploads == 0 and reused == 3
