@@ -121,8 +121,8 @@ def test_reference_name_shims(tmp_path, name):
     n = k.ngk + s.nlotot
     na = s.ngkmax * s.apwordmax * lmm * s.natmtot
     raw = np.fromfile(fout, dtype=np.uint8)
-    z = raw[:16 * (na + 6 * n * n)].view(np.complex128)
-    uploads, reused = raw[16 * (na + 6 * n * n):].view(np.int64)
+    z = raw[:16 * (na + 5 * n * n)].view(np.complex128)
+    uploads, reused = raw[16 * (na + 5 * n * n):].view(np.int64)
     A = z[:na].reshape((s.ngkmax, s.apwordmax, lmm, s.natmtot), order="F")
     H, O, Oalone, H2, O2 = (z[na + i * n * n:na + (i + 1) * n * n].reshape((n, n), order="F")
                             for i in range(5))
