@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "exh_device.h"
@@ -241,6 +242,37 @@ int oz_prepare(exh_context *c, int nmod, OzLayout &L, int &bits) {
 
 }  // namespace exh
 
+// Device scratch of exh_setup_device_noalloc: one copy per GPU, shared by
+// every context on that GPU and held until the process ends.  Host code never
+// passes `scratch` itself to a kernel (on the host the name is only a
+// stand-in for the symbol): exh_create looks up each device's address once,
+// with that device current, and the entry point uses that address.
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices];
+
+// Address of `scratch` on `device`, which must be the current device.
+int scratch_address(int device, void **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return EXH_ERANGE;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return e != hipSuccess ? (int)e : 1;
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return 0;
+}
+
+inline size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+}  // namespace
+
 using namespace exh;
 
 #define EXH_CHECK_CTX(c) \
@@ -270,6 +302,12 @@ int exh_create(int device, exh_context **out) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return rc > 0 ? rc : 1;
+  }
+  if (int rc = scratch_address(device, &c->noalloc_scratch)) {
+    (void)hipEventDestroy(c->done_ev);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
   }
   *out = c;
   return 0;
@@ -699,6 +737,44 @@ int exh_overlapsetup_device(exh_context *c, void *stream, int ngp, int rank,
                             const exh_z *apwalm, const int *igpig, const double *vgpc,
                             exh_z *h, exh_z *o, int flags) {
   return setup_impl(c, stream, ngp, rank, apwalm, igpig, vgpc, h, o, flags, true);
+}
+
+int exh_setup_device_noalloc(exh_context *c, void *stream, int ngp, int rank,
+                             const exh_z *apwalm, const int *igpig, const double *vgpc, exh_z *h,
+                             exh_z *o, int flags) {
+  int rc = check_setup(c, ngp, rank, apwalm, igpig, vgpc, h, o, flags, false);
+  if (rc) return rc;
+  if (flags & 0xff00)
+    return fail(c, EXH_ENOTSUP,
+                "setup_noalloc: split precision needs more scratch than the static buffer holds; "
+                "use exh_setup_device");
+  if (!c->noalloc_scratch) return fail(c, EXH_ESTATE, "setup_noalloc: no scratch on this device");
+  // [ivgk | Bt | A2t (iora*) | packed A^T (non-uniform layouts)], 256-B aligned
+  const size_t op = align256((size_t)std::max(c->Ktot, 1) * ((c->ngkmax + 63) / 64 * 64) *
+                             sizeof(double2));
+  const size_t nivgk = align256((size_t)std::max(c->ngkmax, 1) * sizeof(int4));
+  const size_t need = nivgk + op * (1 + (c->have_h1aa ? 1 : 0) + (c->contig ? 0 : 1));
+  if (need > kNoallocScratchBytes) {
+    char buf[200];
+    snprintf(buf, sizeof buf,
+             "setup_noalloc: this basis needs %zu bytes of scratch (natmtot*maxaa = %d, ngkmax "
+             "= %d); the static buffer holds %zu",
+             need, c->Ktot, c->ngkmax, kNoallocScratchBytes);
+    return fail(c, EXH_ERANGE, buf);
+  }
+  unsigned char *base = (unsigned char *)c->noalloc_scratch;
+  SetupScratch w;
+  w.ivgk = (int4 *)base;
+  size_t off = nivgk;
+  w.bt = (double2 *)(base + off);
+  off += op;
+  if (c->have_h1aa) {
+    w.a2t = (double2 *)(base + off);
+    off += op;
+  }
+  if (!c->contig) w.at = (double2 *)(base + off);
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return setup_launch(c, s, ngp, rank, apwalm, igpig, vgpc, h, o, flags, false, w, false);
 }
 
 // Host setups: bring rows 1..ngp of the host apwalm into s_apwalm, unless

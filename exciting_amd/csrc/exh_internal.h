@@ -160,6 +160,8 @@ struct exh_context {
 
   // per-call scratch
   exh::DevBuf bt, a2t, at, ivgk;
+  // this device's copy of the static scratch of exh_setup_device_noalloc
+  void *noalloc_scratch = nullptr;
   exh::DevBuf ozqx, ozqb, ozr, ozex, ozeb;
   bool oz_consts = false;
   bool contig = true;  // colA[k] == k * ngkmax

@@ -57,7 +57,7 @@ module exh_lapw
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
             exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device, &
             exh_mt_pot, exh_mt_pot_device, exh_mt_so, exh_mt_so_device, exh_stream, &
-            exh_match_device, exh_setup_device, &
+            exh_match_device, exh_setup_device, exh_setup_device_noalloc, &
             exh_set_apwalm_reuse, exh_apwalm_transfers, &
             exh_batch_plan_count, exh_batch_plan_info, &
             exh_profile_enable, exh_profile_read, exh_profile_reset
@@ -244,6 +244,16 @@ module exh_lapw
 
     integer(c_int) function exh_setup_device(ctx, stream, ngp, rank, apwalm, igpig, vgpc, h, o, &
         flags) bind(C, name="exh_setup_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream, apwalm, igpig, vgpc, h, o
+      integer(c_int), value :: ngp, rank, flags
+    end function
+
+    ! Same as exh_setup_device without any device allocation: its scratch is
+    ! a static 64 MiB array per GPU, shared by every context there (calls
+    ! that overlap on one GPU race).  FP64 only.
+    integer(c_int) function exh_setup_device_noalloc(ctx, stream, ngp, rank, apwalm, igpig, &
+        vgpc, h, o, flags) bind(C, name="exh_setup_device_noalloc")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, stream, apwalm, igpig, vgpc, h, o
       integer(c_int), value :: ngp, rank, flags

@@ -76,6 +76,7 @@ def lib():
     L.exh_hamiltonandoverlapsetup.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p]
     L.exh_hamiltonandoverlapsetup_ex.argtypes = [_p, _i, _i, _p, _p, _p, _p, _p, _i]
     L.exh_setup_device.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
+    L.exh_setup_device_noalloc.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
     L.exh_set_apwalm_reuse.argtypes = [_p, _i]
     L.exh_apwalm_transfers.argtypes = [_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.exh_profile_enable.argtypes = [_p, _i]
@@ -126,7 +127,7 @@ EXPORTS = (
     "exh_genmeffig", "exh_genveffig_device", "exh_set_mt_device", "exh_hmlint",
     "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
     "exh_mt_kinetic_device", "exh_mt_pot", "exh_mt_pot_device", "exh_mt_so",
-    "exh_mt_so_device", "exh_set_apwalm_reuse",
+    "exh_mt_so_device", "exh_setup_device_noalloc", "exh_set_apwalm_reuse",
     "exh_apwalm_transfers",
 )
 

@@ -276,6 +276,23 @@ int exh_setup_device(exh_context *ctx, void *stream, int ngp, int rank,
                      const exh_z *apwalm, const int *igpig, const double *vgpc,
                      exh_z *h, exh_z *o, int flags);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same setup as exh_setup_device (FP64; flags
+ * EXH_ACCUMULATE or 0), with bitwise the same result, but its scratch
+ * (G+k index map, aa.A operand, and for iora* or non-uniform layouts one
+ * more operand) lives in a static 64 MiB device array instead of the
+ * context's buffers: it calls no hipMalloc, hipFree or allocating library
+ * routine.  Bases needing more than 64 MiB (16 * natmtot*maxaa * ngkmax
+ * rounded up, per operand) are refused with EXH_ERANGE, split precision
+ * with EXH_ENOTSUP.  Every context on a GPU shares that GPU's array, so
+ * calls that overlap on one GPU (two streams, two host threads) race; the
+ * array keeps the last call's contents, and every kernel writes its scratch
+ * before reading it.  No profiling events are recorded. */
+int exh_setup_device_noalloc(exh_context *ctx, void *stream, int ngp, int rank,
+                             const exh_z *apwalm, const int *igpig, const double *vgpc,
+                             exh_z *h, exh_z *o, int flags);
+
 /* No counterpart in the reference (its arrays never leave host memory).
  * With reuse on, the host-pointer setups (exh_hamiltonandoverlapsetup[_ex],
  * exh_overlapsetup) skip uploading apwalm when it is the array the last
