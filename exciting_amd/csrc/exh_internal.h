@@ -43,9 +43,54 @@ struct OzLayout {
   size_t qx_bytes = 0, qb_bytes = 0, r_bytes = 0;
 };
 
+// APW-lo border tables (hamiltonsetup.f90:124-151, overlapsetup.f90:116-133)
+// of one structure, device pointers.
+struct LoTables {
+  const long long *colA;
+  const int *Koff, *Ms, *atom_species, *losize, *lobase;
+  const double2 *loa;
+  int maxnlo, maxaa;
+  const int *nlorb, *lorbl;
+  int nlomax;
+  const int *apword;
+  int lmaxapw, pmax, lmmax, ngkmax;
+  const double *salo;
+};
+
+// Per-structure descriptor of a grouped launch, in device memory: the
+// device addresses and scalars of one context's tables that the grouped
+// kernels read, so that one launch can cover k-points of several structures.
+struct SDesc {
+  // basis and the stacked K index (atom, l, m, io)
+  int natmtot, Ktot, maxaa, lmaxapw, pmax, lmmax, ngkmax, have_h1aa;
+  long long ldx;  // leading dimension of the K-major A operand
+  int ldb;        // leading dimension of the batch scratch operands
+  const long long *colA;
+  const int *Koff, *Ms, *kinfo;
+  const double2 *aa;
+  const double *h1aa;
+  // match
+  const int *atom_species, *apword, *omax, *piv;
+  const double *rmt, *lu;
+  double fpso;  // 4 pi / sqrt(omega)
+  // G vectors and the interstitial box
+  const int *ivg;
+  int ngrtot;
+  const IstEntry *ist;
+  int lo0, lo1, lo2, n0, n1, n2;
+  // local orbitals
+  LoTables lo;
+  int nlotot, maxnlo, nlomax;
+  const LoRow *lorow;
+  const int *lobase;
+  const double2 *lolo;
+  const double *ololo_eff;
+};
+
 // Per-k-point descriptor of a grouped (batched) launch, in device memory.
 // Match inputs, apwalm, the G+k maps and H/O are the caller's; x, at, bt,
-// a2t and ivgk point into the context's batch scratch.
+// a2t and ivgk point into the owning context's batch scratch; sd is the
+// owning structure's descriptor.
 struct KDesc {
   int ngp, ldh;
   const double *gpc, *tpgpc;
@@ -57,6 +102,16 @@ struct KDesc {
   const double2 *x;  // K-major A: apwalm itself (equally spaced columns) or at
   double2 *at, *bt, *a2t;
   int4 *ivgk;
+  const SDesc *sd;
+};
+
+// Extent of one group of a grouped launch: the largest sizes over its
+// k-points and structures, which set the launch grids (blocks beyond a
+// k-point's own sizes exit).
+struct GroupDims {
+  int nk = 0, ngpmax = 0, natmax = 0, maxaa = 0, Kmax = 0, nlotot = 0, maxnlo = 0;
+  unsigned lmax_mask = 0;  // bit l: some structure has lmaxapw = l
+  bool any_pack = false, any_a2t = false;
 };
 
 // Tile shapes of the grouped H/O kernel: output TMI x TMJ per workgroup
@@ -247,25 +302,20 @@ hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
                         const double *gpc, const double *tpgpc,
                         const double2 *sfacgp, double2 *apwalm);
 
-// Grouped launchers: one launch covers nk k-points described by kd (device);
-// ngpmax bounds their ngp.
-hipError_t launch_match_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                                int ngpmax);
-hipError_t launch_ivgk_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                               int ngpmax);
-hipError_t launch_pack_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                               int ngpmax, int ldb);
-hipError_t launch_bt_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                             int ngpmax, long long ldx, int ldb);
-hipError_t launch_a2t_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                              int ngpmax, int ldb);
-hipError_t launch_ho_grouped(hipStream_t s, const exh_context *c, int shape, const KDesc *kd,
-                             const int4 *tiles, int ntiles, int nwg, long long ldx, int ldb,
-                             int accumulate);
+// Grouped launchers: one launch covers the g.nk k-points described by kd
+// (device), of one or several structures.
+SDesc make_sdesc(const exh_context *c);
+hipError_t launch_match_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd);
+hipError_t launch_ivgk_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd);
+hipError_t launch_pack_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd);
+hipError_t launch_bt_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd);
+hipError_t launch_a2t_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd);
+hipError_t launch_ho_grouped(hipStream_t s, int shape, const KDesc *kd, const int4 *tiles,
+                             int ntiles, int nwg, int accumulate);
 int ho_grouped_occupancy(int shape);
-hipError_t launch_lo_border_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                                    int ngpmax, int accumulate);
-hipError_t launch_lolo_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+hipError_t launch_lo_border_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd,
+                                    int accumulate);
+hipError_t launch_lolo_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd,
                                int accumulate);
 
 }  // namespace exh

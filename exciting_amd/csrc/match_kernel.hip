@@ -361,17 +361,17 @@ __global__ __launch_bounds__(64) void match_kernel(
                  apwalm, atom_species, rmt, apword, omaxs, lu, piv, fpso);
 }
 
-// Grouped over k-points: blockIdx.z selects the descriptor.
+// Grouped over k-points: blockIdx.z selects the descriptor, blockIdx.y the
+// atom.  One launch per lmaxapw present in the group; k-points of structures
+// with another lmaxapw exit at once.
 template <int LM>
-__global__ __launch_bounds__(64) void match_grouped_kernel(
-    const KDesc *__restrict__ kd, int ld, int pmax, const int *__restrict__ atom_species,
-    const double *__restrict__ rmt, const int *__restrict__ apword,
-    const int *__restrict__ omaxs, const double *__restrict__ lu,
-    const int *__restrict__ piv, double fpso) {
+__global__ __launch_bounds__(64) void match_grouped_kernel(const KDesc *__restrict__ kd) {
   const KDesc &d = kd[blockIdx.z];
-  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, ld, pmax, gptr(d.gpc),
-                 gptr(d.tpgpc), gptr(d.sfac), gptr(d.apwalm), atom_species, rmt, apword, omaxs,
-                 lu, piv, fpso);
+  const SDesc &S = *d.sd;
+  if (S.lmaxapw != LM || (int)blockIdx.y >= S.natmtot) return;
+  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, S.ngkmax, S.pmax,
+                 gptr(d.gpc), gptr(d.tpgpc), gptr(d.sfac), gptr(d.apwalm), S.atom_species,
+                 S.rmt, S.apword, S.omax, S.lu, S.piv, S.fpso);
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
@@ -408,30 +408,20 @@ hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
   return hipGetLastError();
 }
 
-hipError_t launch_match_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                                int ngpmax) {
-  if (nk <= 0 || ngpmax <= 0 || c->natmtot <= 0) return hipSuccess;
-  const double fpso = 12.566370614359172954 / sqrt(c->omega);
-  dim3 grid(cdiv(ngpmax, 64), c->natmtot, nk);
-  const int *as = (const int *)c->atom_species_d.p;
-  const int *apw = (const int *)c->apword_d.p;
-  const int *om = (const int *)c->omax_d.p;
-  const double *lu = (const double *)c->mt_match.lu.p;
-  const int *pv = (const int *)c->mt_match.piv.p;
-#define EXH_M(L)                                                                        \
-  case L:                                                                               \
-    hipLaunchKernelGGL(match_grouped_kernel<L>, grid, dim3(64), 0, s, kd, c->ngkmax,    \
-                       c->apwordmax, as, (const double *)c->rmt_d.p, apw, om, lu, pv,   \
-                       fpso);                                                           \
-    break;
-  switch (c->lmaxapw) {
-    EXH_M(0) EXH_M(1) EXH_M(2) EXH_M(3) EXH_M(4) EXH_M(5) EXH_M(6)
-    EXH_M(7) EXH_M(8) EXH_M(9) EXH_M(10) EXH_M(11) EXH_M(12)
-    default:
-      return hipErrorInvalidValue;
+hipError_t launch_match_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd) {
+  if (g.nk <= 0 || g.ngpmax <= 0 || g.natmax <= 0) return hipSuccess;
+  dim3 grid(cdiv(g.ngpmax, 64), g.natmax, g.nk);
+  if (g.lmax_mask >> (kMatchLmaxCap + 1)) return hipErrorInvalidValue;
+#define EXH_M(L)                                                                  \
+  if (g.lmax_mask & (1u << L)) {                                                  \
+    hipLaunchKernelGGL(match_grouped_kernel<L>, grid, dim3(64), 0, s, kd);        \
+    hipError_t e = hipGetLastError();                                             \
+    if (e != hipSuccess) return e;                                                \
   }
+  EXH_M(0) EXH_M(1) EXH_M(2) EXH_M(3) EXH_M(4) EXH_M(5) EXH_M(6)
+  EXH_M(7) EXH_M(8) EXH_M(9) EXH_M(10) EXH_M(11) EXH_M(12)
 #undef EXH_M
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 }  // namespace exh

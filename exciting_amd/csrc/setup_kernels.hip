@@ -14,6 +14,8 @@
 // MFMA: v_mfma_f64_16x16x4f64.  Operand lanes: A[i=lane%16][k=lane/16],
 // B[k=lane/16][j=lane%16]; result D[i=lane/16+4*r][j=lane%16], r=0..3
 // (established by the parity tests against the oracle).
+#include <cstring>
+
 #include "exh_device.h"
 
 namespace exh {
@@ -55,10 +57,10 @@ __global__ void ivgk_kernel(int ngp, const int *__restrict__ igpig,
 }
 
 // Grouped over k-points: blockIdx.y selects the descriptor.
-__global__ void ivgk_grouped_kernel(const KDesc *__restrict__ kd, const int *__restrict__ ivg,
-                                    int ngrtot) {
+__global__ void ivgk_grouped_kernel(const KDesc *__restrict__ kd) {
   const KDesc &d = kd[blockIdx.y];
-  ivgk_body(blockIdx.x * blockDim.x + threadIdx.x, d.ngp, gptr(d.igpig), ivg, ngrtot,
+  const SDesc &S = *d.sd;
+  ivgk_body(blockIdx.x * blockDim.x + threadIdx.x, d.ngp, gptr(d.igpig), S.ivg, S.ngrtot,
             gptr(d.ivgk));
 }
 
@@ -105,12 +107,14 @@ __global__ void pack_kernel(int ngp, const double2 *__restrict__ apwalm,
   if (g < ngp) at[(size_t)k * ldb + g] = apwalm[colA[k] + g];
 }
 
-__global__ void pack_grouped_kernel(const KDesc *__restrict__ kd,
-                                    const long long *__restrict__ colA, int ldb) {
+__global__ void pack_grouped_kernel(const KDesc *__restrict__ kd) {
   const KDesc &d = kd[blockIdx.z];
+  const SDesc &S = *d.sd;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
-  if (g < d.ngp) st(gptr(d.at) + (size_t)k * ldb + g, ld(gptr((const double2 *)d.apwalm) + colA[k] + g));
+  if (!d.at || k >= S.Ktot) return;  // equally spaced columns: nothing to pack
+  if (g < d.ngp)
+    st(gptr(d.at) + (size_t)k * S.ldb + g, ld(gptr((const double2 *)d.apwalm) + S.colA[k] + g));
 }
 
 // ---------------------------------------------------------------------------
@@ -118,15 +122,15 @@ __global__ void pack_grouped_kernel(const KDesc *__restrict__ kd,
 // (hamiltonsetup.f90:75-88, zm = aa * apwi).  X holds A^T K-major with
 // leading dimension ldx.  Grid (G tiles, f tiles, atoms).
 // ---------------------------------------------------------------------------
-template <typename PX, typename PB>
+template <typename PX, typename PB, typename PAA = const double2 *>
 __device__ __forceinline__ void bt_body(
     int ias, int I0, int F0, int ngp, PX __restrict__ X, long long ldx,
     const int *__restrict__ Koff, const int *__restrict__ Ms,
-    const double2 *__restrict__ aa, int maxaa, PB __restrict__ bt, int ldb) {
+    PAA __restrict__ aa, int maxaa, PB __restrict__ bt, int ldb) {
   const int M = Ms[ias];
   if (F0 >= M || I0 >= ngp) return;
   const int koff = Koff[ias];
-  const double2 *aat = aa + (size_t)ias * maxaa * maxaa;
+  const PAA aat = aa + (size_t)ias * maxaa * maxaa;
 
   __shared__ double sXr[TK][TM + 16], sXi[TK][TM + 16], sYr[TK][TM + 16], sYi[TK][TM + 16];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -134,7 +138,7 @@ __device__ __forceinline__ void bt_body(
   const int ii = tid & 63, kq = tid >> 6;
   const bool vi = I0 + ii < ngp, vf = F0 + ii < M;
   const PX xp = X + (size_t)koff * ldx + (vi ? I0 + ii : ngp - 1);
-  const double2 *yp = aat + (vf ? F0 + ii : M - 1);
+  const PAA yp = aat + (vf ? F0 + ii : M - 1);
 
   dx4 cre[2][2], cim[2][2];
 #pragma unroll
@@ -156,7 +160,7 @@ __device__ __forceinline__ void bt_body(
       const bool kv = k < M;
       const int kc = kv ? k : M - 1;
       px[q] = ld(xp + (size_t)kc * ldx);
-      py[q] = yp[(size_t)maxaa * kc];
+      py[q] = ld(yp + (size_t)maxaa * kc);
       fxv[q] = (kv && vi) ? 1.0 : 0.0;
       fyv[q] = (kv && vf) ? 1.0 : 0.0;
     }
@@ -221,14 +225,16 @@ __global__ __launch_bounds__(NTH) void bt_kernel(
           ldb);
 }
 
-// Grouped: blockIdx.z = ias + natmtot * (k-point).
-__global__ __launch_bounds__(NTH) void bt_grouped_kernel(
-    const KDesc *__restrict__ kd, int natmtot, long long ldx, const int *__restrict__ Koff,
-    const int *__restrict__ Ms, const double2 *__restrict__ aa, int maxaa, int ldb) {
-  const int k = blockIdx.z / natmtot, ias = blockIdx.z - k * natmtot;
+// Grouped: blockIdx.z = ias + natmax * (k-point), natmax the largest atom
+// count of the group's structures.
+__global__ __launch_bounds__(NTH) void bt_grouped_kernel(const KDesc *__restrict__ kd,
+                                                         int natmax) {
+  const int k = blockIdx.z / natmax, ias = blockIdx.z - k * natmax;
   const KDesc &d = kd[k];
-  bt_body(ias, blockIdx.x * TM, blockIdx.y * TM, d.ngp, gptr(d.x), ldx, Koff, Ms, aa, maxaa,
-          gptr(d.bt), ldb);
+  const SDesc &S = *d.sd;
+  if (ias >= S.natmtot) return;
+  bt_body(ias, blockIdx.x * TM, blockIdx.y * TM, d.ngp, gptr(d.x), S.ldx, S.Koff, S.Ms,
+          gptr(S.aa), S.maxaa, gptr(d.bt), S.ldb);
 }
 
 // ---------------------------------------------------------------------------
@@ -270,18 +276,15 @@ __global__ void a2t_kernel(int ngp, int natmtot, const double2 *__restrict__ apw
            Koff, Ms, kinfo, h1aa, pmax, lmaxapw, a2t, ldb);
 }
 
-// Grouped: blockIdx.z = ias + natmtot * (k-point).
-__global__ void a2t_grouped_kernel(const KDesc *__restrict__ kd, int natmtot,
-                                   const long long *__restrict__ colA,
-                                   const int *__restrict__ Koff, const int *__restrict__ Ms,
-                                   const int *__restrict__ kinfo,
-                                   const double *__restrict__ h1aa, int pmax, int lmaxapw,
-                                   int ldb) {
-  const int k = blockIdx.z / natmtot, ias = blockIdx.z - k * natmtot;
+// Grouped: blockIdx.z = ias + natmax * (k-point).
+__global__ void a2t_grouped_kernel(const KDesc *__restrict__ kd, int natmax) {
+  const int k = blockIdx.z / natmax, ias = blockIdx.z - k * natmax;
   const KDesc &d = kd[k];
+  const SDesc &S = *d.sd;
+  if (!S.have_h1aa || ias >= S.natmtot) return;
   a2t_body(blockIdx.x * blockDim.x + threadIdx.x, ias, blockIdx.y, d.ngp,
-           gptr((const double2 *)d.apwalm), colA, Koff, Ms, kinfo, h1aa, pmax, lmaxapw,
-           gptr(d.a2t), ldb);
+           gptr((const double2 *)d.apwalm), S.colA, S.Koff, S.Ms, S.kinfo, S.h1aa, S.pmax,
+           S.lmaxapw, gptr(d.a2t), S.ldb);
 }
 
 // ---------------------------------------------------------------------------
@@ -538,9 +541,7 @@ __global__ __launch_bounds__(64 * (TO / 32) * (TO / 32)) void o_kernel(
 // or grid, so all plans of a shape give bitwise identical results.
 template <int TMI, int TMJ, int TK>
 __global__ __launch_bounds__(64 * (TMI / 32) * (TMJ / 32)) void ho_grouped_kernel(
-    const KDesc *__restrict__ kd, const int4 *__restrict__ tiles, int ntiles, int Ktot,
-    long long ldx, long long ldb, int use_a2t, const IstEntry *__restrict__ T, int lo0,
-    int lo1, int lo2, int n0, int n1, int n2, int accumulate) {
+    const KDesc *__restrict__ kd, const int4 *__restrict__ tiles, int ntiles, int accumulate) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int4 e = tiles[t];
     const int k = __builtin_amdgcn_readfirstlane(e.x);
@@ -548,10 +549,14 @@ __global__ __launch_bounds__(64 * (TMI / 32) * (TMJ / 32)) void ho_grouped_kerne
     const int I0 = __builtin_amdgcn_readfirstlane(e.y);
     const int J0 = __builtin_amdgcn_readfirstlane(e.z);
     const KDesc &d = kd[k];
+    const SDesc &S = *d.sd;
+    const int Ktot = __builtin_amdgcn_readfirstlane(S.Ktot);
+    const long long ldx = S.ldx, ldb = S.ldb;
     const IstBoxT<const EXH_G int4 *, const EXH_G double *> box{
-        gptr((const int4 *)d.ivgk), gptr(d.vgpc), T, lo0, lo1, lo2, n0, n1, n2};
+        gptr((const int4 *)d.ivgk), gptr(d.vgpc), S.ist, S.lo0, S.lo1, S.lo2, S.n0, S.n1, S.n2};
     const EXH_G double2 *X = gptr(d.x);
     const EXH_G double2 *YB = gptr((const double2 *)d.bt);
+    const bool use_a2t = S.have_h1aa != 0;
     const EXH_G double2 *YA = use_a2t ? gptr((const double2 *)d.a2t) : X;
     const long long lda = use_a2t ? ldb : ldx;
     ho_tile<TMI, TMJ, TK>(d.ngp, Ktot, X, ldx, YB, ldb, YA, lda, box, gptr(d.h), gptr(d.o),
@@ -565,18 +570,6 @@ __global__ __launch_bounds__(64 * (TMI / 32) * (TMJ / 32)) void ho_grouped_kerne
 //                            O(G, lo cols) = sum_io conj(A_io,lm s_io) (+ mirror)
 // with s = oalo + h1loa.
 // ---------------------------------------------------------------------------
-struct LoTables {
-  const long long *colA;
-  const int *Koff, *Ms, *atom_species, *losize, *lobase;
-  const double2 *loa;
-  int maxnlo, maxaa;
-  const int *nlorb, *lorbl;
-  int nlomax;
-  const int *apword;
-  int lmaxapw, pmax, lmmax, ngkmax;
-  const double *salo;
-};
-
 template <int MAXLO, typename PA, typename PH>
 __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp, PA __restrict__ apwalm,
                                                const LoTables &t, PH __restrict__ H,
@@ -662,11 +655,14 @@ __global__ __launch_bounds__(64) void lo_border_kernel(int ngp, const double2 *_
                         accumulate);
 }
 
-// Grouped: blockIdx.z selects the k-point.
+// Grouped: blockIdx.z selects the k-point, blockIdx.y the atom.
 template <int MAXLO>
 __global__ __launch_bounds__(64) void lo_border_grouped_kernel(const KDesc *__restrict__ kd,
-                                                               LoTables t, int accumulate) {
+                                                               int accumulate) {
   const KDesc &d = kd[blockIdx.z];
+  const SDesc &S = *d.sd;
+  if ((int)blockIdx.y >= S.natmtot || S.nlotot == 0) return;
+  const LoTables t = S.lo;
   lo_border_body<MAXLO>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp,
                         gptr((const double2 *)d.apwalm), t, gptr(d.h), gptr(d.o), d.ldh,
                         accumulate);
@@ -721,16 +717,12 @@ __global__ void lolo_kernel(int ngp, int nlotot, const LoRow *__restrict__ lorow
             nlotot, lorow, lobase, lolo, maxnlo, ololo_eff, nlomax, H, O, ldh, accumulate);
 }
 
-__global__ void lolo_grouped_kernel(const KDesc *__restrict__ kd, int nlotot,
-                                    const LoRow *__restrict__ lorow,
-                                    const int *__restrict__ lobase,
-                                    const double2 *__restrict__ lolo, int maxnlo,
-                                    const double *__restrict__ ololo_eff, int nlomax,
-                                    int accumulate) {
+__global__ void lolo_grouped_kernel(const KDesc *__restrict__ kd, int accumulate) {
   const KDesc &d = kd[blockIdx.z];
+  const SDesc &S = *d.sd;
   lolo_body(blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4), d.ngp,
-            nlotot, lorow, lobase, lolo, maxnlo, ololo_eff, nlomax, gptr(d.h), gptr(d.o),
-            d.ldh, accumulate);
+            S.nlotot, S.lorow, S.lobase, S.lolo, S.maxnlo, S.ololo_eff, S.nlomax, gptr(d.h),
+            gptr(d.o), d.ldh, accumulate);
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
@@ -930,67 +922,99 @@ hipError_t launch_mirror(hipStream_t s, int n, double2 *m, int ld) {
 const HoShapeInfo kHoShapeInfo[kHoShapes] = {
     {64, 64, 16}, {64, 64, 8}, {128, 64, 16}, {128, 64, 8}, {64, 128, 8}};
 
-hipError_t launch_ivgk_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                               int ngpmax) {
-  if (nk <= 0 || ngpmax <= 0) return hipSuccess;
-  hipLaunchKernelGGL(ivgk_grouped_kernel, dim3(cdiv(ngpmax, 256), nk), dim3(256), 0, s, kd,
-                     (const int *)c->ivg.p, c->ngrtot);
+// The device addresses and scalars of a context's tables, as the grouped
+// kernels read them (one per structure of a grouped launch).
+SDesc make_sdesc(const exh_context *c) {
+  SDesc S;
+  std::memset(&S, 0, sizeof S);
+  const int *ki = (const int *)c->katom.p;  // [Koff(natmtot), Ms(natmtot), kinfo(Ktot)]
+  S.natmtot = c->natmtot;
+  S.Ktot = c->Ktot;
+  S.maxaa = c->maxaa;
+  S.lmaxapw = c->lmaxapw;
+  S.pmax = c->apwordmax;
+  S.lmmax = c->lmmax;
+  S.ngkmax = c->ngkmax;
+  S.have_h1aa = c->have_h1aa ? 1 : 0;
+  S.ldb = (c->ngkmax + 63) / 64 * 64;
+  S.ldx = c->contig ? c->ngkmax : S.ldb;
+  S.colA = (const long long *)c->colA.p;
+  S.Koff = ki;
+  S.Ms = ki ? ki + c->natmtot : nullptr;
+  S.kinfo = ki ? ki + 2 * c->natmtot : nullptr;
+  S.aa = (const double2 *)c->aa.p;
+  S.h1aa = (const double *)c->h1aa.p;
+  S.atom_species = (const int *)c->atom_species_d.p;
+  S.apword = (const int *)c->apword_d.p;
+  S.omax = (const int *)c->omax_d.p;
+  S.piv = (const int *)c->mt_match.piv.p;
+  S.rmt = (const double *)c->rmt_d.p;
+  S.lu = (const double *)c->mt_match.lu.p;
+  S.fpso = c->omega > 0.0 ? 12.566370614359172954 / sqrt(c->omega) : 0.0;
+  S.ivg = (const int *)c->ivg.p;
+  S.ngrtot = c->ngrtot;
+  S.ist = (const IstEntry *)c->ist.p;
+  S.lo0 = c->intgv[0];
+  S.lo1 = c->intgv[1];
+  S.lo2 = c->intgv[2];
+  S.n0 = c->intgv[3] - c->intgv[0] + 1;
+  S.n1 = c->intgv[4] - c->intgv[1] + 1;
+  S.n2 = c->intgv[5] - c->intgv[2] + 1;
+  S.lo = lo_tables(c);
+  S.nlotot = c->nlotot;
+  S.maxnlo = c->maxnlo;
+  S.nlomax = c->nlomax;
+  S.lorow = (const LoRow *)c->lorow.p;
+  S.lobase = (const int *)c->lobase_d.p;
+  S.lolo = (const double2 *)c->lolo.p;
+  S.ololo_eff = (const double *)c->ololo_eff.p;
+  return S;
+}
+
+hipError_t launch_ivgk_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd) {
+  if (g.nk <= 0 || g.ngpmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ivgk_grouped_kernel, dim3(cdiv(g.ngpmax, 256), g.nk), dim3(256), 0, s, kd);
   return hipGetLastError();
 }
 
-hipError_t launch_pack_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                               int ngpmax, int ldb) {
-  if (nk <= 0 || ngpmax <= 0 || c->contig || c->Ktot == 0) return hipSuccess;
-  hipLaunchKernelGGL(pack_grouped_kernel, dim3(cdiv(ngpmax, 256), c->Ktot, nk), dim3(256), 0, s,
-                     kd, (const long long *)c->colA.p, ldb);
+hipError_t launch_pack_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd) {
+  if (g.nk <= 0 || g.ngpmax <= 0 || !g.any_pack || g.Kmax == 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_grouped_kernel, dim3(cdiv(g.ngpmax, 256), g.Kmax, g.nk), dim3(256), 0,
+                     s, kd);
   return hipGetLastError();
 }
 
-hipError_t launch_bt_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                             int ngpmax, long long ldx, int ldb) {
-  if (nk <= 0 || ngpmax <= 0 || c->natmtot <= 0) return hipSuccess;
-  dim3 grid(cdiv(ngpmax, TM), cdiv(c->maxaa, TM), c->natmtot * nk);
-  hipLaunchKernelGGL(bt_grouped_kernel, grid, dim3(NTH), 0, s, kd, c->natmtot, ldx,
-                     (const int *)c->katom.p, (const int *)c->katom.p + c->natmtot,
-                     (const double2 *)c->aa.p, c->maxaa, ldb);
+hipError_t launch_bt_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd) {
+  if (g.nk <= 0 || g.ngpmax <= 0 || g.natmax <= 0) return hipSuccess;
+  dim3 grid(cdiv(g.ngpmax, TM), cdiv(g.maxaa, TM), g.natmax * g.nk);
+  hipLaunchKernelGGL(bt_grouped_kernel, grid, dim3(NTH), 0, s, kd, g.natmax);
   return hipGetLastError();
 }
 
-hipError_t launch_a2t_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                              int ngpmax, int ldb) {
-  if (nk <= 0 || ngpmax <= 0 || c->natmtot <= 0) return hipSuccess;
-  dim3 grid(cdiv(ngpmax, 256), c->maxaa, c->natmtot * nk);
-  hipLaunchKernelGGL(a2t_grouped_kernel, grid, dim3(256), 0, s, kd, c->natmtot,
-                     (const long long *)c->colA.p, (const int *)c->katom.p,
-                     (const int *)c->katom.p + c->natmtot,
-                     (const int *)c->katom.p + 2 * c->natmtot, (const double *)c->h1aa.p,
-                     c->apwordmax, c->lmaxapw, ldb);
+hipError_t launch_a2t_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd) {
+  if (g.nk <= 0 || g.ngpmax <= 0 || g.natmax <= 0 || !g.any_a2t) return hipSuccess;
+  dim3 grid(cdiv(g.ngpmax, 256), g.maxaa, g.natmax * g.nk);
+  hipLaunchKernelGGL(a2t_grouped_kernel, grid, dim3(256), 0, s, kd, g.natmax);
   return hipGetLastError();
 }
 
 template <int TMI, int TMJ, int TK>
-static hipError_t ho_grouped_launch(hipStream_t s, const exh_context *c, const KDesc *kd,
-                                    const int4 *tiles, int ntiles, int nwg, long long ldx,
-                                    int ldb, int accumulate) {
-  const int lo0 = c->intgv[0], lo1 = c->intgv[1], lo2 = c->intgv[2];
-  const int n0 = c->intgv[3] - lo0 + 1, n1 = c->intgv[4] - lo1 + 1, n2 = c->intgv[5] - lo2 + 1;
+static hipError_t ho_grouped_launch(hipStream_t s, const KDesc *kd, const int4 *tiles,
+                                    int ntiles, int nwg, int accumulate) {
   hipLaunchKernelGGL((ho_grouped_kernel<TMI, TMJ, TK>), dim3(nwg),
-                     dim3(HoShape<TMI, TMJ, TK>::NTH), 0, s, kd, tiles, ntiles, c->Ktot, ldx,
-                     (long long)ldb, c->have_h1aa ? 1 : 0, (const IstEntry *)c->ist.p, lo0,
-                     lo1, lo2, n0, n1, n2, accumulate);
+                     dim3(HoShape<TMI, TMJ, TK>::NTH), 0, s, kd, tiles, ntiles, accumulate);
   return hipGetLastError();
 }
 
-hipError_t launch_ho_grouped(hipStream_t s, const exh_context *c, int shape, const KDesc *kd,
-                             const int4 *tiles, int ntiles, int nwg, long long ldx, int ldb,
-                             int accumulate) {
+hipError_t launch_ho_grouped(hipStream_t s, int shape, const KDesc *kd, const int4 *tiles,
+                             int ntiles, int nwg, int accumulate) {
   if (ntiles <= 0 || nwg <= 0) return hipSuccess;
   switch (shape) {
-    case 0: return ho_grouped_launch<64, 64, 16>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
-    case 1: return ho_grouped_launch<64, 64, 8>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
-    case 2: return ho_grouped_launch<128, 64, 16>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
-    case 3: return ho_grouped_launch<128, 64, 8>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
-    case 4: return ho_grouped_launch<64, 128, 8>(s, c, kd, tiles, ntiles, nwg, ldx, ldb, accumulate);
+    case 0: return ho_grouped_launch<64, 64, 16>(s, kd, tiles, ntiles, nwg, accumulate);
+    case 1: return ho_grouped_launch<64, 64, 8>(s, kd, tiles, ntiles, nwg, accumulate);
+    case 2: return ho_grouped_launch<128, 64, 16>(s, kd, tiles, ntiles, nwg, accumulate);
+    case 3: return ho_grouped_launch<128, 64, 8>(s, kd, tiles, ntiles, nwg, accumulate);
+    case 4: return ho_grouped_launch<64, 128, 8>(s, kd, tiles, ntiles, nwg, accumulate);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1015,28 +1039,24 @@ int ho_grouped_occupancy(int shape) {
   return e == hipSuccess && n > 0 ? n : 1;
 }
 
-hipError_t launch_lo_border_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
-                                    int ngpmax, int accumulate) {
-  if (nk <= 0 || ngpmax <= 0 || c->nlotot == 0) return hipSuccess;
-  dim3 grid(cdiv(ngpmax, 64), c->natmtot, nk);
-  const LoTables t = lo_tables(c);
-  if (c->maxnlo <= 16)
-    hipLaunchKernelGGL(lo_border_grouped_kernel<16>, grid, dim3(64), 0, s, kd, t, accumulate);
-  else if (c->maxnlo <= 32)
-    hipLaunchKernelGGL(lo_border_grouped_kernel<32>, grid, dim3(64), 0, s, kd, t, accumulate);
+hipError_t launch_lo_border_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd,
+                                    int accumulate) {
+  if (g.nk <= 0 || g.ngpmax <= 0 || g.nlotot == 0 || g.natmax <= 0) return hipSuccess;
+  dim3 grid(cdiv(g.ngpmax, 64), g.natmax, g.nk);
+  if (g.maxnlo <= 16)
+    hipLaunchKernelGGL(lo_border_grouped_kernel<16>, grid, dim3(64), 0, s, kd, accumulate);
+  else if (g.maxnlo <= 32)
+    hipLaunchKernelGGL(lo_border_grouped_kernel<32>, grid, dim3(64), 0, s, kd, accumulate);
   else
-    hipLaunchKernelGGL(lo_border_grouped_kernel<64>, grid, dim3(64), 0, s, kd, t, accumulate);
+    hipLaunchKernelGGL(lo_border_grouped_kernel<64>, grid, dim3(64), 0, s, kd, accumulate);
   return hipGetLastError();
 }
 
-hipError_t launch_lolo_grouped(hipStream_t s, const exh_context *c, const KDesc *kd, int nk,
+hipError_t launch_lolo_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd,
                                int accumulate) {
-  if (nk <= 0 || c->nlotot == 0) return hipSuccess;
-  dim3 grid(cdiv(c->nlotot, 16), cdiv(c->nlotot, 16), nk);
-  hipLaunchKernelGGL(lolo_grouped_kernel, grid, dim3(256), 0, s, kd, c->nlotot,
-                     (const LoRow *)c->lorow.p, (const int *)c->lobase_d.p,
-                     (const double2 *)c->lolo.p, c->maxnlo, (const double *)c->ololo_eff.p,
-                     c->nlomax, accumulate);
+  if (g.nk <= 0 || g.nlotot == 0) return hipSuccess;
+  dim3 grid(cdiv(g.nlotot, 16), cdiv(g.nlotot, 16), g.nk);
+  hipLaunchKernelGGL(lolo_grouped_kernel, grid, dim3(256), 0, s, kd, accumulate);
   return hipGetLastError();
 }
 

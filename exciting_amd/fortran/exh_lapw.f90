@@ -51,7 +51,8 @@ module exh_lapw
   public :: exh_create, exh_destroy, exh_set_basis, exh_set_radial, exh_set_mt, &
             exh_set_gvectors, exh_set_interstitial, exh_match, &
             exh_hamiltonandoverlapsetup, exh_hamiltonandoverlapsetup_ex, &
-            exh_error_message, exh_split, exh_setup_batch_device, exh_tune_batch, &
+            exh_error_message, exh_split, exh_setup_batch_device, exh_setup_multi_device, &
+            exh_tune_batch, &
             exh_set_batch_plan, exh_get_batch_plan, exh_set_batch_workspace, &
             exh_set_interstitial_device, exh_gencfun, exh_gencfun_device, exh_genveffig, &
             exh_genmeffig, exh_genveffig_device, exh_hmlint, exh_hmlint_device, &
@@ -397,6 +398,17 @@ module exh_lapw
         bind(C, name="exh_setup_batch_device")
       import :: c_int, c_ptr, exh_kpoint
       type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: nk, flags
+      type(exh_kpoint), intent(in) :: kpts(*)
+    end function
+
+    ! k-points of several structures in one grouped launch: kpts(i) belongs
+    ! to the context ctxs(i).
+    integer(c_int) function exh_setup_multi_device(ctxs, stream, nk, kpts, flags) &
+        bind(C, name="exh_setup_multi_device")
+      import :: c_int, c_ptr, exh_kpoint
+      type(c_ptr), intent(in) :: ctxs(*)
+      type(c_ptr), value :: stream
       integer(c_int), value :: nk, flags
       type(exh_kpoint), intent(in) :: kpts(*)
     end function

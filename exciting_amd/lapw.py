@@ -83,6 +83,7 @@ def lib():
     L.exh_profile_read.argtypes = [_p, _i, C.POINTER(_d), C.POINTER(_i)]
     L.exh_profile_reset.argtypes = [_p]
     L.exh_setup_batch_device.argtypes = [_p, _p, _i, _p, _i]
+    L.exh_setup_multi_device.argtypes = [C.POINTER(_p), _p, _i, _p, _i]
     L.exh_batch_plan_count.argtypes = []
     L.exh_batch_plan_info.argtypes = [_i] + [C.POINTER(_i)] * 5
     L.exh_set_batch_plan.argtypes = [_p, _i]
@@ -128,7 +129,7 @@ EXPORTS = (
     "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
     "exh_mt_kinetic_device", "exh_mt_pot", "exh_mt_pot_device", "exh_mt_so",
     "exh_mt_so_device", "exh_setup_device_noalloc", "exh_set_apwalm_reuse",
-    "exh_apwalm_transfers",
+    "exh_apwalm_transfers", "exh_setup_multi_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
@@ -708,6 +709,21 @@ class Context:
     def set_batch_workspace(self, nbytes: int):
         self._check(lib().exh_set_batch_workspace(self._h, int(nbytes)),
                     "exh_set_batch_workspace")
+
+
+def setup_multi(ctxs, kpts, flags: int = ACCUMULATE, stream=None):
+    """One grouped match (with MATCH) + H/O setup over k-points of several
+    structures: ``kpts[i]`` (a KPoint) belongs to ``ctxs[i]`` (a Context).
+    Runs on ``stream``, or on the first context's stream ordered after
+    PyTorch's current stream as the single-context calls do."""
+    if len(ctxs) != len(kpts) or not ctxs:
+        raise ValueError("setup_multi: one context per k-point, at least one")
+    lead = ctxs[0]
+    arr = (KPoint * len(kpts))(*kpts)
+    handles = (_p * len(ctxs))(*[c._h for c in ctxs])
+    lead._check(lead._device_call(stream, lambda st: lib().exh_setup_multi_device(
+        handles, st, len(kpts), arr, flags)), "exh_setup_multi_device")
+    return arr
 
 
 def kpoint_struct(d: "DeviceKPoint", h, o, apwalm=None) -> KPoint:

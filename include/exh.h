@@ -361,6 +361,18 @@ enum {
  * larger batches run as consecutive groups. */
 int exh_setup_batch_device(exh_context *ctx, void *stream, int nk, const exh_kpoint *kpts,
                            int flags);
+/* The same over k-points of several structures at once: kpts[i] belongs to
+ * ctxs[i] (each context holds one structure's tables; all on one device).
+ * Every stage is one grouped launch over all k-points (match once per
+ * lmaxapw present), the FP64 H/O contraction one persistent launch over the
+ * upper tiles of all of them; per k-point the result is bitwise that of
+ * exh_setup_batch_device on its own context.  ctxs[0] supplies the plan,
+ * the default stream, the descriptor/graph cache and the profiling records;
+ * each context's batch workspace limit bounds its k-points per group.  No
+ * reference counterpart: the reference sets up one k-point of one structure
+ * per call (seceqnfv.f90:76-77); this is the serving loop's launch. */
+int exh_setup_multi_device(exh_context *const *ctxs, void *stream, int nk,
+                           const exh_kpoint *kpts, int flags);
 /* Plans of the grouped H/O kernel: tile shape (rows x columns x K slab),
  * persistent grid (one workgroup per resident slot on every CU, walking the
  * tile list) or one workgroup per tile, and tile order (k-major, or split
