@@ -284,14 +284,20 @@ def clean(r):
 def serving_sample(args):
     """Bounded sample of the serving loop: Poisson requests over the 2, 8,
     16, 32 and 64-atom cells (1..kmax k-points each, FP64 or split), each
-    match + setup + GPU eigensolve of the lowest states per k-point."""
+    match + setup + GPU eigensolve of the lowest states per k-point.  The
+    setups of the requests in flight run as grouped launches over several
+    cells; the admission budget is capped so that requests wait for HBM,
+    later ones bypass them and idle contexts are evicted."""
     from exciting_amd import serve, synth
     specs = synth.serving_cells(args.serve_kmax)
     cells = {name: synth.build(spec, kpoints=ks) for name, (spec, ks) in specs.items()}
-    make, close, execute = serve.gpu_executor(0, max_solves=args.serve_solves)
+    ex = serve.GpuExecutor(0, max_solves=args.serve_solves)
+    make, close, execute = ex.make_context, ex.close_context, ex.execute
     from exciting_amd import solve as _solve
     budget = (serve.free_hbm(0) - (48 << 30)
               - args.serve_solves * _solve.WORKSPACE_BYTES)   # the solvers' hipBLAS workspaces
+    if args.serve_budget_gib > 0:
+        budget = min(budget, int(args.serve_budget_gib * 2**30))
     srv = serve.Server(cells, execute, make, close, budget, max_inflight=8,
                        on_finish=lambda r: log(f"serve: request {r.rid} {r.cell} "
                                                f"nk={len(r.kpoints)} nmod={r.nmod} {r.status} "
@@ -302,15 +308,28 @@ def serving_sample(args):
     reqs = serve.poisson_requests(args.serve_requests, args.serve_rate, cells, seed=7,
                                   kmax=args.serve_kmax, nmod=args.serve_moduli,
                                   nstfv=args.serve_nstfv)
+    warm_stats = ex.stats()
     t0 = time.perf_counter()
     srv.run(reqs)
     out = serve.summarize(reqs, time.perf_counter() - t0)
+    st = ex.stats()
+    launches = st["launches"] - warm_stats["launches"]
+    kpts = st["kpoints"] - warm_stats["kpoints"]
     out.update({"sample": (f"{args.serve_requests} Poisson requests at {args.serve_rate}/s over "
                            f"fcc2/sc8/ortho16/sc32/sc64, 1..{args.serve_kmax} k-points each, "
                            f"half split ({args.serve_moduli} moduli), {args.serve_nstfv} states, "
-                           f"up to {args.serve_solves} eigensolves at once"),
-                "peak_inflight": srv.peak_inflight, "contexts": srv.pool.created})
+                           f"up to {args.serve_solves} eigensolves at once, admission budget "
+                           f"{budget / 2**30:.0f} GiB"),
+                "budget_gib": budget / 2**30,
+                "peak_inflight": srv.peak_inflight, "contexts": srv.pool.created,
+                "evictions": srv.pool.evicted, "bypassed": srv.bypassed,
+                "hbm_waits": srv.hbm_waits, "peak_context_gib": srv.peak_bytes / 2**30,
+                "setup_launches": launches,
+                "kpoints_per_launch": kpts / max(launches, 1),
+                "max_kpoints_per_launch": st["kpoints_per_launch"]["max"],
+                "max_cells_per_launch": st["cells_per_launch"]["max"]})
     srv.pool.close_all()
+    ex.close()
     return out
 
 
@@ -679,9 +698,12 @@ def main():
     ap.add_argument("--uneven", choices=("on", "off"), default="on",
                     help="also run the 16-atom cell with 128 uneven k-points as one batch")
     ap.add_argument("--uneven-moduli", default="auto")
-    ap.add_argument("--serve-requests", type=int, default=16,
+    ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests in the serving sample (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-rate", type=float, default=3.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=80.0,
+                    help="admission budget cap of the serving sample (two 64-atom contexts "
+                         "and a few small ones; 0 = free HBM minus the reserve)")
     ap.add_argument("--serve-kmax", type=int, default=4)
     ap.add_argument("--serve-nstfv", type=int, default=40)
     ap.add_argument("--serve-solves", type=int, default=4,

@@ -69,6 +69,9 @@ class Request:
     error: str = ""
     eigenvalues: Optional[List[np.ndarray]] = None
     digests: Optional[List[str]] = None   # per k: hash of H and O (if asked)
+    retries_setup: int = 0   # setup launches rerun after an out-of-memory error
+    retries_solve: int = 0   # eigensolves rerun after an out-of-memory error
+    launches: int = 0        # grouped setup launches its k-points were part of
 
     @property
     def latency(self) -> float:
@@ -169,6 +172,9 @@ class Server:
         self.cancelled = set()
         self.peak_inflight = 0
         self.peak_bytes = 0
+        self.bypassed = 0        # admitted while an earlier arrival waited for HBM
+        self.hbm_waits = 0       # requests that had to wait for HBM at least once
+        self._waited = set()
 
     def cancel(self, rid: int) -> None:
         """Cancel a request that has not been admitted yet."""
@@ -232,6 +238,7 @@ class Server:
             while nxt < len(arrivals) and arrivals[nxt].arrival <= now:
                 queue.append(arrivals[nxt])
                 nxt += 1
+            blocked = False
             for req in list(queue):
                 with self.lock:
                     if req.rid in self.cancelled:
@@ -249,7 +256,13 @@ class Server:
                         queue.remove(req)
                         continue
                     if got is None:
+                        blocked = True
+                        if req.rid not in self._waited:
+                            self._waited.add(req.rid)
+                            self.hbm_waits += 1
                         continue  # bypass: later requests may fit
+                    if blocked:
+                        self.bypassed += 1
                     req.status = "running"
                     req.admitted = now
                     queue.remove(req)
@@ -270,27 +283,189 @@ class Server:
 # ---------------------------------------------------------------------------
 # GPU executor
 # ---------------------------------------------------------------------------
-def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
-    """(make_context, close_context, execute) running the library and the
-    eigensolve on ``cuda:device``.  At most ``max_solves`` eigensolves run at
-    once.  With ``digests`` each request also records a SHA-1 of the bytes of
-    H and O per k-point (copies them to the host: for tests on small
-    cells)."""
-    from . import solve as _solve
-    _solve.configure()
-    import torch
-    from . import lapw
-    dev = f"cuda:{device}"
-    # long-lived solver threads, one stream each: at most max_solves hipBLAS
-    # workspaces, however many contexts and request threads come and go
-    solvers = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, max_solves),
-                                                    thread_name_prefix="exh-solve")
-    local = threading.local()
+_SOLVER_POOLS: Dict[tuple, tuple] = {}
+_SOLVER_LOCK = threading.Lock()
 
-    def solve_on_worker(ready, H, O, n, nst):
-        st = getattr(local, "stream", None)
+
+def _solver_pool(device: int, max_solves: int):
+    """The process's solver threads for one device, created once and shared by
+    every executor: at most ``max_solves`` (thread, stream) pairs run BLAS,
+    so PyTorch holds at most that many hipBLAS workspaces however many
+    executors, contexts and request threads come and go."""
+    key = (device, max(1, max_solves))
+    with _SOLVER_LOCK:
+        if key not in _SOLVER_POOLS:
+            pool = concurrent.futures.ThreadPoolExecutor(max_workers=key[1],
+                                                         thread_name_prefix="exh-solve")
+            _SOLVER_POOLS[key] = (pool, threading.local())
+        return _SOLVER_POOLS[key]
+
+
+@dataclasses.dataclass
+class _Setup:
+    req: Request
+    ctx: object
+    d: object            # lapw.DeviceKPoint
+    flags: int
+    fut: concurrent.futures.Future
+
+
+class Batcher:
+    """Merges the k-point setups of every request in flight into grouped
+    launches.  Request threads submit one k-point at a time (match + H/O
+    setup into their context's apwalm, H and O); the batcher thread takes
+    whatever is pending -- up to ``max_k`` k-points, of any cells -- and
+    runs it as one exh_setup_multi_device call per precision on its own
+    stream, then waits for that launch.  Setups that arrive meanwhile queue
+    and form the next launch, so the batch grows with the load; when the
+    GPU is idle the first pending setup waits at most ``linger_s`` for others
+    to join it.  Per k-point the result is bitwise what the single-context
+    path gives, whatever else shares the launch."""
+
+    def __init__(self, device: int = 0, max_k: int = 32, plan: int = -1,
+                 linger_s: float = 0.002):
+        import torch
+        self.device = device
+        self.max_k = max_k
+        self.plan = plan
+        self.linger_s = linger_s
+        self.stream = torch.cuda.Stream(device=f"cuda:{device}")
+        self.cv = threading.Condition()
+        self.q: List[_Setup] = []
+        self.closed = False
+        self.launches = 0
+        self.kpoints = 0
+        self.max_kpoints = 0
+        self.cells = 0          # sum over launches of distinct cells
+        self.max_cells = 0
+        self.requests = 0       # sum over launches of distinct requests
+        self.retries = 0
+        self.busy_s = 0.0
+        self.thread = threading.Thread(target=self._loop, name="exh-batcher", daemon=True)
+        self.thread.start()
+
+    def submit(self, req: Request, ctx, d, flags: int) -> concurrent.futures.Future:
+        """Queue match + setup of one k-point; the future gives the event
+        that completes with it."""
+        fut: concurrent.futures.Future = concurrent.futures.Future()
+        with self.cv:
+            if self.closed:
+                raise RuntimeError("batcher closed")
+            self.q.append(_Setup(req, ctx, d, flags, fut))
+            self.cv.notify()
+        return fut
+
+    def close(self) -> None:
+        with self.cv:
+            self.closed = True
+            self.cv.notify()
+        self.thread.join()
+
+    def _loop(self) -> None:
+        while True:
+            with self.cv:
+                while not self.q and not self.closed:
+                    self.cv.wait()
+                if not self.q:
+                    return
+                deadline = time.perf_counter() + self.linger_s
+                while len(self.q) < self.max_k and not self.closed:
+                    rem = deadline - time.perf_counter()
+                    if rem <= 0:
+                        break
+                    self.cv.wait(rem)
+                items, self.q = self.q[:self.max_k], self.q[self.max_k:]
+            groups: Dict[int, List[_Setup]] = {}
+            for it in items:
+                groups.setdefault(it.flags, []).append(it)
+            for flags, its in groups.items():
+                try:
+                    ev = self._launch(its, flags)
+                except Exception as e:  # fails the requests of this launch only
+                    for it in its:
+                        it.fut.set_exception(e)
+                    continue
+                for it in its:
+                    it.fut.set_result(ev)
+
+    def _launch(self, its: List[_Setup], flags: int):
+        import torch
+        from . import lapw
+        kp = [lapw.kpoint_struct(it.d, it.ctx.H, it.ctx.O, apwalm=it.ctx.apwalm) for it in its]
+        ctxs = [it.ctx for it in its]
+        lead = ctxs[0]
+        t0 = time.perf_counter()
+
+        def run():
+            if self.plan >= 0:
+                lead.batch_plan = self.plan
+            lapw.setup_multi(ctxs, kp, flags=flags | lapw.MATCH,
+                             stream=self.stream.cuda_stream)
+
+        try:
+            run()
+        except lapw.ExhError as e:
+            # The contexts' scratch grows on first use of a larger setup (the
+            # first split one, say).  Blocks PyTorch's caching allocator holds
+            # free for the eigensolves are not visible to hipMalloc: release
+            # them and run the launch once more (it overwrites H and O).
+            if "out of memory" not in str(e):
+                raise
+            for it in its:
+                it.req.retries_setup += 1
+            self.retries += 1
+            torch.cuda.synchronize(self.device)
+            torch.cuda.empty_cache()
+            run()
+        ev = torch.cuda.Event()
+        ev.record(self.stream)
+        ev.synchronize()        # setups arriving meanwhile form the next launch
+        self.busy_s += time.perf_counter() - t0
+        ncell = len({id(it.ctx.sys) for it in its})
+        self.launches += 1
+        self.kpoints += len(its)
+        self.max_kpoints = max(self.max_kpoints, len(its))
+        self.cells += ncell
+        self.max_cells = max(self.max_cells, ncell)
+        self.requests += len({it.req.rid for it in its})
+        for it in its:
+            it.req.launches += 1
+        return ev
+
+    def stats(self) -> dict:
+        n = max(self.launches, 1)
+        return {"launches": self.launches, "kpoints": self.kpoints,
+                "kpoints_per_launch": {"mean": self.kpoints / n, "max": self.max_kpoints},
+                "cells_per_launch": {"mean": self.cells / n, "max": self.max_cells},
+                "requests_per_launch": self.requests / n, "launch_retries": self.retries,
+                "busy_s": self.busy_s}
+
+
+class GpuExecutor:
+    """Runs requests on ``cuda:device``: the k-point setups of all requests
+    in flight go through one Batcher (grouped multi-structure launches), the
+    eigensolves through the process's solver threads for the device, at most
+    ``max_solves`` at once.  ``make_context``/``close_context``/``execute``
+    plug into Server.  With ``digests`` each request also records a SHA-1 of
+    the bytes of H and O per k-point (copies them to the host: for tests on
+    small cells)."""
+
+    def __init__(self, device: int = 0, digests: bool = False, max_solves: int = 4,
+                 max_batch_k: int = 32, plan: int = -1, linger_s: float = 0.002):
+        from . import solve as _solve
+        _solve.configure()
+        self.device = device
+        self.dev = f"cuda:{device}"
+        self.digests = digests
+        self.solvers, self._local = _solver_pool(device, max_solves)
+        self.batcher = Batcher(device, max_k=max_batch_k, plan=plan, linger_s=linger_s)
+
+    def _solve_on_worker(self, req, ready, H, O, n, nst):
+        import torch
+        from . import solve as _solve
+        st = getattr(self._local, "stream", None)
         if st is None:
-            st = local.stream = torch.cuda.Stream(device=dev)
+            st = self._local.stream = torch.cuda.Stream(device=self.dev)
         st.wait_event(ready)                   # the setup of H and O is done
         with torch.cuda.stream(st):
             try:
@@ -298,7 +473,8 @@ def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
             except torch.cuda.OutOfMemoryError:
                 # freed temporaries of other solves may sit in PyTorch's
                 # cache in pieces too small for this one
-                torch.cuda.synchronize(device)
+                req.retries_solve += 1
+                torch.cuda.synchronize(self.device)
                 torch.cuda.empty_cache()
                 w = _solve.geneig(H, O, n, nst, method="auto")
             out = w.cpu().numpy()              # waits for the solve
@@ -309,73 +485,64 @@ def gpu_executor(device: int = 0, digests: bool = False, max_solves: int = 4):
         torch.cuda.empty_cache()
         return out
 
-    def make_context(sys_):
-        ctx = lapw.Context(device).load(sys_)
+    def make_context(self, sys_):
+        import torch
+        from . import lapw
+        ctx = lapw.Context(self.device).load(sys_)
         ctx.kp = {}      # per-k inputs, uploaded on first use and kept
         n = sys_.ngkmax + sys_.nlotot
         lmm = (sys_.lmaxapw + 1) ** 2
-        ctx.tstream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+        ctx.tstream = torch.cuda.ExternalStream(ctx.stream, device=self.dev)
         with torch.cuda.stream(ctx.tstream):
             ctx.apwalm = torch.zeros(sys_.natmtot * lmm * sys_.apwordmax * sys_.ngkmax,
-                                     dtype=torch.complex128, device=dev)
-            ctx.H = torch.empty(n * n, dtype=torch.complex128, device=dev)
-            ctx.O = torch.empty(n * n, dtype=torch.complex128, device=dev)
+                                     dtype=torch.complex128, device=self.dev)
+            ctx.H = torch.empty(n * n, dtype=torch.complex128, device=self.dev)
+            ctx.O = torch.empty(n * n, dtype=torch.complex128, device=self.dev)
         ctx.tstream.synchronize()
         return ctx
 
-    def execute(req: Request, ctx):
+    def execute(self, req: Request, ctx):
+        import torch
+        from . import lapw
         s = ctx.sys
         out, dg = [], []
-        with torch.cuda.stream(ctx.tstream):
-            for ik in req.kpoints:
-                d = ctx.kp.get(ik)
-                if d is None:
-                    d = lapw.DeviceKPoint(s, ik, device=dev, with_apwalm=False)
-                    ctx.kp[ik] = d
-                nn = d.rank * d.rank
-                sysd = lapw.EvSystem(lapw.HermitianMatrix(d.rank, ctx.H[:nn]),
-                                     lapw.HermitianMatrix(d.rank, ctx.O[:nn]))
-
-                def setup():
-                    ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, ctx.apwalm, stream=ctx.stream)
-                    ctx.hamiltonandoverlapsetup(sysd, d.ngk, ctx.apwalm, d.igpig, d.vgpc,
-                                                flags=lapw.SPLIT(req.nmod) if req.nmod else 0,
-                                                stream=ctx.stream)
-                try:
-                    setup()
-                except lapw.ExhError as e:
-                    # The library's scratch grows on first use of a larger
-                    # setup (e.g. the first split one of a context).  Blocks
-                    # PyTorch's caching allocator holds free for the
-                    # eigensolves are not visible to hipMalloc: release them
-                    # and run the setup once more (it overwrites H and O).
-                    if "out of memory" not in str(e):
-                        raise
-                    torch.cuda.synchronize(device)
-                    torch.cuda.empty_cache()
-                    setup()
-                if digests:
-                    h = hashlib.sha1(ctx.H[:nn].cpu().numpy().tobytes())
-                    h.update(ctx.O[:nn].cpu().numpy().tobytes())
-                    dg.append(h.hexdigest())
-                ready = torch.cuda.Event()
-                ready.record(ctx.tstream)
-                # waits for the eigenvalues before the next k-point's setup
-                # overwrites H and O
-                out.append(solvers.submit(solve_on_worker, ready, ctx.H, ctx.O, d.rank,
-                                          min(req.nstfv, d.rank)).result())
-        if digests:
+        flags = lapw.SPLIT(req.nmod) if req.nmod else 0
+        for ik in req.kpoints:
+            d = ctx.kp.get(ik)
+            if d is None:
+                with torch.cuda.stream(ctx.tstream):
+                    d = lapw.DeviceKPoint(s, ik, device=self.dev, with_apwalm=False)
+                ctx.tstream.synchronize()     # uploaded before the batcher reads it
+                ctx.kp[ik] = d
+            # one k-point of this request in flight: its H and O are free
+            # again once the previous eigensolve has returned
+            ready = self.batcher.submit(req, ctx, d, flags).result()
+            nn = d.rank * d.rank
+            if self.digests:
+                ready.synchronize()
+                h = hashlib.sha1(ctx.H[:nn].cpu().numpy().tobytes())
+                h.update(ctx.O[:nn].cpu().numpy().tobytes())
+                dg.append(h.hexdigest())
+            out.append(self.solvers.submit(self._solve_on_worker, req, ready, ctx.H, ctx.O,
+                                           d.rank, min(req.nstfv, d.rank)).result())
+        if self.digests:
             req.digests = dg
         return out
 
-    def close_context(ctx):
+    def close_context(self, ctx):
+        import torch
         ctx.tstream.synchronize()
         ctx.close()
         ctx.kp.clear()
         ctx.apwalm = ctx.H = ctx.O = None
         torch.cuda.empty_cache()
 
-    return make_context, close_context, execute
+    def stats(self) -> dict:
+        return self.batcher.stats()
+
+    def close(self) -> None:
+        """Stop the batcher thread (the solver threads stay for the process)."""
+        self.batcher.close()
 
 
 def free_hbm(device: int = 0) -> int:
@@ -429,6 +596,9 @@ def summarize(reqs: List[Request], wall: float) -> dict:
                       "max": float(lat.max())},
         "queue_wait_s": {"p50": float(np.percentile(wait, 50)),
                          "p99": float(np.percentile(wait, 99))},
+        "retries_setup": int(sum(r.retries_setup for r in reqs)),
+        "retries_solve": int(sum(r.retries_solve for r in reqs)),
+        "launches_per_request": (float(np.mean([r.launches for r in done])) if done else 0.0),
     }
 
 
@@ -445,6 +615,8 @@ def main(argv=None) -> int:
     ap.add_argument("--max-inflight", type=int, default=8)
     ap.add_argument("--max-solves", type=int, default=4,
                     help="eigensolves allowed to run at once")
+    ap.add_argument("--max-batch-k", type=int, default=32,
+                    help="k-points one grouped setup launch takes at most")
     ap.add_argument("--reserve-gib", type=float, default=48.0,
                     help="HBM left outside the admission budget (library and "
                          "allocator working room)")
@@ -463,7 +635,8 @@ def main(argv=None) -> int:
     print(f"[serve] cells built in {time.perf_counter() - t:.1f} s: "
           + ", ".join(f"{k} ngk<={c.ngkmax} nk={len(c.kpts)}" for k, c in cells.items()),
           file=sys.stderr, flush=True)
-    make, close, execute = gpu_executor(0, max_solves=args.max_solves)
+    ex = GpuExecutor(0, max_solves=args.max_solves, max_batch_k=args.max_batch_k)
+    make, close, execute = ex.make_context, ex.close_context, ex.execute
     budget = (free_hbm(0) - int(args.reserve_gib * 2**30)
               - args.max_solves * _solve.WORKSPACE_BYTES)   # the solvers' hipBLAS workspaces
     def report(r):
@@ -496,10 +669,12 @@ def main(argv=None) -> int:
                 "torch_allocated_gib_at_end": _torch_allocated() / 2**30,
                 "torch_peak_allocated_gib": _torch_allocated(peak=True) / 2**30,
                 "peak_inflight": srv.peak_inflight, "contexts_created": srv.pool.created,
-                "contexts_evicted": srv.pool.evicted,
-                "peak_context_gib": srv.peak_bytes / 2**30})
+                "contexts_evicted": srv.pool.evicted, "bypassed": srv.bypassed,
+                "hbm_waits": srv.hbm_waits,
+                "peak_context_gib": srv.peak_bytes / 2**30, "batching": ex.stats()})
     print(json.dumps(out), flush=True)
     srv.pool.close_all()
+    ex.close()
     return 0 if out["failed"] == 0 else 1
 
 

@@ -2,7 +2,9 @@
 
 A fixed list of setup + eigensolve requests over three cells (FP64 and split
 precision, 1-4 k-points each) is served twice: one request at a time, then
-all at once with up to six in flight on their own streams.  Tolerance: the H
+all at once with up to six in flight, their k-point setups merged into
+grouped launches over several cells (exh_setup_multi_device) and their
+eigensolves overlapping on the solver threads.  Tolerance: the H
 and O matrices must be bitwise identical (SHA-1 of their bytes per k-point),
 and the eigenvalues identical to 1e-12 Ha (they come from the same library
 on bitwise identical inputs).  The FP64 eigenvalues are also checked against
@@ -37,7 +39,8 @@ def _requests():
 @pytest.fixture(scope="module")
 def served():
     cells = _cells()
-    make, close, execute = serve.gpu_executor(0, digests=True)
+    ex = serve.GpuExecutor(0, digests=True)
+    make, close, execute = ex.make_context, ex.close_context, ex.execute
     budget = serve.free_hbm(0) - (16 << 30)
     alone = []
     srv = serve.Server(cells, execute, make, close, budget, max_inflight=1)
@@ -48,25 +51,48 @@ def served():
         srv.run([r])
         alone.append(r)
     srv.pool.close_all()
+    alone_stats = ex.stats()
+    ex.close()
     print("[gpu-serve] mix of 10 requests, up to 6 in flight", flush=True)
     mix = _requests()
-    srv2 = serve.Server(cells, execute, make, close, budget, max_inflight=6)
+    # the ten requests arrive within 20 ms: a 20 ms linger lets the setups
+    # of concurrent requests of different cells share launches
+    ex2 = serve.GpuExecutor(0, digests=True, linger_s=0.02)
+    srv2 = serve.Server(cells, ex2.execute, ex2.make_context, ex2.close_context, budget,
+                        max_inflight=6)
     srv2.run(mix)
     srv2.pool.close_all()
-    yield cells, alone, mix, srv2
+    mix_stats = ex2.stats()
+    ex2.close()
+    print(f"  batching alone: {alone_stats}\n  batching in the mix: {mix_stats}", flush=True)
+    yield cells, alone, mix, srv2, alone_stats, mix_stats
 
 
 @pytest.mark.gpu
 def test_every_request_succeeds_and_overlaps(served):
-    cells, alone, mix, srv = served
+    cells, alone, mix, srv, alone_stats, mix_stats = served
     for r in alone + mix:
         assert r.status == "done", (r.rid, r.error)
+        assert r.launches == len(r.kpoints)
     assert srv.peak_inflight > 1
 
 
 @pytest.mark.gpu
+def test_requests_of_several_cells_share_launches(served):
+    """In the mix the setups of concurrent requests run as grouped launches
+    over several cells: fewer launches than k-points, and at least one
+    launch holding k-points of two or more cells."""
+    cells, alone, mix, srv, alone_stats, mix_stats = served
+    nk = sum(len(r.kpoints) for r in mix)
+    assert alone_stats["kpoints_per_launch"]["max"] == 1
+    assert mix_stats["kpoints"] == nk
+    assert mix_stats["launches"] < nk
+    assert mix_stats["cells_per_launch"]["max"] >= 2
+
+
+@pytest.mark.gpu
 def test_same_result_alone_and_in_the_mix(served):
-    cells, alone, mix, srv = served
+    cells, alone, mix, srv, _, _ = served
     nbit = 0
     for a, m in zip(alone, mix):
         assert a.rid == m.rid
@@ -79,7 +105,7 @@ def test_same_result_alone_and_in_the_mix(served):
 
 @pytest.mark.gpu
 def test_served_eigenvalues_match_the_oracle(served):
-    cells, alone, mix, srv = served
+    cells, alone, mix, srv, _, _ = served
     ref = {}
     for r in mix:
         s = cells[r.cell]
@@ -105,18 +131,38 @@ def test_solver_workspaces_do_not_grow_with_contexts():
     once the first solve has made its workspace."""
     import threading
     import torch
+    from exciting_amd import solve
     cell = _cases.system("fcc2")
-    make, close, execute = serve.gpu_executor(0, max_solves=2)
+    fresh_pool = (0, 2) not in serve._SOLVER_POOLS
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated(0)
     held = []
     for i in range(5):
         print(f"[gpu-serve] context {i}", flush=True)
-        ctx = make(cell)
+        # a new executor each time: they share the process's solver threads
+        ex = serve.GpuExecutor(0, max_solves=2)
+        ctx = ex.make_context(cell)
         req = serve.Request(rid=i, cell="fcc2", kpoints=[0], nmod=0, nstfv=20)
-        t = threading.Thread(target=lambda: execute(req, ctx))
+        res = {}
+
+        def run():
+            try:
+                res["w"] = ex.execute(req, ctx)
+            except BaseException as e:     # surfaced below, not lost in the thread
+                res["e"] = e
+        t = threading.Thread(target=run)
         t.start()
         t.join()
-        close(ctx)
+        ex.close_context(ctx)
+        ex.close()
+        assert "e" not in res, res.get("e")
+        assert len(res["w"]) == 1 and len(res["w"][0]) == 20
         torch.cuda.synchronize()
         held.append(torch.cuda.memory_allocated(0))
-    print(f"  PyTorch allocated after each context: {[h / 2**30 for h in held]} GiB")
+    print(f"  PyTorch allocated after each context: {[h / 2**30 for h in held]} GiB "
+          f"(before: {before / 2**30:.2f} GiB)")
+    # the first solve on a new solver thread made a workspace, and none of
+    # the later contexts and executors added one
+    if fresh_pool:
+        assert held[0] - before >= solve.WORKSPACE_BYTES
     assert held[-1] <= held[0]

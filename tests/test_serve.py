@@ -124,6 +124,7 @@ def test_bypass_lets_small_requests_pass_a_waiting_large_one():
     assert r2.admitted < r1.admitted
     assert r1.admitted >= r0.finished - 1e-3     # big waited for memory
     assert h.max_live_bytes <= 89
+    assert h.srv.bypassed == 1 and h.srv.hbm_waits == 1
 
 
 def test_request_larger_than_budget_fails_alone():
