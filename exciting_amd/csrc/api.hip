@@ -324,7 +324,7 @@ void exh_destroy(exh_context *c) {
                     &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
                     &c->aa, &c->loa, &c->lolo, &c->salo, &c->ololo_eff, &c->h1aa,
                     &c->ivg, &c->ivgig, &c->ist, &c->cf_stage, &c->vf_stage, &c->mf_stage,
-                    &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex, &c->ozeb, &c->s_apwalm, &c->s_igpig,
+                    &c->m2_stage, &c->bt, &c->a2t, &c->at, &c->ivgk, &c->mtab, &c->bmtab, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex, &c->ozeb, &c->s_apwalm, &c->s_igpig,
                     &c->s_vgpc, &c->s_h, &c->s_o, &c->s_gpc, &c->s_tpgpc, &c->s_sfac,
                     &c->bbt, &c->bat, &c->ba2t, &c->bivgk, &c->ifz, &c->igeo, &c->i_in0,
                     &c->i_in1, &c->i_out0, &c->i_out1, &c->i_out2, &c->apwfr_d, &c->hml_fl,
@@ -607,8 +607,13 @@ int exh_match_device(exh_context *c, void *stream, int ngp, const double *gpc,
   if (ngp > 0 && (!gpc || !tpgpc || !sfacgp || !apwalm))
     return fail(c, EXH_EINVAL, "match: null array");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  if (ngp > 0) {
+    const int rc = ensure(c, c->mtab, match_table_bytes(c));
+    if (rc) return rc;
+  }
   hipError_t e = timed(c, EXH_K_MATCH, s, [&] {
-    return launch_match(s, c, ngp, gpc, tpgpc, (const double2 *)sfacgp, (double2 *)apwalm);
+    return launch_match(s, c, ngp, gpc, tpgpc, (const double2 *)sfacgp, (double2 *)apwalm,
+                        (double *)c->mtab.p);
   });
   if (e == hipSuccess) e = mark_done(c, s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "match kernel");
@@ -857,6 +862,7 @@ int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
   if (!rc) rc = ensure(c, c->s_tpgpc, 2 * ld * sizeof(double));
   if (!rc) rc = ensure(c, c->s_sfac, ld * c->natmtot * sizeof(double2));
   if (!rc) rc = ensure(c, c->s_apwalm, ld * ncol * sizeof(double2));
+  if (!rc) rc = ensure(c, c->mtab, match_table_bytes(c));
   if (rc) return rc;
   hipStream_t s = c->stream;
   c->apw_host = nullptr;  // s_apwalm is rewritten below
@@ -873,7 +879,8 @@ int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
   if (e != hipSuccess) return hip_fail(c, e, "match: upload");
   e = timed(c, EXH_K_MATCH, s, [&] {
     return launch_match(s, c, ngp, (const double *)c->s_gpc.p, (const double *)c->s_tpgpc.p,
-                        (const double2 *)c->s_sfac.p, (double2 *)c->s_apwalm.p);
+                        (const double2 *)c->s_sfac.p, (double2 *)c->s_apwalm.p,
+                        (double *)c->mtab.p);
   });
   if (e == hipSuccess)
     e = hipMemcpy2DAsync(apwalm, ld * sizeof(double2), c->s_apwalm.p, ld * sizeof(double2),

@@ -100,7 +100,7 @@ uint64_t state_signature(const exh_context *c) {
                           &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu,
                           &c->mt_match.piv, &c->aa, &c->loa, &c->lolo, &c->salo,
                           &c->ololo_eff, &c->h1aa, &c->ivg, &c->ist, &c->bbt, &c->bat,
-                          &c->ba2t, &c->bivgk, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex,
+                          &c->ba2t, &c->bivgk, &c->bmtab, &c->ozqx, &c->ozqb, &c->ozr, &c->ozex,
                           &c->ozeb};
   for (const DevBuf *b : bufs) mix(&b->p, sizeof b->p);
   const int sc[] = {c->nspecies, c->natmtot, c->lmaxapw, c->lmaxmat, c->apwordmax, c->nlomax,
@@ -153,10 +153,12 @@ int validate(exh_context *lead, int nk, const exh_kpoint *kp, exh_context *const
 int ldb_of(const exh_context *c) { return (c->ngkmax + 63) / 64 * 64; }
 
 // Per-k-point scratch bytes of a context.
+// Per-k-point scratch bytes of a context: the K-major operands, the G+k box
+// indices and the match kernel's radial table.
 size_t perk_bytes(const exh_context *c) {
   const size_t op = (size_t)std::max(c->Ktot, 1) * ldb_of(c) * sizeof(double2);
   return op * (1 + (c->contig ? 0 : 1) + (c->have_h1aa ? 1 : 0)) +
-         (size_t)std::max(c->ngkmax, 1) * sizeof(int4);
+         (size_t)std::max(c->ngkmax, 1) * sizeof(int4) + match_table_bytes(c);
 }
 
 int ensure_scratch(exh_context *c, int nkg) {
@@ -165,6 +167,7 @@ int ensure_scratch(exh_context *c, int nkg) {
   if (!rc && !c->contig) rc = ensure(c, c->bat, op * nkg);
   if (!rc && c->have_h1aa) rc = ensure(c, c->ba2t, op * nkg);
   if (!rc) rc = ensure(c, c->bivgk, (size_t)std::max(c->ngkmax, 1) * sizeof(int4) * nkg);
+  if (!rc) rc = ensure(c, c->bmtab, match_table_bytes(c) * nkg);
   return rc;
 }
 
@@ -172,7 +175,8 @@ bool scratch_fits(const exh_context *c, int nkg) {
   const size_t op = (size_t)std::max(c->Ktot, 1) * ldb_of(c) * sizeof(double2);
   return c->bbt.p && c->bbt.bytes >= op * nkg && (c->contig || c->bat.bytes >= op * nkg) &&
          (!c->have_h1aa || c->ba2t.bytes >= op * nkg) &&
-         c->bivgk.bytes >= (size_t)std::max(c->ngkmax, 1) * sizeof(int4) * nkg;
+         c->bivgk.bytes >= (size_t)std::max(c->ngkmax, 1) * sizeof(int4) * nkg &&
+         c->bmtab.bytes >= match_table_bytes(c) * nkg;
 }
 
 exh_context::BatchEntry *find_entry(exh_context *c, const std::vector<unsigned char> &key) {
@@ -288,11 +292,13 @@ exh_context::BatchEntry *build_entry(exh_context *lead, hipStream_t s, int nk,
     d.at = c->contig ? nullptr : (double2 *)c->bat.p + slot * op;
     d.a2t = c->have_h1aa ? (double2 *)c->ba2t.p + slot * op : nullptr;
     d.ivgk = (int4 *)c->bivgk.p + (size_t)slot * std::max(c->ngkmax, 1);
+    d.mtab = (double *)((char *)c->bmtab.p + (size_t)slot * match_table_bytes(c));
     d.x = c->contig ? (const double2 *)d.apwalm : d.at;
     d.sd = sd_dev + j;
     GroupDims &D = g.dims;
     D.ngpmax = std::max(D.ngpmax, k.ngp);
     D.natmax = std::max(D.natmax, c->natmtot);
+    D.nspmax = std::max(D.nspmax, c->nspecies);
     D.maxaa = std::max(D.maxaa, c->maxaa);
     D.Kmax = std::max(D.Kmax, c->Ktot);
     D.nlotot = std::max(D.nlotot, c->nlotot);

@@ -23,6 +23,16 @@ __device__ __forceinline__ EXH_G T *gptr(T *p) {
   return (EXH_G T *)p;
 }
 
+// Read-only tables whose addresses come from a descriptor: in the constant
+// address space a load at a wave-uniform address becomes a scalar load (as
+// for kernel arguments), instead of a vector load of the same value in
+// every lane.
+#define EXH_C __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const EXH_C T *cptr(const T *p) {
+  return (const EXH_C T *)p;
+}
+
 // Loads and stores of double2 / int4 through either kind of pointer (HIP's
 // vector classes cannot be assigned in an explicit address space, so the
 // EXH_G forms go through clang's native vector types).
@@ -33,6 +43,12 @@ __device__ __forceinline__ double2 ld(const EXH_G double2 *p) {
   const dv2 v = *(const EXH_G dv2 *)p;
   return z2(v.x, v.y);
 }
+__device__ __forceinline__ double2 ld(const EXH_C double2 *p) {
+  const dv2 v = *(const EXH_C dv2 *)p;
+  return z2(v.x, v.y);
+}
+__device__ __forceinline__ double ld(const double *p) { return *p; }
+__device__ __forceinline__ double ld(const EXH_G double *p) { return *p; }
 __device__ __forceinline__ int4 ld(const int4 *p) { return *p; }
 __device__ __forceinline__ int4 ld(const EXH_G int4 *p) {
   const iv4 v = *(const EXH_G iv4 *)p;

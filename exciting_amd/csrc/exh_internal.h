@@ -62,7 +62,7 @@ struct LoTables {
 // kernels read, so that one launch can cover k-points of several structures.
 struct SDesc {
   // basis and the stacked K index (atom, l, m, io)
-  int natmtot, Ktot, maxaa, lmaxapw, pmax, lmmax, ngkmax, have_h1aa;
+  int natmtot, nspecies, Ktot, maxaa, lmaxapw, pmax, lmmax, ngkmax, have_h1aa;
   long long ldx;  // leading dimension of the K-major A operand
   int ldb;        // leading dimension of the batch scratch operands
   const long long *colA;
@@ -102,6 +102,7 @@ struct KDesc {
   const double2 *x;  // K-major A: apwalm itself (equally spaced columns) or at
   double2 *at, *bt, *a2t;
   int4 *ivgk;
+  double *mtab;  // match: radial and harmonics tables of this k-point (batch scratch)
   const SDesc *sd;
 };
 
@@ -109,7 +110,7 @@ struct KDesc {
 // k-points and structures, which set the launch grids (blocks beyond a
 // k-point's own sizes exit).
 struct GroupDims {
-  int nk = 0, ngpmax = 0, natmax = 0, maxaa = 0, Kmax = 0, nlotot = 0, maxnlo = 0;
+  int nk = 0, ngpmax = 0, natmax = 0, nspmax = 0, maxaa = 0, Kmax = 0, nlotot = 0, maxnlo = 0;
   unsigned lmax_mask = 0;  // bit l: some structure has lmaxapw = l
   bool any_pack = false, any_a2t = false;
 };
@@ -208,13 +209,13 @@ struct exh_context {
   // wait for it before rewriting tables that work may still read
   hipEvent_t done_ev = nullptr;
   bool done_pending = false;
-  exh::DevBuf bbt, bat, ba2t, bivgk;
+  exh::DevBuf bbt, bat, ba2t, bivgk, bmtab;
   struct BatchEntry;
   std::vector<BatchEntry *> bcache;
   uint64_t bclock = 0;
 
-  // per-call scratch
-  exh::DevBuf bt, a2t, at, ivgk;
+  // per-call scratch (mtab: the match kernel's radial and harmonics tables)
+  exh::DevBuf bt, a2t, at, ivgk, mtab;
   // this device's copy of the static scratch of exh_setup_device_noalloc
   void *noalloc_scratch = nullptr;
   exh::DevBuf ozqx, ozqb, ozr, ozex, ozeb;
@@ -298,9 +299,12 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
                         const OzLayout &L, int8_t *QX, int8_t *QB, uint8_t *R, int *ex,
                         int *eb, const IstBox &box, double2 *h, double2 *o, int ldh,
                         int accumulate, int stage_mask);
+// tab: match_table_bytes(c) of scratch for the k-point's radial factors and
+// spherical harmonics
+size_t match_table_bytes(const exh_context *c);
 hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
                         const double *gpc, const double *tpgpc,
-                        const double2 *sfacgp, double2 *apwalm);
+                        const double2 *sfacgp, double2 *apwalm, double *tab);
 
 // Grouped launchers: one launch covers the g.nk k-points described by kd
 // (device), of one or several structures.

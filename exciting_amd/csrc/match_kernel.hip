@@ -1,15 +1,19 @@
 // APW matching coefficients on the GPU (match.f90:13-194).
 //
-// One thread per (G+k vector, atom).  Each thread evaluates the spherical
-// Bessel functions and their derivatives at |G+k| R (sbessel.f90,
-// sbesseldm.f90), the complex spherical harmonics of (G+k)^ by the
-// Masters & Richards-Dinger recursion (genylm.f90), forms
+// The reference solves, per (l, atom), D A = b with
 //   b_io = 4 pi i^l / sqrt(Omega) |G+k|^(io-1) j_l^(io-1)(|G+k|R) S_a(G+k) Y*_lm
-// and solves D A = b with the LU factors of the boundary-derivative matrix D
-// (zgetrf/zgetrs order; factors computed once per (l, atom) on the host).
-// Writes apwalm(G, io, lm, atom): consecutive threads write consecutive G,
-// so every store is a coalesced 1 KB row segment.  The kernel is bound by
-// those writes (16 B x G x p x (lmax+1)^2 x atoms).
+// (spherical Bessel functions and derivatives from sbessel.f90 and
+// sbesseldm.f90, complex spherical harmonics by the Masters &
+// Richards-Dinger recursion of genylm.f90, zgesv on D).  b is a real radial
+// vector times one complex factor, so the solve runs on the radial vector
+// (zgetrs order with the LU factors of D, computed once per (l, atom) on the
+// host) and the factor multiplies the solution.  The harmonics depend on
+// G+k only and the radial factors on (G+k, species): a workgroup computes
+// them once for its G+k vectors into LDS and reuses them for its atoms.
+// The kernel is bound by the writes of apwalm(G, io, lm, atom)
+// (16 B x G x p x (lmax+1)^2 x atoms), all coalesced along G.
+#include <algorithm>
+
 #include "exh_device.h"
 
 namespace exh {
@@ -177,28 +181,115 @@ __device__ __forceinline__ void d_sbesseldm(int m, double x,
   }
 }
 
-template <int LM, typename PD, typename PS, typename PA>
-__device__ __forceinline__ void match_body(
-    int g, int ias, int ngp, int ld, int pmax, PD __restrict__ gpc, PD __restrict__ tpgpc,
-    PS __restrict__ sfacgp, PA __restrict__ apwalm, const int *__restrict__ atom_species,
-    const double *__restrict__ rmt, const int *__restrict__ apword,
-    const int *__restrict__ omaxs, const double *__restrict__ lu,
-    const int *__restrict__ piv, double fpso) {
-  if (g >= ngp) return;
-  const int is = atom_species[ias];
-  const int omax = omaxs[is];
-  constexpr int LMM = (LM + 1) * (LM + 1);
-  const double q = gpc[g];
-  const double x = q * rmt[is];
+// ---------------------------------------------------------------------------
+// One workgroup: MG consecutive G+k vectors x MA consecutive atoms, after
+// match_prep has tabulated, per G+k, conj(Y_lm) * 4 pi i^l / sqrt(Omega)
+// and, per (G+k, species), |G+k|^io j_l^(io)(|G+k| R).
+//   Stage 2 (per species run of the tile's atoms): the radial rows into LDS.
+//   Stage 3: the boundary solve D v = (j_l^(io)) per (G+k, l, atom) with the
+//     LU factors of (l, atom) (zgetrs order on the real radial vector; the
+//     complex factor S_a(G+k) conj(Y_lm) fpso i^l multiplies the solution),
+//     then apwalm(G, io, lm, a) = v_io * S_a * yt_lm.  Lane half h writes
+//     io = h, h + 2, ... of MG consecutive G: 512-byte row segments.
+// The write loop holds few registers and little LDS, so many waves keep
+// stores in flight.
+// ---------------------------------------------------------------------------
+constexpr int MG = 32;    // G+k vectors per workgroup
+constexpr int MA = 8;     // atoms per workgroup
+constexpr int MTH = 256;  // threads: MG x 8 slots
 
-  // radial factors djl[io][l] = |G+k|^io j_l^(io)(x)      (match.f90:114-131)
-  double djl[PCAP][LM + 1];
+template <int LM>
+struct MatchLds {
+  double dj[PCAP][LM + 1][MG];  // |G+k|^io j_l^(io)(|G+k| R) of one species
+};
+
+// conj(Y_lm) * fpso * i^l for m = -L..L of one G+k direction (genylm.f90:56-90),
+// into out[(L + m) * stride] (out points at lm = L*L).
+template <int L, typename PO>
+__device__ __forceinline__ void ylm_row(double sn, double cs, double phi, double fpso, PO out,
+                                        size_t stride) {
+  double xm[L + 1];
+  double tn = 0.0;
+  if constexpr (L == 0) {
+    xm[0] = 1.0;
+  } else {
+    xm[L] = (L % 2 == 0) ? 1.0 : -1.0;
+    double dx = 0.0;
+#pragma unroll
+    for (int m = L; m >= 1; m--) {
+      const double t1 = sqrt((double)((L + m) * (L - m + 1)));
+      xm[m - 1] = -(sn * dx + (double)(2 * m) * cs * xm[m]) / t1;
+      dx = sn * xm[m] * t1;
+    }
+    double t1 = sn, sum = 0.0;
+#pragma unroll
+    for (int m = 1; m <= L; m++) {
+      xm[m] = t1 * xm[m];
+      sum = sum + xm[m] * xm[m];
+      t1 = t1 * sn;
+    }
+    sum = 2.0 * sum + xm[0] * xm[0];
+    tn = sqrt((double)(2 * L + 1) / (12.566370614359172954 * sum));
+  }
+  // zt1 = fpso * i^l (match.f90:139)
+  double z1r = 0.0, z1i = 0.0;
+  switch (L & 3) {
+    case 0: z1r = fpso; break;
+    case 1: z1i = fpso; break;
+    case 2: z1r = -fpso; break;
+    default: z1i = -fpso; break;
+  }
+#pragma unroll
+  for (int m = -L; m <= L; m++) {
+    double yr, yi;
+    if (L == 0) {
+      yr = 0.28209479177387814347;
+      yi = 0.0;
+    } else if (m == 0) {
+      yr = tn * xm[0];
+      yi = 0.0;
+    } else {
+      const int am = m < 0 ? -m : m;
+      const double t1 = (double)am * phi;
+      const double s = tn * xm[am];
+      yr = s * cos(t1);
+      yi = s * sin(t1);
+      if (m < 0) {
+        yi = -yi;
+        if (am & 1) {
+          yr = -yr;
+          yi = -yi;
+        }
+      }
+    }
+    const double ycr = yr, yci = -yi;  // conj
+    st(out + (size_t)(L + m) * stride, z2(fma(z1r, ycr, -(z1i * yci)), fma(z1r, yci, z1i * ycr)));
+  }
+}
+
+template <int LM, int L = 0, typename PO>
+__device__ __forceinline__ void ylm_all(double sn, double cs, double phi, double fpso, PO yt,
+                                        size_t stride) {
+  if constexpr (L <= LM) {
+    ylm_row<L>(sn, cs, phi, fpso, yt + (size_t)L * L * stride, stride);
+    ylm_all<LM, L + 1>(sn, cs, phi, fpso, yt, stride);
+  }
+}
+
+// |G+k|^io j_l^(io)(|G+k| R), io < omax (match.f90:114-131)
+template <int LM>
+__device__ __forceinline__ void radial_row(double q, double rmt, int omax, double (&djl)[PCAP][LM + 1]) {
+  const double x = q * rmt;
   {
     double jl[LM + 1];
     d_sbessel<LM>(x, jl);
 #pragma unroll
     for (int l = 0; l <= LM; l++) djl[0][l] = jl[l];
   }
+#pragma unroll
+  for (int io = 1; io < PCAP; io++)
+#pragma unroll
+    for (int l = 0; l <= LM; l++) djl[io][l] = 0.0;
   if (omax > 1) {
     double jp1[LM + 2];
 #pragma unroll
@@ -216,162 +307,202 @@ __device__ __forceinline__ void match_body(
       }
     }
   }
+}
 
-  // spherical harmonics ingredients (genylm.f90:56-62)
-  const double sn = sin(tpgpc[2 * g]), cs = cos(tpgpc[2 * g]);
-  double zr[LM + 1], zi[LM + 1];
-#pragma unroll
-  for (int m = 1; m <= LM; m++) {
-    const double t1 = (double)m * tpgpc[2 * g + 1];
-    zr[m] = cos(t1);
-    zi[m] = sin(t1);
-  }
-  const double2 S = exh::ld(sfacgp + g + (size_t)ld * ias);
-  const PA out = apwalm + g + (size_t)ld * pmax * LMM * ias;
+// Tables of one k-point, in its match scratch (ld = ngkmax):
+//   rad[((is * PCAP + io) * (LM + 1) + l) * ld + g] = |G+k|^io j_l^(io)(|G+k| R_is),
+//     zero for io >= omax(is);
+//   yt[lm * ld + g] = conj(Y_lm(G+k)) * 4 pi i^l / sqrt(Omega).
+// One thread per (G+k, role): role < nspecies the radial factors of that
+// species, role == nspecies the spherical harmonics.
+__host__ __device__ inline size_t match_rad_doubles(int nspecies, int lmaxapw, int ld) {
+  // rounded so that the harmonics that follow are 256-byte aligned
+  return ((size_t)nspecies * PCAP * (lmaxapw + 1) * ld + 31) / 32 * 32;
+}
 
+template <int LM, typename PD, typename PR, typename PY>
+__device__ __forceinline__ void prep_body(int g, int role, int ngp, int nspecies, int ld,
+                                          PD __restrict__ gpc, PD __restrict__ tpgpc,
+                                          const double *__restrict__ rmt,
+                                          const int *__restrict__ omaxs, double fpso,
+                                          PR __restrict__ rad, PY __restrict__ yt) {
+  if (g >= ngp) return;
+  if (role < nspecies) {
+    double djl[PCAP][LM + 1];
+    radial_row<LM>(gpc[g], rmt[role], omaxs[role], djl);
 #pragma unroll
-  for (int l = 0; l <= LM; l++) {
-    // x_m (genylm.f90:63-81) for this l
-    double xm[LM + 1];
-    double tn = 0.0;
-    if (l == 0) {
-      xm[0] = 1.0;
-    } else {
-      xm[l] = (l % 2 == 0) ? 1.0 : -1.0;
-      double dx = 0.0;
+    for (int io = 0; io < PCAP; io++)
 #pragma unroll
-      for (int m = l; m >= 1; m--) {
-        const double t1 = sqrt((double)((l + m) * (l - m + 1)));
-        xm[m - 1] = -(sn * dx + (double)(2 * m) * cs * xm[m]) / t1;
-        dx = sn * xm[m] * t1;
-      }
-      double t1 = sn, sum = 0.0;
-#pragma unroll
-      for (int m = 1; m <= l; m++) {
-        xm[m] = t1 * xm[m];
-        sum = sum + xm[m] * xm[m];
-        t1 = t1 * sn;
-      }
-      sum = 2.0 * sum + xm[0] * xm[0];
-      tn = sqrt((double)(2 * l + 1) / (12.566370614359172954 * sum));
-    }
-    // zt1 = fpso * i^l ; zt2 = zt1 * S       (match.f90:139,152)
-    double z1r = 0.0, z1i = 0.0;
-    switch (l & 3) {
-      case 0: z1r = fpso; break;
-      case 1: z1i = fpso; break;
-      case 2: z1r = -fpso; break;
-      default: z1i = -fpso; break;
-    }
-    const double z2r = z1r * S.x - z1i * S.y, z2i = z1r * S.y + z1i * S.x;
-    const int p = apword[l + (LM + 1) * is];
-    const double *LU = lu + (size_t)PCAP * PCAP * (l + (LM + 1) * ias);
-    const int *PV = piv + PCAP * (l + (LM + 1) * ias);
-#pragma unroll
-    for (int m = -l; m <= l; m++) {
-      // Y_lm (genylm.f90:82-90), then conj
-      double yr, yi;
-      if (l == 0) {
-        yr = 0.28209479177387814347;
-        yi = 0.0;
-      } else if (m == 0) {
-        yr = tn * xm[0];
-        yi = 0.0;
-      } else {
-        const int am = m < 0 ? -m : m;
-        const double s = tn * xm[am];
-        yr = s * zr[am];
-        yi = s * zi[am];
-        if (m < 0) {
-          yi = -yi;
-          if (am & 1) {
-            yr = -yr;
-            yi = -yi;
-          }
-        }
-      }
-      const double ycr = yr, yci = -yi;
-      double br[PCAP], bi[PCAP];
-#pragma unroll
-      for (int io = 0; io < PCAP; io++) {
-        if (io < p) {
-          const double tr = djl[io][l] * z2r, ti = djl[io][l] * z2i;
-          br[io] = tr * ycr - ti * yci;
-          bi[io] = tr * yci + ti * ycr;
-        }
-      }
-      // zgetrs: row interchanges, unit-lower then upper solve
-#pragma unroll
-      for (int i = 0; i < PCAP; i++)
-        if (i < p) {
-          const int ip = PV[i] - 1;
-          if (ip != i) {
-            double t = br[i];
-            br[i] = br[ip];
-            br[ip] = t;
-            t = bi[i];
-            bi[i] = bi[ip];
-            bi[ip] = t;
-          }
-        }
-#pragma unroll
-      for (int k = 0; k < PCAP; k++)
-#pragma unroll
-        for (int i = k + 1; i < PCAP; i++)
-          if (i < p) {
-            const double lv = LU[i + PCAP * k];
-            br[i] -= br[k] * lv;
-            bi[i] -= bi[k] * lv;
-          }
-#pragma unroll
-      for (int k = PCAP - 1; k >= 0; k--)
-        if (k < p) {
-          const double u = LU[k + PCAP * k];
-          br[k] = br[k] / u;
-          bi[k] = bi[k] / u;
-#pragma unroll
-          for (int i = 0; i < k; i++) {
-            const double uv = LU[i + PCAP * k];
-            br[i] -= br[k] * uv;
-            bi[i] -= bi[k] * uv;
-          }
-        }
-      const int lm = l * (l + 1) + m;
-#pragma unroll
-      for (int io = 0; io < PCAP; io++)
-        if (io < p) {
-          double2 v;
-          v.x = br[io];
-          v.y = bi[io];
-          st(out + (size_t)ld * (io + (size_t)pmax * lm), v);
-        }
-    }
+      for (int l = 0; l <= LM; l++)
+        rad[((size_t)(role * PCAP + io) * (LM + 1) + l) * ld + g] = djl[io][l];
+  } else {
+    const double th = tpgpc[2 * g], phi = tpgpc[2 * g + 1];
+    ylm_all<LM>(sin(th), cos(th), phi, fpso, yt + g, (size_t)ld);
   }
 }
 
 template <int LM>
-__global__ __launch_bounds__(64) void match_kernel(
-    int ngp, int ld, int pmax, const double *__restrict__ gpc,
-    const double *__restrict__ tpgpc, const double2 *__restrict__ sfacgp,
+__global__ __launch_bounds__(64) void match_prep_kernel(int ngp, int nspecies, int ld,
+                                                        const double *__restrict__ gpc,
+                                                        const double *__restrict__ tpgpc,
+                                                        const double *__restrict__ rmt,
+                                                        const int *__restrict__ omaxs,
+                                                        double fpso, double *__restrict__ tab) {
+  prep_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, ngp, nspecies, ld, gpc, tpgpc, rmt,
+                omaxs, fpso, tab,
+                (double2 *)(tab + match_rad_doubles(nspecies, LM, ld)));
+}
+
+template <int LM>
+__global__ __launch_bounds__(64) void match_prep_grouped_kernel(const KDesc *__restrict__ kd) {
+  const KDesc &d = kd[blockIdx.z];
+  const SDesc &S = *d.sd;
+  if (S.lmaxapw != LM || (int)blockIdx.y > S.nspecies) return;
+  double *tab = d.mtab;
+  prep_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, S.nspecies, S.ngkmax,
+                gptr(d.gpc), gptr(d.tpgpc), S.rmt, S.omax, S.fpso, gptr(tab),
+                gptr((double2 *)(tab + match_rad_doubles(S.nspecies, LM, S.ngkmax))));
+}
+
+template <int LM, typename PS, typename PA, typename PR, typename PY>
+__device__ __forceinline__ void match_tile(
+    int g0, int a0, int ngp, int natmtot, int ld, int pmax, PS __restrict__ sfacgp,
+    PA __restrict__ apwalm, const int *__restrict__ atom_species, PR __restrict__ rad,
+    PY __restrict__ yt, const int *__restrict__ apword, const double *__restrict__ lu,
+    const int *__restrict__ piv, MatchLds<LM> &sm) {
+  constexpr int LMM = (LM + 1) * (LM + 1);
+  const int a1 = min(natmtot, a0 + MA);
+  if (g0 >= ngp || a0 >= a1) return;  // uniform per workgroup
+  // stage 3 layout: wave wv writes lm = wv, wv + 4, ... (uniform, so the
+  // LU factors are scalar loads); lane half h writes io = h, h + 2, ...
+  const int t = threadIdx.x, gl = t % MG, h = (t >> 5) & 1;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int g = g0 + gl;
+  const bool gv = g < ngp;
+  const EXH_C double *LUt = cptr(lu);
+  const EXH_C int *PVt = cptr(piv);
+  const EXH_C int *APt = cptr(apword);
+
+  int ia = a0;
+  while (ia < a1) {
+    const int is = atom_species[ia];
+    int ib = ia + 1;
+    while (ib < a1 && atom_species[ib] == is) ib++;
+    __syncthreads();  // stage 1 written / previous species' stage 3 done
+    // stage 2: the radial rows of species is, from the k-point's radial table
+    for (int it = t; it < PCAP * (LM + 1) * MG; it += MTH) {
+      const int ig = it % MG, row = it / MG;
+      const int gg = min(g0 + ig, ngp - 1);
+      (&sm.dj[0][0][0])[row * MG + ig] = exh::ld(rad + ((size_t)is * PCAP * (LM + 1) + row) * ld + gg);
+    }
+    __syncthreads();
+    // stage 3: wave wv takes lm = wv, wv + 4, ...; for each it reads the
+    // harmonics row segment once (the table stays in cache) and writes it
+    // for every atom of the run, whose boundary solutions (per G, l and
+    // atom) are redone only when l changes
+    const int na = ib - ia;  // <= MA, uniform
+    double2 S[MA];
+#pragma unroll
+    for (int u = 0; u < MA; u++)
+      S[u] = (u < na && gv) ? exh::ld(sfacgp + g + (size_t)ld * (ia + u)) : z2(0.0, 0.0);
+    int lcur = -1, p = 0;
+    double v[MA][PCAP];
+    for (int lm = wv; lm < LMM; lm += MTH / 64) {
+      int l = (int)sqrt((double)lm);
+      l -= (l * l > lm) ? 1 : 0;
+      l += ((l + 1) * (l + 1) <= lm) ? 1 : 0;
+      if (l != lcur) {
+        lcur = l;
+        p = APt[l + (LM + 1) * is];
+        double d0[PCAP];
+#pragma unroll
+        for (int io = 0; io < PCAP; io++) d0[io] = io < p ? sm.dj[io][l][gl] : 0.0;
+#pragma unroll
+        for (int u = 0; u < MA; u++) {
+          if (u >= na) break;
+          const EXH_C double *LU = LUt + (size_t)PCAP * PCAP * (l + (LM + 1) * (ia + u));
+          const EXH_C int *PV = PVt + PCAP * (l + (LM + 1) * (ia + u));
+          double x[PCAP];
+#pragma unroll
+          for (int io = 0; io < PCAP; io++) x[io] = d0[io];
+          // zgetrs: row interchanges, unit-lower then upper solve
+#pragma unroll
+          for (int i = 0; i < PCAP; i++)
+            if (i < p) {
+              const int ip = PV[i] - 1;
+              if (ip != i) {
+                double tt = x[i];
+#pragma unroll
+                for (int j = 0; j < PCAP; j++)
+                  if (j == ip) {
+                    x[i] = x[j];
+                    x[j] = tt;
+                  }
+              }
+            }
+#pragma unroll
+          for (int k = 0; k < PCAP; k++)
+#pragma unroll
+            for (int i = k + 1; i < PCAP; i++)
+              if (i < p) x[i] = fma(-x[k], LU[i + PCAP * k], x[i]);
+#pragma unroll
+          for (int k = PCAP - 1; k >= 0; k--)
+            if (k < p) {
+              x[k] = x[k] / LU[k + PCAP * k];
+#pragma unroll
+              for (int i = 0; i < k; i++) x[i] = fma(-x[k], LU[i + PCAP * k], x[i]);
+            }
+#pragma unroll
+          for (int io = 0; io < PCAP; io++) v[u][io] = x[io];
+        }
+      }
+      if (!gv) continue;
+      const double2 y = exh::ld(yt + (size_t)lm * ld + g);
+#pragma unroll
+      for (int u = 0; u < MA; u++) {
+        if (u >= na) break;
+        // explicit fma: the same rounding in every instantiation of the tile
+        // (the compiler would otherwise fuse either product)
+        const double wr = fma(S[u].x, y.x, -(S[u].y * y.y));
+        const double wi = fma(S[u].x, y.y, S[u].y * y.x);
+        const PA out = apwalm + g + (size_t)ld * pmax * LMM * (ia + u);
+#pragma unroll
+        for (int io = 0; io < PCAP; io++)
+          if (io < p && (io & 1) == h)
+            st(out + (size_t)ld * (io + (size_t)pmax * lm), z2(v[u][io] * wr, v[u][io] * wi));
+      }
+    }
+    ia = ib;
+  }
+}
+
+template <int LM>
+__global__ __launch_bounds__(MTH) void match_kernel(
+    int ngp, int natmtot, int nspecies, int ld, int pmax, const double2 *__restrict__ sfacgp,
     double2 *__restrict__ apwalm, const int *__restrict__ atom_species,
-    const double *__restrict__ rmt, const int *__restrict__ apword,
-    const int *__restrict__ omaxs, const double *__restrict__ lu,
-    const int *__restrict__ piv, double fpso) {
-  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, ngp, ld, pmax, gpc, tpgpc, sfacgp,
-                 apwalm, atom_species, rmt, apword, omaxs, lu, piv, fpso);
+    const double *__restrict__ tab, const int *__restrict__ apword,
+    const double *__restrict__ lu, const int *__restrict__ piv) {
+  __shared__ MatchLds<LM> sm;
+  match_tile<LM>(blockIdx.x * MG, blockIdx.y * MA, ngp, natmtot, ld, pmax, sfacgp, apwalm,
+                 atom_species, tab,
+                 (const double2 *)(tab + match_rad_doubles(nspecies, LM, ld)), apword, lu, piv,
+                 sm);
 }
 
 // Grouped over k-points: blockIdx.z selects the descriptor, blockIdx.y the
-// atom.  One launch per lmaxapw present in the group; k-points of structures
-// with another lmaxapw exit at once.
+// atom chunk.  One launch per lmaxapw present in the group; k-points of
+// structures with another lmaxapw exit at once.
 template <int LM>
-__global__ __launch_bounds__(64) void match_grouped_kernel(const KDesc *__restrict__ kd) {
+__global__ __launch_bounds__(MTH) void match_grouped_kernel(const KDesc *__restrict__ kd) {
+  __shared__ MatchLds<LM> sm;
   const KDesc &d = kd[blockIdx.z];
-  const SDesc &S = *d.sd;
-  if (S.lmaxapw != LM || (int)blockIdx.y >= S.natmtot) return;
-  match_body<LM>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp, S.ngkmax, S.pmax,
-                 gptr(d.gpc), gptr(d.tpgpc), gptr(d.sfac), gptr(d.apwalm), S.atom_species,
-                 S.rmt, S.apword, S.omax, S.lu, S.piv, S.fpso);
+  const EXH_C SDesc &S = *cptr(d.sd);
+  if (S.lmaxapw != LM) return;
+  const double *tab = d.mtab;
+  match_tile<LM>(blockIdx.x * MG, blockIdx.y * MA, d.ngp, S.natmtot, S.ngkmax, S.pmax,
+                 gptr(d.sfac), gptr(d.apwalm), S.atom_species, gptr(tab),
+                 gptr((const double2 *)(tab + match_rad_doubles(S.nspecies, LM, S.ngkmax))),
+                 S.apword, S.lu, S.piv, sm);
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
@@ -381,22 +512,32 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 // Largest lmaxapw with a compiled match kernel.
 constexpr int kMatchLmaxCap = 12;
 
+size_t match_table_bytes(const exh_context *c) {
+  const int ld = std::max(c->ngkmax, 1);
+  return sizeof(double) * match_rad_doubles(std::max(c->nspecies, 1), c->lmaxapw, ld) +
+         sizeof(double2) * (size_t)(c->lmaxapw + 1) * (c->lmaxapw + 1) * ld;
+}
+
 hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
                         const double *gpc, const double *tpgpc,
-                        const double2 *sfacgp, double2 *apwalm) {
+                        const double2 *sfacgp, double2 *apwalm, double *tab) {
   if (ngp <= 0 || c->natmtot <= 0) return hipSuccess;
   const double fpso = 12.566370614359172954 / sqrt(c->omega);
-  dim3 grid(cdiv(ngp, 64), c->natmtot);
+  dim3 gridp(cdiv(ngp, 64), c->nspecies + 1);
+  dim3 grid(cdiv(ngp, MG), cdiv(c->natmtot, MA));
   const int *as = (const int *)c->atom_species_d.p;
   const int *apw = (const int *)c->apword_d.p;
   const int *om = (const int *)c->omax_d.p;
   const double *lu = (const double *)c->mt_match.lu.p;
   const int *pv = (const int *)c->mt_match.piv.p;
-#define EXH_M(L)                                                                      \
-  case L:                                                                             \
-    hipLaunchKernelGGL(match_kernel<L>, grid, dim3(64), 0, s, ngp, c->ngkmax,         \
-                       c->apwordmax, gpc, tpgpc, sfacgp, apwalm, as,                  \
-                       (const double *)c->rmt_d.p, apw, om, lu, pv, fpso);            \
+  const double *rmt = (const double *)c->rmt_d.p;
+#define EXH_M(L)                                                                          \
+  case L:                                                                                 \
+    hipLaunchKernelGGL(match_prep_kernel<L>, gridp, dim3(64), 0, s, ngp, c->nspecies,     \
+                       c->ngkmax, gpc, tpgpc, rmt, om, fpso, tab);                        \
+    hipLaunchKernelGGL(match_kernel<L>, grid, dim3(MTH), 0, s, ngp, c->natmtot,           \
+                       c->nspecies, c->ngkmax, c->apwordmax, sfacgp, apwalm, as, tab, apw, \
+                       lu, pv);                                                           \
     break;
   switch (c->lmaxapw) {
     EXH_M(0) EXH_M(1) EXH_M(2) EXH_M(3) EXH_M(4) EXH_M(5) EXH_M(6)
@@ -410,11 +551,13 @@ hipError_t launch_match(hipStream_t s, const exh_context *c, int ngp,
 
 hipError_t launch_match_grouped(hipStream_t s, const GroupDims &g, const KDesc *kd) {
   if (g.nk <= 0 || g.ngpmax <= 0 || g.natmax <= 0) return hipSuccess;
-  dim3 grid(cdiv(g.ngpmax, 64), g.natmax, g.nk);
+  dim3 gridp(cdiv(g.ngpmax, 64), g.nspmax + 1, g.nk);
+  dim3 grid(cdiv(g.ngpmax, MG), cdiv(g.natmax, MA), g.nk);
   if (g.lmax_mask >> (kMatchLmaxCap + 1)) return hipErrorInvalidValue;
 #define EXH_M(L)                                                                  \
   if (g.lmax_mask & (1u << L)) {                                                  \
-    hipLaunchKernelGGL(match_grouped_kernel<L>, grid, dim3(64), 0, s, kd);        \
+    hipLaunchKernelGGL(match_prep_grouped_kernel<L>, gridp, dim3(64), 0, s, kd);  \
+    hipLaunchKernelGGL(match_grouped_kernel<L>, grid, dim3(MTH), 0, s, kd);       \
     hipError_t e = hipGetLastError();                                             \
     if (e != hipSuccess) return e;                                                \
   }

@@ -15,7 +15,8 @@
 //        H: P1 = xr.br, P2 = xi.bi, P3 = (xr+xi).(br-bi)
 //        O: Q1 = xr.xr, Q2 = xi.xi, Q3 = (xr+xi).(xr-xi)
 //     an upper-triangular int8 GEMM accumulates exactly in int32 and stores
-//     the result modulo p_m as one byte per element.
+//     the result modulo p_m as one byte per element (operands streamed
+//     global -> LDS by direct DMA loads).
 //  4. Re = P1+P2 and Im = P1-P2-P3 are formed modulo each p_m and turned
 //     back into exact integers by Garner's mixed-radix algorithm with
 //     symmetric digits (no cancellation for small results), evaluated in
@@ -134,10 +135,36 @@ __global__ void residue_kernel(int ngp, int Ktot, int Gpad, const double2 *__res
 // Operand lane layout: row/column = lane%32, 16 k bytes from k block
 // 2*ks + lane/32; result D[8*(r/4) + 4*(lane/32) + r%4][lane%32].
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void store_residues(const v16i (&acc)[2][2], int p, float rp, int wr,
-                                               int wc, int lh, int l32, uint8_t *out) {
+// One workgroup computes the product pair (H_q, O_q) of one modulus for a
+// panel of OZ_P = 3 vertically stacked 128x128 tiles (I = 3*IT .. 3*IT+2,
+// one column block J): a 384-row panel of the row operand against
+// 128-column panels of the H and O column operands, which the three tiles
+// share.  8 waves as 4 row groups x 2 column groups; each wave owns 96 rows
+// x 64 columns of both products (3 x 2 blocks of v_mfma_i32_32x32x32_i8
+// each, 192 accumulator registers), which reads 7 operand fragments from LDS
+// per 12 MFMAs.  Operands stream global -> LDS directly (global_load_lds,
+// 16 bytes per lane, one 64-row chunk of one k block per instruction), two
+// stages of 64 k double-buffered.  Panels walk J-major: the panels IT =
+// 0..J/3 of column block J are consecutive, so co-resident workgroups share
+// the column panels in L2.
+constexpr int OZ_P = 3;
+constexpr int OZ_R = OZ_P * OZ_T;  // rows of the row panel per workgroup
+
+#define EXH_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+#define EXH_GLB(p) ((const __attribute__((address_space(1))) void *)(p))
+
+// Residues mod p of one 32 x 64 strip (block row a of a wave) per product,
+// four consecutive rows packed per 32-bit store, into the 128x128 tile it
+// lies in when that tile is on or above the diagonal.
+__device__ __forceinline__ void store_residues(const v16i (&acc)[OZ_P][2], int p, float rp,
+                                               int rg, int wc, int lh, int l32, int IT, int J,
+                                               uint8_t *Rp) {
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < OZ_P; a++) {
+    const int r0 = rg * 32 * OZ_P + a * 32;
+    const int I = OZ_P * IT + (r0 >> 7);
+    if (I > J) continue;  // below the diagonal (uniform per wave)
+    uint8_t *out = Rp + ((long long)J * (J + 1) / 2 + I) * (OZ_T * OZ_T);
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
@@ -152,128 +179,124 @@ __device__ __forceinline__ void store_residues(const v16i (&acc)[2][2], int p, f
           r -= (r >= p) ? p : 0;
           packed |= (unsigned)r << (8 * t4);
         }
-        const int il = wr * 64 + a * 32 + 8 * h + 4 * lh;
+        const int il = (r0 & 127) + 8 * h + 4 * lh;
         const int jl = wc * 64 + b * 32 + l32;
         *(unsigned *)(out + il + OZ_T * jl) = packed;
       }
+  }
 }
 
-// One workgroup computes the product pair (H_q, O_q) of one modulus for two
-// vertically stacked 128x128 tiles (I = 2*IP and 2*IP+1, one column block
-// J): a 256-row panel of the row operand against 128-column panels of the H
-// and O column operands, which the two halves share.  8 waves in two halves
-// of 2x2 waves, each wave 64x64 per product.  Tiles walk J-major: the pair
-// rows IP = 0..J/2 of column block J are consecutive, so co-resident
-// workgroups share the column panels in L2.
-constexpr int OZ_R = 2 * OZ_T;  // rows of the row panel per workgroup
+// Panels per GEMM: column block J has J/3 + 1 of them; blocks 3c .. 3c+2
+// together hold 3(c+1), starting at 3c(c+1)/2.
+__host__ __device__ inline long long oz_panels(int nt) {
+  const long long C = nt / OZ_P;
+  return OZ_P * C * (C + 1) / 2 + (long long)(nt - OZ_P * C) * (C + 1);
+}
 
 __global__ __launch_bounds__(512) void ozgemm_kernel(int nt, int nslab, int Gpad, int nmod,
                                                      const int8_t *__restrict__ QX,
                                                      const int8_t *__restrict__ QB,
                                                      size_t qstride, uint8_t *__restrict__ R) {
-  // XCD-contiguous remap over the whole grid, then (pair*nmod+m, tile pair)
+  // XCD-contiguous remap over the whole grid, then (pair*nmod+m, panel)
   const int bid = blockIdx.x, nb = gridDim.x;
   const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
   const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + i8;
-  // tile pairs per GEMM: column block J has J/2 + 1 of them; blocks 2c and
-  // 2c+1 together start at c(c+1)
-  const int ncb = (nt + 1) / 2;
-  const int per = ncb * (ncb + 1) - ((nt & 1) ? ncb : 0);
+  const int per = (int)oz_panels(nt);
   const int qm = t / per, o = t - qm * per;
   const int q = qm / nmod, m = qm - q * nmod;
-  int c = (int)((sqrt(4.0 * (double)o + 1.0) - 1.0) * 0.5);
-  while ((c + 1) * (c + 2) <= o) c++;
-  while (c * (c + 1) > o) c--;
-  const int rr = o - c * (c + 1);
-  const int J = rr <= c ? 2 * c : 2 * c + 1;
-  const int IP = rr <= c ? rr : rr - (c + 1);
+  int c = (int)((sqrt(1.0 + 8.0 * (double)o / OZ_P) - 1.0) * 0.5);
+  while (OZ_P * (c + 1) * (c + 2) / 2 <= o) c++;
+  while (OZ_P * c * (c + 1) / 2 > o) c--;
+  const int rr = o - OZ_P * c * (c + 1) / 2;
+  const int J = OZ_P * c + rr / (c + 1);
+  const int IT = rr % (c + 1);
   const long long npairs = (long long)nt * (nt + 1) / 2;
-  const int I0 = IP * OZ_R, J0 = J * OZ_T;
+  const int I0 = IT * OZ_R, J0 = J * OZ_T;
 
   const int8_t *Ab = QX + (size_t)(m * 4 + q) * qstride;
   const int8_t *Hb = QB + (size_t)(m * 3 + q) * qstride;
   const int8_t *Ob = QX + (size_t)(m * 4 + (q == 2 ? 3 : q)) * qstride;
 
-  __shared__ v4i sA[2][4][OZ_R], sH[2][4][OZ_T], sO[2][4][OZ_T];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int half = w >> 2, wr = (w >> 1) & 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
-
-  v16i ah[2][2], ao[2][2];
+  // one LDS array per stage and operand: the compiler then sees the DMA into
+  // one stage and the reads of the other as disjoint (no wait inserted)
+  __shared__ v4i sA0[4][OZ_R], sH0[4][OZ_T], sO0[4][OZ_T];
+  __shared__ v4i sA1[4][OZ_R], sH1[4][OZ_T], sO1[4][OZ_T];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rg = w >> 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
+  v16i ah[OZ_P][2], ao[OZ_P][2];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < OZ_P; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) {
       ah[a][b] = v16i{0};
       ao[a][b] = v16i{0};
     }
 
-  // global -> register prefetch one slab ahead (A: 2 chunks of 16 B per
-  // thread, H and O: 1 each); stored to LDS after the next slab's MFMAs
-  v4i pa[2], ph, po;
-  auto load = [&](int s) {
+  // 40 chunks of 64 rows x 16 bytes per stage: wave w loads A chunks
+  // 3w .. 3w+2 (4 k blocks x 6 row chunks), H chunk w and O chunk w
+  // (4 k blocks x 2); each lane moves its 16 bytes to chunk base + 16 lane
+  const int voff = lane * 16;
+  auto issue = [&](int s, v4i (*dA)[OZ_R], v4i (*dH)[OZ_T], v4i (*dO)[OZ_T]) {
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int cc = tid + 512 * u, kk = cc >> 8, row = cc & 255;
-      pa[u] = *(const v4i *)(Ab + (((size_t)s * 4 + kk) * Gpad + I0 + row) * 16);
+    for (int u = 0; u < 3; u++) {
+      const int ci = w * 3 + u;
+      const int kk = ci / 6, rc = ci - 6 * kk;
+      const int8_t *src = Ab + (((size_t)s * 4 + kk) * Gpad + I0 + rc * 64) * 16;
+      __builtin_amdgcn_global_load_lds(EXH_GLB(src + voff), EXH_LDS(&dA[kk][rc * 64]), 16, 0, 0);
     }
-    const int kk = tid >> 7, col = tid & 127;
-    ph = *(const v4i *)(Hb + (((size_t)s * 4 + kk) * Gpad + J0 + col) * 16);
-    po = *(const v4i *)(Ob + (((size_t)s * 4 + kk) * Gpad + J0 + col) * 16);
+    const int kk = w >> 1, rc = w & 1;
+    const size_t boff = (((size_t)s * 4 + kk) * Gpad + J0 + rc * 64) * 16;
+    __builtin_amdgcn_global_load_lds(EXH_GLB(Hb + boff + voff), EXH_LDS(&dH[kk][rc * 64]), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(EXH_GLB(Ob + boff + voff), EXH_LDS(&dO[kk][rc * 64]), 16, 0, 0);
   };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int cc = tid + 512 * u, kk = cc >> 8, row = cc & 255;
-      sA[buf][kk][row] = pa[u];
-    }
-    const int kk = tid >> 7, col = tid & 127;
-    sH[buf][kk][col] = ph;
-    sO[buf][kk][col] = po;
-  };
-  load(0);
-  store(0);
-  __syncthreads();
-  if (nslab > 1) load(1);
-  for (int s = 0; s < nslab; s++) {
-    const int buf = s & 1;
+  // Operand lane layout: row/column = lane%32, 16 k bytes from k block
+  // 2*ks + lane/32; result D[8*(r/4) + 4*(lane/32) + r%4][lane%32].
+  auto compute = [&](v4i (*sA)[OZ_R], v4i (*sH)[OZ_T], v4i (*sO)[OZ_T]) {
 #pragma unroll
     for (int ks = 0; ks < 2; ks++) {
       const int kk = 2 * ks + lh;
-      v4i av[2], hv[2], ov[2];
+      v4i av[OZ_P], hv[2], ov[2];
 #pragma unroll
-      for (int a = 0; a < 2; a++) av[a] = sA[buf][kk][half * OZ_T + wr * 64 + a * 32 + l32];
+      for (int a = 0; a < OZ_P; a++) av[a] = sA[kk][rg * 32 * OZ_P + a * 32 + l32];
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        hv[b] = sH[buf][kk][wc * 64 + b * 32 + l32];
-        ov[b] = sO[buf][kk][wc * 64 + b * 32 + l32];
+        hv[b] = sH[kk][wc * 64 + b * 32 + l32];
+        ov[b] = sO[kk][wc * 64 + b * 32 + l32];
       }
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < OZ_P; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
           ah[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], hv[b], ah[a][b], 0, 0, 0);
           ao[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], ov[b], ao[a][b], 0, 0, 0);
         }
     }
-    if (s + 1 < nslab) {
-      store(buf ^ 1);
-      if (s + 2 < nslab) load(s + 2);
-    }
+  };
+  // vmcnt(0): every DMA of this wave has landed in LDS; the barrier then
+  // makes all waves' chunks visible and frees the stage read last
+  issue(0, sA0, sH0, sO0);
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  for (int s = 0; s < nslab; s += 2) {
+    if (s + 1 < nslab) issue(s + 1, sA1, sH1, sO1);
+    compute(sA0, sH0, sO0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
+    if (s + 1 < nslab) {
+      if (s + 2 < nslab) issue(s + 2, sA0, sH0, sO0);
+      compute(sA1, sH1, sO1);
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      __syncthreads();
+    }
   }
 
-  // residues mod p_m of this half's 128x128 tile (when it is an upper tile
-  // of the matrix), four consecutive rows packed per 32-bit store; H product
-  // q and O product 3 + q keep the per-product layout of R
-  const int I = 2 * IP + half;
-  if (I > J) return;  // lower half of a diagonal pair: below the diagonal
-  const long long tile = (long long)J * (J + 1) / 2 + I;
   const int p = c_p[m];
   const float rp = c_rp[m];
-  store_residues(ah, p, rp, wr, wc, lh, l32,
-                 R + ((size_t)(q * nmod + m) * npairs + tile) * (OZ_T * OZ_T));
-  store_residues(ao, p, rp, wr, wc, lh, l32,
-                 R + ((size_t)((3 + q) * nmod + m) * npairs + tile) * (OZ_T * OZ_T));
+  store_residues(ah, p, rp, rg, wc, lh, l32, IT, J,
+                 R + (size_t)(q * nmod + m) * npairs * (OZ_T * OZ_T));
+  store_residues(ao, p, rp, rg, wc, lh, l32, IT, J,
+                 R + (size_t)((3 + q) * nmod + m) * npairs * (OZ_T * OZ_T));
 }
 
 // Garner with symmetric digits: the integer x in (-M/2, M/2) with residues r_k.
@@ -448,11 +471,11 @@ hipError_t oz_upload_constants(int nmod) {
 
 OzLayout oz_layout(int ngkmax, int Ktot, int nmod) {
   OzLayout L;
-  // rows padded to the 256-row panel of the GEMM workgroups
-  L.Gpad = (ngkmax + 2 * OZ_T - 1) / (2 * OZ_T) * (2 * OZ_T);
+  // rows padded to the 384-row panel of the GEMM workgroups
+  L.Gpad = (ngkmax + OZ_R - 1) / OZ_R * OZ_R;
   L.Kpad = (Ktot + OZ_KS - 1) / OZ_KS * OZ_KS;
   L.qstride = (size_t)L.Kpad * L.Gpad;
-  const long long nt = L.Gpad / OZ_T;
+  const long long nt = (ngkmax + OZ_T - 1) / OZ_T;  // tile columns of the largest basis
   L.max_pairs = nt * (nt + 1) / 2;
   L.qx_bytes = (size_t)nmod * 4 * L.qstride;
   L.qb_bytes = (size_t)nmod * 3 * L.qstride;
@@ -485,8 +508,7 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
     if (e != hipSuccess) return e;
   }
   if (stage_mask & 2) {
-    const long long ncb = (nt + 1) / 2;
-    const long long per = ncb * (ncb + 1) - ((nt & 1) ? ncb : 0);
+    const long long per = oz_panels((int)nt);
     hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)(per * 3 * nmod)), dim3(512), 0, s, (int)nt,
                        nslab, L.Gpad, nmod, QX, QB, L.qstride, R);
     hipError_t e = hipGetLastError();

@@ -575,25 +575,26 @@ __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp, PA __res
                                                const LoTables &t, PH __restrict__ H,
                                                PH __restrict__ O, long long ldh,
                                                int accumulate) {
-  const long long *__restrict__ colA = t.colA;
-  const int *__restrict__ nlorb = t.nlorb;
-  const int *__restrict__ lorbl = t.lorbl;
-  const int *__restrict__ apword = t.apword;
-  const double *__restrict__ salo = t.salo;
+  // every table index below is uniform across the wave: scalar loads
+  const EXH_C long long *__restrict__ colA = cptr(t.colA);
+  const EXH_C int *__restrict__ nlorb = cptr(t.nlorb);
+  const EXH_C int *__restrict__ lorbl = cptr(t.lorbl);
+  const EXH_C int *__restrict__ apword = cptr(t.apword);
+  const EXH_C double *__restrict__ salo = cptr(t.salo);
   const int maxnlo = t.maxnlo, nlomax = t.nlomax, lmaxapw = t.lmaxapw, pmax = t.pmax,
             lmmax = t.lmmax, ngkmax = t.ngkmax;
-  const int *__restrict__ Koff = t.Koff;
-  const int *__restrict__ Ms = t.Ms;
-  const int *__restrict__ lobase = t.lobase;
-  const double2 *__restrict__ loa = t.loa;
+  const EXH_C int *__restrict__ Koff = cptr(t.Koff);
+  const EXH_C int *__restrict__ Ms = cptr(t.Ms);
+  const EXH_C int *__restrict__ lobase = cptr(t.lobase);
+  const EXH_C double2 *__restrict__ loa = cptr(t.loa);
   const int maxaa = t.maxaa;
-  const int is = t.atom_species[ias];
-  const int los = t.losize[is];
+  const int is = cptr(t.atom_species)[ias];
+  const int los = cptr(t.losize)[is];
   if (los == 0 || g >= ngp) return;
   const int r0 = ngp + lobase[ias];
-  const long long *col = colA + Koff[ias];
+  const EXH_C long long *col = colA + Koff[ias];
   const int M = Ms[ias];
-  const double2 *L = loa + (size_t)ias * maxnlo * maxaa;
+  const EXH_C double2 *L = loa + (size_t)ias * maxnlo * maxaa;
 
   if (H) {  // null for overlapsetup alone
   double2 acc[MAXLO];
@@ -601,11 +602,11 @@ __device__ __forceinline__ void lo_border_body(int g, int ias, int ngp, PA __res
   for (int r = 0; r < MAXLO; r++) acc[r] = z2(0, 0);
   for (int f = 0; f < M; f++) {
     const double2 a = ld(apwalm + col[f] + g);
-    const double2 *Lf = L + (size_t)maxnlo * f;
+    const EXH_C double2 *Lf = L + (size_t)maxnlo * f;
 #pragma unroll
     for (int r = 0; r < MAXLO; r++)
       if (r < los) {
-        const double2 lv = Lf[r];
+        const double2 lv = ld(Lf + r);
         acc[r].x += a.x * lv.x - a.y * lv.y;
         acc[r].y += a.x * lv.y + a.y * lv.x;
       }
@@ -660,9 +661,29 @@ template <int MAXLO>
 __global__ __launch_bounds__(64) void lo_border_grouped_kernel(const KDesc *__restrict__ kd,
                                                                int accumulate) {
   const KDesc &d = kd[blockIdx.z];
-  const SDesc &S = *d.sd;
+  const EXH_C SDesc &S = *cptr(d.sd);
   if ((int)blockIdx.y >= S.natmtot || S.nlotot == 0) return;
-  const LoTables t = S.lo;
+  // field by field: scalar loads from the descriptor
+  const EXH_C LoTables &L = S.lo;
+  LoTables t;
+  t.colA = L.colA;
+  t.Koff = L.Koff;
+  t.Ms = L.Ms;
+  t.atom_species = L.atom_species;
+  t.losize = L.losize;
+  t.lobase = L.lobase;
+  t.loa = L.loa;
+  t.maxnlo = L.maxnlo;
+  t.maxaa = L.maxaa;
+  t.nlorb = L.nlorb;
+  t.lorbl = L.lorbl;
+  t.nlomax = L.nlomax;
+  t.apword = L.apword;
+  t.lmaxapw = L.lmaxapw;
+  t.pmax = L.pmax;
+  t.lmmax = L.lmmax;
+  t.ngkmax = L.ngkmax;
+  t.salo = L.salo;
   lo_border_body<MAXLO>(blockIdx.x * 64 + threadIdx.x, blockIdx.y, d.ngp,
                         gptr((const double2 *)d.apwalm), t, gptr(d.h), gptr(d.o), d.ldh,
                         accumulate);
@@ -929,6 +950,7 @@ SDesc make_sdesc(const exh_context *c) {
   std::memset(&S, 0, sizeof S);
   const int *ki = (const int *)c->katom.p;  // [Koff(natmtot), Ms(natmtot), kinfo(Ktot)]
   S.natmtot = c->natmtot;
+  S.nspecies = c->nspecies;
   S.Ktot = c->Ktot;
   S.maxaa = c->maxaa;
   S.lmaxapw = c->lmaxapw;

@@ -88,9 +88,9 @@ def context_bytes(sys_, nmod_max: int = 12) -> int:
     ldb = (sys_.ngkmax + 63) // 64 * 64
     scratch = 16 * K * ldb                       # aa.A operand
     if nmod_max:
-        gpad = (sys_.ngkmax + 127) // 128 * 128
+        gpad = (sys_.ngkmax + 383) // 384 * 384     # rows padded to the GEMM's panel
         kpad = (K + 63) // 64 * 64
-        nt = gpad // 128
+        nt = (sys_.ngkmax + 127) // 128
         scratch += 7 * nmod_max * kpad * gpad    # int8 residues of A and aa.A
         scratch += 6 * nmod_max * nt * (nt + 1) // 2 * 128 * 128   # product residues
     perk = len(sys_.kpts) * sys_.ngkmax * (16 * sys_.natmtot + 8 * 7 + 4)
