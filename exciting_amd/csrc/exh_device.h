@@ -58,6 +58,18 @@ __device__ __forceinline__ void st(double2 *p, double2 v) { *p = v; }
 __device__ __forceinline__ void st(EXH_G double2 *p, double2 v) {
   *(EXH_G dv2 *)p = dv2{v.x, v.y};
 }
+// Streaming stores of data that will not be read from cache again (arrays
+// far larger than the caches): non-temporal, one 16-byte store.
+__device__ __forceinline__ void st_nt(double2 *p, double2 v) {
+  double *q = (double *)p;
+  __builtin_nontemporal_store(v.x, q);
+  __builtin_nontemporal_store(v.y, q + 1);
+}
+__device__ __forceinline__ void st_nt(EXH_G double2 *p, double2 v) {
+  EXH_G double *q = (EXH_G double *)p;
+  __builtin_nontemporal_store(v.x, q);
+  __builtin_nontemporal_store(v.y, q + 1);
+}
 __device__ __forceinline__ void st(int4 *p, int4 v) { *p = v; }
 __device__ __forceinline__ void st(EXH_G int4 *p, int4 v) {
   *(EXH_G iv4 *)p = iv4{v.x, v.y, v.z, v.w};

@@ -182,26 +182,24 @@ __device__ __forceinline__ void d_sbesseldm(int m, double x,
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup: MG consecutive G+k vectors x MA consecutive atoms, after
-// match_prep has tabulated, per G+k, conj(Y_lm) * 4 pi i^l / sqrt(Omega)
-// and, per (G+k, species), |G+k|^io j_l^(io)(|G+k| R).
-//   Stage 2 (per species run of the tile's atoms): the radial rows into LDS.
-//   Stage 3: the boundary solve D v = (j_l^(io)) per (G+k, l, atom) with the
-//     LU factors of (l, atom) (zgetrs order on the real radial vector; the
-//     complex factor S_a(G+k) conj(Y_lm) fpso i^l multiplies the solution),
-//     then apwalm(G, io, lm, a) = v_io * S_a * yt_lm.  Lane half h writes
-//     io = h, h + 2, ... of MG consecutive G: 512-byte row segments.
-// The write loop holds few registers and little LDS, so many waves keep
-// stores in flight.
+// One workgroup: MG = 256 consecutive G+k vectors (one per lane) x MA
+// consecutive atoms, after match_prep has tabulated, per G+k,
+// conj(Y_lm) * 4 pi i^l / sqrt(Omega) and, per (G+k, species),
+// |G+k|^io j_l^(io)(|G+k| R).  For each l the workgroup solves the boundary
+// system D v = (j_l^(io)) per (G+k, atom) with the LU factors of (l, atom)
+// (zgetrs order on the real radial vector; the complex factor
+// S_a(G+k) conj(Y_lm) fpso i^l multiplies the solution), then for each m
+// reads the harmonics once and writes apwalm(G, io, lm, a) = v_io S_a yt_lm
+// for every atom of the tile.  All four waves walk the same column sequence,
+// so every column receives 4 KB contiguous per workgroup step: the widest
+// row segments apwalm's leading dimension (the caller's ngkmax, not aligned
+// to cache lines) allows.  The tables (a few MB per k-point) stay in cache;
+// the loop holds few registers and no LDS, so many waves keep stores in
+// flight.
 // ---------------------------------------------------------------------------
-constexpr int MG = 32;    // G+k vectors per workgroup
-constexpr int MA = 8;     // atoms per workgroup
-constexpr int MTH = 256;  // threads: MG x 8 slots
-
-template <int LM>
-struct MatchLds {
-  double dj[PCAP][LM + 1][MG];  // |G+k|^io j_l^(io)(|G+k| R) of one species
-};
+constexpr int MG = 256;   // G+k vectors per workgroup
+constexpr int MA = 4;     // atoms per workgroup
+constexpr int MTH = 256;  // threads: one per G+k
 
 // conj(Y_lm) * fpso * i^l for m = -L..L of one G+k direction (genylm.f90:56-90),
 // into out[(L + m) * stride] (out points at lm = L*L).
@@ -369,94 +367,72 @@ __device__ __forceinline__ void match_tile(
     int g0, int a0, int ngp, int natmtot, int ld, int pmax, PS __restrict__ sfacgp,
     PA __restrict__ apwalm, const int *__restrict__ atom_species, PR __restrict__ rad,
     PY __restrict__ yt, const int *__restrict__ apword, const double *__restrict__ lu,
-    const int *__restrict__ piv, MatchLds<LM> &sm) {
-  constexpr int LMM = (LM + 1) * (LM + 1);
-  const int a1 = min(natmtot, a0 + MA);
-  if (g0 >= ngp || a0 >= a1) return;  // uniform per workgroup
-  // stage 3 layout: wave wv writes lm = wv, wv + 4, ... (uniform, so the
-  // LU factors are scalar loads); lane half h writes io = h, h + 2, ...
-  const int t = threadIdx.x, gl = t % MG, h = (t >> 5) & 1;
-  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int g = g0 + gl;
-  const bool gv = g < ngp;
+    const int *__restrict__ piv) {
+  const int na = min(natmtot, a0 + MA) - a0;
+  if (g0 >= ngp || na <= 0) return;  // uniform per workgroup
+  const int g = g0 + (int)threadIdx.x;
+  if (g >= ngp) return;  // no barriers below
   const EXH_C double *LUt = cptr(lu);
   const EXH_C int *PVt = cptr(piv);
   const EXH_C int *APt = cptr(apword);
-
-  int ia = a0;
-  while (ia < a1) {
-    const int is = atom_species[ia];
-    int ib = ia + 1;
-    while (ib < a1 && atom_species[ib] == is) ib++;
-    __syncthreads();  // stage 1 written / previous species' stage 3 done
-    // stage 2: the radial rows of species is, from the k-point's radial table
-    for (int it = t; it < PCAP * (LM + 1) * MG; it += MTH) {
-      const int ig = it % MG, row = it / MG;
-      const int gg = min(g0 + ig, ngp - 1);
-      (&sm.dj[0][0][0])[row * MG + ig] = exh::ld(rad + ((size_t)is * PCAP * (LM + 1) + row) * ld + gg);
-    }
-    __syncthreads();
-    // stage 3: wave wv takes lm = wv, wv + 4, ...; for each it reads the
-    // harmonics row segment once (the table stays in cache) and writes it
-    // for every atom of the run, whose boundary solutions (per G, l and
-    // atom) are redone only when l changes
-    const int na = ib - ia;  // <= MA, uniform
-    double2 S[MA];
+  const EXH_C int *ASt = cptr(atom_species);
+  int is[MA];
+  double2 S[MA];
 #pragma unroll
-    for (int u = 0; u < MA; u++)
-      S[u] = (u < na && gv) ? exh::ld(sfacgp + g + (size_t)ld * (ia + u)) : z2(0.0, 0.0);
-    int lcur = -1, p = 0;
+  for (int u = 0; u < MA; u++) {
+    is[u] = u < na ? ASt[a0 + u] : 0;
+    S[u] = u < na ? exh::ld(sfacgp + g + (size_t)ld * (a0 + u)) : z2(0.0, 0.0);
+  }
+#pragma unroll 1
+  for (int l = 0; l <= LM; l++) {
+    // boundary solutions of every atom of the tile for this l
     double v[MA][PCAP];
-    for (int lm = wv; lm < LMM; lm += MTH / 64) {
-      int l = (int)sqrt((double)lm);
-      l -= (l * l > lm) ? 1 : 0;
-      l += ((l + 1) * (l + 1) <= lm) ? 1 : 0;
-      if (l != lcur) {
-        lcur = l;
-        p = APt[l + (LM + 1) * is];
-        double d0[PCAP];
+    int p[MA];
 #pragma unroll
-        for (int io = 0; io < PCAP; io++) d0[io] = io < p ? sm.dj[io][l][gl] : 0.0;
+    for (int u = 0; u < MA; u++) {
+      p[u] = 0;
+      if (u >= na) continue;
+      const int pu = APt[l + (LM + 1) * is[u]];
+      p[u] = pu;
+      const EXH_C double *LU = LUt + (size_t)PCAP * PCAP * (l + (LM + 1) * (a0 + u));
+      const EXH_C int *PV = PVt + PCAP * (l + (LM + 1) * (a0 + u));
+      double x[PCAP];
 #pragma unroll
-        for (int u = 0; u < MA; u++) {
-          if (u >= na) break;
-          const EXH_C double *LU = LUt + (size_t)PCAP * PCAP * (l + (LM + 1) * (ia + u));
-          const EXH_C int *PV = PVt + PCAP * (l + (LM + 1) * (ia + u));
-          double x[PCAP];
+      for (int io = 0; io < PCAP; io++)
+        x[io] = io < pu ? exh::ld(rad + ((size_t)(is[u] * PCAP + io) * (LM + 1) + l) * ld + g)
+                        : 0.0;
+      // zgetrs: row interchanges, unit-lower then upper solve
 #pragma unroll
-          for (int io = 0; io < PCAP; io++) x[io] = d0[io];
-          // zgetrs: row interchanges, unit-lower then upper solve
+      for (int i = 0; i < PCAP; i++)
+        if (i < pu) {
+          const int ip = PV[i] - 1;
+          if (ip != i) {
+            double tt = x[i];
 #pragma unroll
-          for (int i = 0; i < PCAP; i++)
-            if (i < p) {
-              const int ip = PV[i] - 1;
-              if (ip != i) {
-                double tt = x[i];
-#pragma unroll
-                for (int j = 0; j < PCAP; j++)
-                  if (j == ip) {
-                    x[i] = x[j];
-                    x[j] = tt;
-                  }
+            for (int j = 0; j < PCAP; j++)
+              if (j == ip) {
+                x[i] = x[j];
+                x[j] = tt;
               }
-            }
-#pragma unroll
-          for (int k = 0; k < PCAP; k++)
-#pragma unroll
-            for (int i = k + 1; i < PCAP; i++)
-              if (i < p) x[i] = fma(-x[k], LU[i + PCAP * k], x[i]);
-#pragma unroll
-          for (int k = PCAP - 1; k >= 0; k--)
-            if (k < p) {
-              x[k] = x[k] / LU[k + PCAP * k];
-#pragma unroll
-              for (int i = 0; i < k; i++) x[i] = fma(-x[k], LU[i + PCAP * k], x[i]);
-            }
-#pragma unroll
-          for (int io = 0; io < PCAP; io++) v[u][io] = x[io];
+          }
         }
-      }
-      if (!gv) continue;
+#pragma unroll
+      for (int k = 0; k < PCAP; k++)
+#pragma unroll
+        for (int i = k + 1; i < PCAP; i++)
+          if (i < pu) x[i] = fma(-x[k], LU[i + PCAP * k], x[i]);
+#pragma unroll
+      for (int k = PCAP - 1; k >= 0; k--)
+        if (k < pu) {
+          x[k] = x[k] / LU[k + PCAP * k];
+#pragma unroll
+          for (int i = 0; i < k; i++) x[i] = fma(-x[k], LU[i + PCAP * k], x[i]);
+        }
+#pragma unroll
+      for (int io = 0; io < PCAP; io++) v[u][io] = x[io];
+    }
+#pragma unroll 1
+    for (int lm = l * l; lm < (l + 1) * (l + 1); lm++) {
       const double2 y = exh::ld(yt + (size_t)lm * ld + g);
 #pragma unroll
       for (int u = 0; u < MA; u++) {
@@ -465,14 +441,13 @@ __device__ __forceinline__ void match_tile(
         // (the compiler would otherwise fuse either product)
         const double wr = fma(S[u].x, y.x, -(S[u].y * y.y));
         const double wi = fma(S[u].x, y.y, S[u].y * y.x);
-        const PA out = apwalm + g + (size_t)ld * pmax * LMM * (ia + u);
+        const PA out = apwalm + g + (size_t)ld * pmax * ((LM + 1) * (LM + 1)) * (a0 + u);
 #pragma unroll
         for (int io = 0; io < PCAP; io++)
-          if (io < p && (io & 1) == h)
-            st(out + (size_t)ld * (io + (size_t)pmax * lm), z2(v[u][io] * wr, v[u][io] * wi));
+          if (io < p[u])
+            st_nt(out + (size_t)ld * (io + (size_t)pmax * lm), z2(v[u][io] * wr, v[u][io] * wi));
       }
     }
-    ia = ib;
   }
 }
 
@@ -482,11 +457,9 @@ __global__ __launch_bounds__(MTH) void match_kernel(
     double2 *__restrict__ apwalm, const int *__restrict__ atom_species,
     const double *__restrict__ tab, const int *__restrict__ apword,
     const double *__restrict__ lu, const int *__restrict__ piv) {
-  __shared__ MatchLds<LM> sm;
   match_tile<LM>(blockIdx.x * MG, blockIdx.y * MA, ngp, natmtot, ld, pmax, sfacgp, apwalm,
                  atom_species, tab,
-                 (const double2 *)(tab + match_rad_doubles(nspecies, LM, ld)), apword, lu, piv,
-                 sm);
+                 (const double2 *)(tab + match_rad_doubles(nspecies, LM, ld)), apword, lu, piv);
 }
 
 // Grouped over k-points: blockIdx.z selects the descriptor, blockIdx.y the
@@ -494,7 +467,6 @@ __global__ __launch_bounds__(MTH) void match_kernel(
 // structures with another lmaxapw exit at once.
 template <int LM>
 __global__ __launch_bounds__(MTH) void match_grouped_kernel(const KDesc *__restrict__ kd) {
-  __shared__ MatchLds<LM> sm;
   const KDesc &d = kd[blockIdx.z];
   const EXH_C SDesc &S = *cptr(d.sd);
   if (S.lmaxapw != LM) return;
@@ -502,7 +474,7 @@ __global__ __launch_bounds__(MTH) void match_grouped_kernel(const KDesc *__restr
   match_tile<LM>(blockIdx.x * MG, blockIdx.y * MA, d.ngp, S.natmtot, S.ngkmax, S.pmax,
                  gptr(d.sfac), gptr(d.apwalm), S.atom_species, gptr(tab),
                  gptr((const double2 *)(tab + match_rad_doubles(S.nspecies, LM, S.ngkmax))),
-                 S.apword, S.lu, S.piv, sm);
+                 S.apword, S.lu, S.piv);
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }

@@ -48,6 +48,41 @@ def _rows(s):
     return out
 
 
+def device_tables(ctx, dev):
+    """The per-structure tables forcek and kinetic_energy read, resident on
+    ``dev`` and kept on the context: uploaded on first use, again only when
+    the context is loaded with another structure or other host muffin-tin
+    tables.  ivg (3, ngrtot) and ivgig (flattened box) as int64; the
+    context's host aa, loa and salo = oalo + h1loa in [ias, ...] order; per
+    species the atom indices, APW row indices and LO columns."""
+    import torch
+    s = ctx.sys
+    key = (str(dev), id(s), id(s.aa), id(s.loa), id(s.oalo), id(s.h1loa))
+    c = getattr(ctx, "_device_tables", None)
+    if c is not None and c["key"] == key:
+        return c
+    species = np.repeat(np.arange(s.nspecies), s.natoms)
+    salo = np.asarray(s.oalo) + (np.asarray(s.h1loa) if s.h1loa is not None else 0.0)
+    c = {
+        "key": key,
+        "ivg": torch.as_tensor(np.asarray(s.ivg, dtype=np.int64), device=dev),
+        "ivgig": torch.as_tensor(np.asarray(s.ivgig, dtype=np.int64).ravel(order="F"),
+                                 device=dev),
+        "aa": torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.aa, 2, 0)), device=dev),
+        "loa": torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.loa, 2, 0)), device=dev),
+        "salo": torch.as_tensor(np.ascontiguousarray(np.moveaxis(salo, (0, 1, 2), (2, 1, 0))),
+                                device=dev),                        # [ias, ilo, io]
+        "lobase": np.concatenate([[0], np.cumsum([s.losize[i] for i in species])])[:-1],
+        "species": [],
+    }
+    for is_, (r_idx, lo_cols) in enumerate(_rows(s)):
+        at = np.nonzero(species == is_)[0]
+        c["species"].append((at, torch.as_tensor(at, device=dev),
+                             torch.as_tensor(r_idx, device=dev), len(r_idx), lo_cols))
+    ctx._device_tables = c
+    return c
+
+
 def density_matrix(evecsv, occsv, nstfv, nspinor=2, spiral_ispn=None):
     """The state weights of the second-variational force sum
     (forcek.f90:341-366): dm = sum_j occsv_j sum_b v_bj v_bj^H over the
@@ -80,16 +115,18 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None, tables=None):
     tables in the reference's Fortran layouts, flattened (aa(maxaa, maxaa,
     natmtot), loa(maxnlo, maxaa, natmtot), salo = oalo + h1loa
     (apwordmax, nlomax, natmtot)), e.g. the outputs of hmlint on the device;
-    without them the context's loaded host tables are uploaded."""
+    without them the context's host tables, kept on the device with the
+    other per-structure tables (device_tables), are used."""
     import torch
     s = ctx.sys
     dev = evec.device
     ctx.match(d.ngk, d.gpc, d.tpgpc, d.sfacgp, d.apwalm)
     ngk, natm, p = d.ngk, s.natmtot, s.apwordmax
     lmm = (s.lmaxapw + 1) ** 2
-    igp = d.igpig.cpu().numpy()[:ngk].astype(np.int64) - 1              # G index of each row
+    dt = device_tables(ctx, dev)
+    igp_t = d.igpig[:ngk].long() - 1                                   # G index of each row
     # index of G_i - G_j for every pair of rows, on the device (ngk^2 entries)
-    ivk = torch.as_tensor(np.asarray(s.ivg)[:, igp], device=dev)        # (3, ngk)
+    ivk = dt["ivg"][:, igp_t]                                          # (3, ngk)
     lo = [int(x) for x in np.asarray(s.intgv)[:, 0]]
     hi = [int(x) for x in np.asarray(s.intgv)[:, 1]]
     nb = [h - l + 1 for l, h in zip(lo, hi)]
@@ -99,8 +136,7 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None, tables=None):
         if bool(((da < lo[a]) | (da > hi[a])).any()):
             raise NotImplementedError("G+k differences outside the FFT box (gmaxvr < 2 gkmax)")
         box = box * nb[a] + (da - lo[a])
-    ivgig = torch.as_tensor(np.asarray(s.ivgig).ravel(order="F"), device=dev)
-    ig_t = ivgig[box].long() - 1                                          # (ngk, ngk)
+    ig_t = dt["ivgig"][box] - 1                                           # (ngk, ngk)
     if int(ig_t.max()) >= s.ngvec:
         raise NotImplementedError("G differences beyond ngvec")
     XG = evec[:, :ngk].transpose(0, 1).contiguous()                     # (ngk, nst)
@@ -108,7 +144,6 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None, tables=None):
     vg = d.vgpc.view(-1, 3)[:ngk]                                        # (ngk, 3) G+k
     dp = 0.5 * (vg @ vg.transpose(0, 1))                                 # dpij
     A_all = d.apwalm.view(natm, lmm * p, s.ngkmax)
-    rows = _rows(s)
     mnlo = max(s.maxnlo, 1)
     if tables is not None:
         aa_t, loa_t, salo_t = tables
@@ -117,24 +152,16 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None, tables=None):
         # [ias, ilo, io]; never read without local orbitals
         salo_d = salo_t.view(natm, s.nlomax, p) if s.nlomax > 0 else None
     else:
-        aa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.aa, 2, 0)), device=dev)
-        loa = torch.as_tensor(np.ascontiguousarray(np.moveaxis(s.loa, 2, 0)), device=dev)
-        salo = np.asarray(s.oalo) + (np.asarray(s.h1loa) if s.h1loa is not None else 0.0)
-        salo_d = torch.as_tensor(np.ascontiguousarray(np.moveaxis(salo, (0, 1, 2), (2, 1, 0))),
-                                 device=dev)                            # [ias, ilo, io]
+        aa, loa, salo_d = dt["aa"], dt["loa"], dt["salo"]
     YH = torch.zeros((natm, ngk, nst), dtype=torch.complex128, device=dev)
     YO = torch.zeros_like(YH)
-    species = np.repeat(np.arange(s.nspecies), s.natoms)
-    lobase = np.concatenate([[0], np.cumsum([s.losize[i] for i in species])])[:-1]
+    lobase = dt["lobase"]
     ngk_lo = ngk
     for is_ in range(s.nspecies):
-        at = np.nonzero(species == is_)[0]
-        r_idx, lo_cols = rows[is_]
-        at_t = torch.as_tensor(at, device=dev)
-        A = A_all[at_t][:, torch.as_tensor(r_idx, device=dev), :ngk]     # (na, nr, ngk)
+        at, at_t, r_t, nr, lo_cols = dt["species"][is_]
+        A = A_all[at_t][:, r_t, :ngk]                                    # (na, nr, ngk)
         B = torch.matmul(A, XG)                                          # (na, nr, nst)
         Ah = A.conj().transpose(1, 2)
-        nr = len(r_idx)
         YH[at_t] = torch.matmul(Ah, torch.matmul(aa[at_t][:, :nr, :nr], B))
         YO[at_t] = torch.matmul(Ah, B)
         los = int(s.losize[is_])
@@ -154,7 +181,7 @@ def forcek(ctx, d, evec, evalfv, occ, wkpt, ffacg, sfacg, dm=None, tables=None):
             YO[at_t] += torch.matmul(OGL.conj(), XL)
         # step-function term of the species: F(i, j) = -ffacg(G_i - G_j)
         F = -ffacg[ig_t, is_]                                           # (ngk, ngk) real
-        ph = sfacg[torch.as_tensor(igp, device=dev)][:, at_t].transpose(0, 1)   # (na, ngk)
+        ph = sfacg[igp_t][:, at_t].transpose(0, 1)                       # (na, ngk)
         PX = ph[:, :, None] * XG[None]                                   # p * X
         W = PX.transpose(0, 1).reshape(ngk, -1)                          # (ngk, na nst)
         # one real GEMM: [F; dpij F] (2 ngk x ngk) times [Re W, Im W]

@@ -127,7 +127,8 @@ def kinetic_energy(ctx, d, evec, cfunir, lofr, valence_rel_none=True, veffmt=Non
     e_mt = (u.conj() * Mu).sum(dim=(0, 1)).real
     # interstitial part
     ngrid = tuple(int(x) for x in s.intgv[:, 1] - s.intgv[:, 0] + 1)
-    igp = d.igpig.cpu().numpy()[:ngp].astype(np.int64) - 1
+    from .forces import device_tables
+    igp_t = d.igpig[:ngp].long() - 1
     cfun = cfunir
     if not valence_rel_none:
         cfun = cfunir / (1.0 - 0.5 * ALPHA * ALPHA * veffir)
@@ -141,7 +142,7 @@ def kinetic_energy(ctx, d, evec, cfunir, lofr, valence_rel_none=True, veffmt=Non
     # one matrix product over the three directions replace 2 x 3 x nstfv
     # grid transforms.
     c = torch.fft.fftn(cf.to(torch.complex128), norm="forward").reshape(-1)
-    ivk = torch.as_tensor(np.asarray(s.ivg)[:, igp], device=dev)          # (3, ngp)
+    ivk = device_tables(ctx, dev)["ivg"][:, igp_t]                       # (3, ngp)
     box = torch.zeros((ngp, ngp), dtype=torch.int64, device=dev)
     for a in (2, 1, 0):                       # first grid index fastest
         da = torch.remainder(ivk[a][:, None] - ivk[a][None, :], ngrid[a])
