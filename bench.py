@@ -73,11 +73,12 @@ PRIMES = (251, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191)
 
 
 def oz_bits(nmod, K):
-    """Operand bits of the split path (the library's rule): the largest b with
-    6 K 2^(2b) below half the product of the moduli."""
+    """Operand bits of the split path (the library's rule): every G column is
+    scaled so that its 2-norm is below 2^c, c the largest value with 2^(2c)
+    below half the product of the moduli (K does not enter)."""
     import math
     lm = sum(math.log2(p) for p in PRIMES[:nmod])
-    return int(min(30, math.floor((lm - 1.0 - math.log2(6.0 * K)) * 0.5)))
+    return int(min(50, math.floor((lm - 1.0) * 0.5 - 1e-9)))
 
 
 def pick_moduli(W, arg, nst):
@@ -829,7 +830,7 @@ def main():
             "n_mean": float(np.mean(W.ngks)) + s.nlotot,
             "natoms": s.natmtot,
             "precision": (f"split: muffin-tin products by int8 Ozaki scheme II with {nmod} "
-                          f"moduli (operands rounded to {oz_bits(nmod, W.K)} bits per G "
+                          f"moduli (G columns scaled to 2-norm 2^{oz_bits(nmod, W.K)} and rounded "
                           f"column); match, LO blocks, interstitial FP64") if nmod else "fp64",
         },
         "gflops": head["gflops"],

@@ -235,8 +235,8 @@ int oz_prepare(exh_context *c, int nmod, OzLayout &L, int &bits) {
   int rc = ensure(c, c->ozqx, L.qx_bytes);
   if (!rc) rc = ensure(c, c->ozqb, L.qb_bytes);
   if (!rc) rc = ensure(c, c->ozr, L.r_bytes);
-  if (!rc) rc = ensure(c, c->ozex, (size_t)c->ngkmax * sizeof(int));
-  if (!rc) rc = ensure(c, c->ozeb, (size_t)c->ngkmax * sizeof(int));
+  if (!rc) rc = ensure(c, c->ozex, L.e_bytes);
+  if (!rc) rc = ensure(c, c->ozeb, L.e_bytes);
   return rc;
 }
 

@@ -413,7 +413,8 @@ int run_batch(exh_context *lead, void *stream, int nk, const exh_kpoint *kp,
         oz.L[j] = oz_layout(c->ngkmax, c->Ktot, nmod);
         oz.bits[j] = oz_bits(nmod, c->Ktot);
         if (c->ozqx.bytes < oz.L[j].qx_bytes || c->ozqb.bytes < oz.L[j].qb_bytes ||
-            c->ozr.bytes < oz.L[j].r_bytes)
+            c->ozr.bytes < oz.L[j].r_bytes || c->ozex.bytes < oz.L[j].e_bytes ||
+            c->ozeb.bytes < oz.L[j].e_bytes)
           return fail(lead, EXH_ESTATE, "batch: make one call outside graph capture first");
       }
     } else {

@@ -41,6 +41,7 @@ struct OzLayout {
   size_t qstride = 0;           // bytes of one residue array (Kpad * Gpad)
   long long max_pairs = 0;      // upper tile pairs at ngkmax
   size_t qx_bytes = 0, qb_bytes = 0, r_bytes = 0;
+  size_t e_bytes = 0, e_part_off = 0;  // per operand: column exponents + partial norms
 };
 
 // APW-lo border tables (hamiltonsetup.f90:124-151, overlapsetup.f90:116-133)

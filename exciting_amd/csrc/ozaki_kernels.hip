@@ -3,10 +3,11 @@
 // The FP64 products H_MT = X^H B and O_MT = X^H X (X = A^T, B = (aa A)^T,
 // K = every atom's (l, m, io) stacked) are emulated with integer arithmetic:
 //
-//  1. Each G column of X and of B gets a power-of-two scale 2^s so that
-//     |Re|, |Im| <= 2^b; the scaled values are rounded to integers.  This
-//     rounding is the only approximation: the error is <= 2^-b of the
-//     column's largest entry.
+//  1. Each G column of X and of B gets a power-of-two scale 2^s so that its
+//     2-norm is at most 2^(c-1); the scaled values are rounded to integers,
+//     which keeps the norm below 2^c.  This rounding is the only
+//     approximation: the error is <= 1/2 of 2^-s per component, i.e. about
+//     2^-c of the column's norm.
 //  2. The integers are reduced modulo N primes p_m <= 251 into int8 residues
 //     in [-(p-1)/2, (p-1)/2], stored in 16-byte k blocks per G row: the
 //     operand format of v_mfma_i32_32x32x32_i8.
@@ -23,8 +24,13 @@
 //     FP64 and rescaled by 2^-(s_i+s_j).  The interstitial terms and the
 //     conj mirror are added as in the FP64 path.
 //
-// b is chosen from N and K so that |6 K 2^(2b)| < prod(p)/2: every product is
-// recovered exactly from its residues.
+// The reconstructed integers are Re and Im of x_i^H b_j (the Karatsuba
+// products only meet modulo p), bounded by ||x_i|| ||b_j|| <= 2^(2c)
+// (Cauchy-Schwarz); c is the largest value with 2^(2c) < prod(p)/2, so every
+// element is recovered exactly from its residues.  Against scaling by the
+// largest entry and the worst case 6 K 2^(2b) this gains
+// (log2(6K) + 2 log2(max/norm ratio of the columns)) / 2 bits per operand.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <mutex>
@@ -46,24 +52,39 @@ __constant__ int c_w[kOzMaxMod];             // (p_0..p_{k-1})^{-1} mod p_k
 __constant__ int c_c[kOzMaxMod][kOzMaxMod];  // (p_0..p_{j-1}) * w_k mod p_k, j < k
 
 // ---------------------------------------------------------------------------
-// Column exponents: emax[g] = max over k of the binary exponent of |Re|,|Im|.
-// Grid (G blocks, K chunks); atomicMax merges the chunks.
+// Column scales from the 2-norms: part[chunk][g] = sum over the chunk's 256
+// k of |x|^2, then per column the chunks in order (a fixed summation order,
+// so the scale is the same in every run) and emax[g] = E + 1 with
+// ||x_g|| < 2^E: the scale 2^(c - emax) takes the norm below 2^(c-1).
 // ---------------------------------------------------------------------------
-__global__ void colexp_kernel(int ngp, int Ktot, const double2 *__restrict__ X, long long ldx,
-                              int *__restrict__ emax) {
+constexpr int OZ_NCH = 256;  // k per partial sum
+
+__global__ void colnorm_part_kernel(int ngp, int Ktot, const double2 *__restrict__ X,
+                                    long long ldx, double *__restrict__ part, int ldp) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ngp) return;
-  const int k0 = blockIdx.y * 256, k1 = min(Ktot, k0 + 256);
-  double mx = 0.0;
+  const int k0 = blockIdx.y * OZ_NCH, k1 = min(Ktot, k0 + OZ_NCH);
+  double s = 0.0;
   for (int k = k0; k < k1; k++) {
     const double2 v = X[(size_t)k * ldx + g];
-    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+    s = fma(v.x, v.x, s);
+    s = fma(v.y, v.y, s);
   }
-  if (mx > 0.0) {
-    int e;
-    (void)frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1): |values| <= 2^e
-    atomicMax(emax + g, e);
+  part[(size_t)blockIdx.y * ldp + g] = s;
+}
+
+__global__ void colnorm_final_kernel(int ngp, int nchunk, const double *__restrict__ part,
+                                     int ldp, int *__restrict__ emax) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngp) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; c++) s += part[(size_t)c * ldp + g];
+  int e = INT_MIN;
+  if (s > 0.0) {
+    (void)frexp(sqrt(s), &e);  // norm = f 2^e, f in [0.5, 1): norm < 2^e
+    e += 1;
   }
+  emax[g] = e;
 }
 
 __device__ __forceinline__ int smod(double x, int m) {
@@ -406,10 +427,6 @@ __global__ __launch_bounds__(256) void ozcombine_kernel(
   }
 }
 
-__global__ void fill_int_kernel(int n, int v, int *__restrict__ a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
-}
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
@@ -418,11 +435,14 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 const int kOzPrimes[kOzMaxMod] = {251, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191};
 
 int oz_bits(int nmod, int Ktot) {
+  (void)Ktot;
   double lm = 0.0;
   for (int k = 0; k < nmod; k++) lm += std::log2((double)kOzPrimes[k]);
-  // |Im| <= |P1| + |P2| + |P3| <= 6 K 2^(2b) must stay below prod(p)/2
-  const double b = std::floor((lm - 1.0 - std::log2(6.0 * std::max(Ktot, 1))) * 0.5);
-  return (int)std::min(30.0, b);
+  // |Re|, |Im| of x^H b <= ||x|| ||b|| <= 2^(2c) must stay below prod(p)/2
+  // (a hair below the exact bound guards the floating-point log2); the
+  // scaled entries stay well inside FP64's exact integers
+  const double c = std::floor((lm - 1.0) * 0.5 - 1e-9);
+  return (int)std::min(50.0, c);
 }
 
 // The tables cover all kOzMaxMod moduli whatever nmod a call uses, so they are
@@ -480,6 +500,10 @@ OzLayout oz_layout(int ngkmax, int Ktot, int nmod) {
   L.qx_bytes = (size_t)nmod * 4 * L.qstride;
   L.qb_bytes = (size_t)nmod * 3 * L.qstride;
   L.r_bytes = (size_t)6 * nmod * L.max_pairs * OZ_T * OZ_T;
+  // per operand: the column exponents, then the partial norms
+  L.e_part_off = ((size_t)std::max(ngkmax, 1) * sizeof(int) + 255) / 256 * 256;
+  L.e_bytes = L.e_part_off +
+              sizeof(double) * (size_t)((Ktot + OZ_NCH - 1) / OZ_NCH) * std::max(ngkmax, 1);
   return L;
 }
 
@@ -494,11 +518,17 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
   const long long npairs = nt * (nt + 1) / 2;
   const int nslab = L.Kpad / OZ_KS;
   if (stage_mask & 1) {
-    hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(ngp, 256)), dim3(256), 0, s, ngp, INT_MIN, ex);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(ngp, 256)), dim3(256), 0, s, ngp, INT_MIN, eb);
-    dim3 ge(cdiv(ngp, 256), cdiv(K, 256));
-    hipLaunchKernelGGL(colexp_kernel, ge, dim3(256), 0, s, ngp, K, X, ldx, ex);
-    hipLaunchKernelGGL(colexp_kernel, ge, dim3(256), 0, s, ngp, K, B, ldb, eb);
+    const int nch = (K + OZ_NCH - 1) / OZ_NCH;
+    const int ldp = std::max(c->ngkmax, 1);
+    double *px = (double *)((char *)ex + L.e_part_off);
+    double *pb = (double *)((char *)eb + L.e_part_off);
+    dim3 ge(cdiv(ngp, 256), nch);
+    hipLaunchKernelGGL(colnorm_part_kernel, ge, dim3(256), 0, s, ngp, K, X, ldx, px, ldp);
+    hipLaunchKernelGGL(colnorm_part_kernel, ge, dim3(256), 0, s, ngp, K, B, ldb, pb, ldp);
+    hipLaunchKernelGGL(colnorm_final_kernel, dim3(cdiv(ngp, 256)), dim3(256), 0, s, ngp, nch,
+                       px, ldp, ex);
+    hipLaunchKernelGGL(colnorm_final_kernel, dim3(cdiv(ngp, 256)), dim3(256), 0, s, ngp, nch,
+                       pb, ldp, eb);
     dim3 gr(cdiv(L.Gpad, 256), L.Kpad / 16);
     hipLaunchKernelGGL(residue_kernel<4>, gr, dim3(256), 0, s, ngp, K, L.Gpad, X, ldx, ex, bits,
                        nmod, L.qstride, QX);

@@ -1,8 +1,8 @@
 """Choice of the split-precision modulus count for a structure.
 
-The split path (EXH_SPLIT(n)) rounds A and aa.A to b(n) bits per G column,
-where b grows by about 3.9 bits per modulus; that rounding is its only
-error.  How much eigenvalue error a given b causes depends on the structure
+The split path (EXH_SPLIT(n)) scales every G column of A and aa.A to a
+2-norm of 2^c(n) and rounds it to integers, where c grows by about 3.9 bits
+per modulus; that rounding is its only error.  How much eigenvalue error a given b causes depends on the structure
 (basis size, number of atoms, conditioning of O), so the count is picked by
 measurement: on one probe k-point, the lowest states of H x = e O x from the
 split setup are compared with the FP64 setup, both solved in FP64 on the

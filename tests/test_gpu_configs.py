@@ -153,14 +153,16 @@ def test_ortho16_fp64_and_split_against_the_oracle(ortho16, ik):
     log(f"  ortho16 k{ik} batch: H {eh:.1e} O {eo:.1e}")
     assert eh <= TOL_MAT and eo <= TOL_MAT
     assert torch.equal(hb, H) and torch.equal(ob, O)
-    # split precision: eigenvalues against the oracle's H and O
-    Hs, Os = device_setup(c.ctx, d, lapw.SPLIT(NMOD))
+    # split precision (7 moduli, the tuner's choice, and 8): eigenvalues
+    # against the oracle's H and O
     w0 = solve.geneig(H0d, O0d, d.rank, NSTATES).cpu().numpy()
-    w = solve.geneig(Hs, Os, d.rank, NSTATES).cpu().numpy()
-    de = float(np.abs(w - w0).max())
-    log(f"  ortho16 k{ik} split({NMOD}): H {drel(Hs, H0d):.1e} O {drel(Os, O0d):.1e} "
-        f"lowest {NSTATES} max|dE| {de:.2e} Ha")
-    assert de <= TOL_SPLIT_HA
+    for nmod in (7, NMOD):
+        Hs, Os = device_setup(c.ctx, d, lapw.SPLIT(nmod))
+        w = solve.geneig(Hs, Os, d.rank, NSTATES).cpu().numpy()
+        de = float(np.abs(w - w0).max())
+        log(f"  ortho16 k{ik} split({nmod}): H {drel(Hs, H0d):.1e} O {drel(Os, O0d):.1e} "
+            f"lowest {NSTATES} max|dE| {de:.2e} Ha")
+        assert de <= TOL_SPLIT_HA
 
 
 @pytest.mark.gpu
@@ -237,13 +239,15 @@ def test_sc64_fp64_against_the_oracle(sc64):
 
 
 @pytest.mark.gpu
-def test_sc64_split_eigenvalues_against_the_oracle(sc64):
+@pytest.mark.parametrize("nmod", [7, NMOD])
+def test_sc64_split_eigenvalues_against_the_oracle(sc64, nmod):
+    """7 moduli: the count the precision tuner picks for this cell."""
     s, ctx, ref = sc64
     d = lapw.DeviceKPoint(s, 0)
-    H, O = device_setup(ctx, d, lapw.SPLIT(NMOD))
+    H, O = device_setup(ctx, d, lapw.SPLIT(nmod))
     w0 = solve.geneig(ref["H"], ref["O"], d.rank, NSTATES).cpu().numpy()
     w = solve.geneig(H, O, d.rank, NSTATES).cpu().numpy()
     de = float(np.abs(w - w0).max())
-    log(f"  sc64 k0 split({NMOD}): H {drel(H, ref['H']):.1e} O {drel(O, ref['O']):.1e} "
+    log(f"  sc64 k0 split({nmod}): H {drel(H, ref['H']):.1e} O {drel(O, ref['O']):.1e} "
         f"lowest {NSTATES} max|dE| {de:.2e} Ha")
     assert de <= TOL_SPLIT_HA

@@ -1,8 +1,9 @@
 """Split-precision (int8 Ozaki scheme II) H/O setup on the GPU.
 
 1. Exactness of the modular machinery: the split result must equal the FP64
-   oracle plus the effect of rounding A and aa.A to b-bit integers per G
-   column, computed here independently in NumPy.  Any error in the residues,
+   oracle plus the effect of rounding A and aa.A to integers after scaling
+   each G column to a 2-norm below 2^(c-1), computed here independently in
+   NumPy.  Any error in the residues,
    the int8 GEMMs, the Karatsuba recombination or the Garner reconstruction
    shows up far above the 1e-12 tolerance.
 2. Accuracy against FP64: relative Frobenius error of H and O, and the
@@ -22,8 +23,10 @@ PRIMES = [251, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191]
 
 
 def oz_bits(nmod, K):
+    """c: the largest value with 2^(2c) < prod(p)/2 (Cauchy-Schwarz bound on
+    Re, Im of x^H b for columns of 2-norm below 2^c)."""
     lm = sum(math.log2(p) for p in PRIMES[:nmod])
-    return int(min(30, math.floor((lm - 1.0 - math.log2(6.0 * K)) * 0.5)))
+    return int(min(50, math.floor((lm - 1.0) * 0.5 - 1e-9)))
 
 
 def rel(a, b):
@@ -31,11 +34,12 @@ def rel(a, b):
 
 
 def trunc(M, bits):
-    """Round the K x G matrix M to integers with a power-of-two scale per
-    column g so that max |Re|,|Im| <= 2^bits; returns (M_int, 2^-s)."""
-    mx = np.maximum(np.abs(M.real).max(axis=0), np.abs(M.imag).max(axis=0))
-    e = np.frexp(mx)[1]
-    s = np.where(mx > 0, bits - e, 0)
+    """Round the K x G matrix M to integers with a power-of-two scale 2^s per
+    column g, s = bits - e - 1 with ||M[:, g]|| < 2^e; returns
+    (M_int, 2^-s)."""
+    nrm = np.linalg.norm(M, axis=0)
+    e = np.frexp(nrm)[1] + 1
+    s = np.where(nrm > 0, bits - e, 0)
     sc = np.ldexp(1.0, s)
     return np.rint(M.real * sc) + 1j * np.rint(M.imag * sc), 1.0 / sc
 
