@@ -255,11 +255,11 @@ class Workload:
                                          for k, v in prof.items() if v[1]},
                 "_prof": prof, "_steps": steps}
 
-    def ho_roofline(self, r, kernel, profile_md):
+    def ho_roofline(self, r, kernel, profile_md, per="kpoint"):
         ms, n = r["_prof"][kernel]
         achieved = r["_steps"] * sum(self.f_ho) / (ms * 1e-3) / 1e12
         kpl = r["_steps"] * self.nk / n           # k-points per launch
-        per_k = measured_traffic(profile_md) if profile_md else None
+        per_k = measured_traffic(profile_md, per) if profile_md else None
         return {"kernel": f"{kernel} (fused upper-triangle H/O contraction + interstitial)",
                 "launches": n, "kpoints_per_launch": kpl, "avg_launch_ms": ms / n,
                 "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
@@ -787,7 +787,10 @@ def main():
     fp64["roofline"] = W.ho_roofline(fp64, "ho_grouped", f"ho_grouped_{W.spec.name}_fp64.md")
     fp64["plan"] = dict(lapw.plan_info(plan), id=plan)
     if fp64_loop:
-        fp64_loop["roofline"] = W.ho_roofline(fp64_loop, "ho", None)
+        # the single-k kernel's traffic: its own row in the same summary
+        fp64_loop["roofline"] = W.ho_roofline(fp64_loop, "ho",
+                                              f"ho_grouped_{W.spec.name}_fp64.md",
+                                              per="kpoint_single")
 
     head = fp64
     split = None
@@ -886,7 +889,7 @@ def main():
             "fp64_graph": {k: ug[k] for k in ("value", "ms_per_step")},
             "fp64_loop": {k: ul[k] for k in ("value", "ms_per_step", "gflops", "frac_fp64_peak")},
             "batch_vs_loop": ub["value"] / ul["value"],
-            "ho_roofline": U.ho_roofline(ub, "ho_grouped", None),
+            "ho_roofline": U.ho_roofline(ub, "ho_grouped", f"ho_grouped_{U.spec.name}_fp64.md"),
             "split_batch": {k: us[k] for k in ("value", "ms_per_step")},
             "split_moduli": um,
             "split_moduli_tuning": um_tuning,
