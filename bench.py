@@ -113,11 +113,79 @@ def accuracy(ctx, kp, dk, H, O, nmod, nst, kpoint=0):
     w64 = solve.geneig(H64, O64, n, nst).cpu().numpy()
     ws = solve.geneig(H, O, n, nst).cpu().numpy()
     de = float(np.abs(ws - w64).max())
+    eig_s = time.perf_counter() - t0
+    vec = eigvec_timing(H64, O64, n, sorted({40, nst}))
     del H64, O64
     return {"kpoint": kpoint, "n": n, "nstates": nst, "rel_frob_H": rh, "rel_frob_O": ro,
             "max_abs_dE_Ha": de, "within_1uHa": bool(de <= 1e-6),
             "solver": "GPU FP64: Cholesky of O, two triangular solves, Hermitian eigensolve",
-            "eig_seconds": time.perf_counter() - t0}
+            "eig_seconds": eig_s, "eigenvectors": vec}
+
+
+def eigvec_timing(H, O, n, nsts):
+    """solvewithlapack (eigenvalues and eigenvectors) on one FP64 H/O pair:
+    seconds of the full Hermitian eigensolve and of the Chebyshev-filtered
+    subspace iteration, and how far apart their eigenvalues are."""
+    import torch
+    from exciting_amd import solve
+    out = []
+    for nv in nsts:
+        row = {"nstates": nv}
+        w = {}
+        for method in ("full", "subspace"):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            w[method], X = solve.solvewithlapack(H, O, n, nv, method=method)
+            torch.cuda.synchronize()
+            row[f"{method}_s"] = round(time.perf_counter() - t, 4)
+            del X
+        row["subspace_steps"] = solve.last_subspace.get("steps")
+        row["subspace_fallback"] = solve.last_subspace.get("fallback")
+        row["max_abs_dE_Ha"] = float((w["full"] - w["subspace"]).abs().max())
+        out.append(row)
+    log(f"eigenvectors: {out}")
+    return out
+
+
+def host_pointer_sample(ctx, s, ik: int = 0, reps: int = 2):
+    """The Fortran drop-in's rate: exh_match, then
+    exh_hamiltonandoverlapsetup on pageable host arrays, as the k loop calls
+    match and hamiltonandoverlapsetup (seceqn.f90:57-58, seceqnfv.f90:76),
+    with H and O accumulated (uploaded and downloaded) and apwalm downloaded
+    by match; once with every array crossing PCIe and once with apwalm reuse
+    on (the setup reads the copy match left in HBM).  Median seconds per
+    k-point over reps calls after one untimed call."""
+    from exciting_amd import lapw
+    k = s.kpts[ik]
+    lmm = (s.lmaxapw + 1) ** 2
+    n = k.ngk + s.nlotot
+    apwalm = np.zeros((s.ngkmax, s.apwordmax, lmm, s.natmtot), np.complex128, order="F")
+    H = np.zeros((n, n), np.complex128, order="F")
+    O = np.zeros((n, n), np.complex128, order="F")
+    system = lapw.EvSystem(lapw.HermitianMatrix(n, H), lapw.HermitianMatrix(n, O))
+    apw_bytes = 16 * k.ngk * s.apwordmax * lmm * s.natmtot
+    out = {"kpoint": ik, "n": n, "arrays": "NumPy (pageable), Fortran order"}
+    for reuse in (False, True):
+        ctx.set_apwalm_reuse(reuse)
+        u0, r0 = ctx.apwalm_transfers()
+        ts = []
+        for _ in range(reps + 1):
+            H.fill(0)
+            O.fill(0)
+            t = time.perf_counter()
+            ctx.match(k.ngk, k.gpc, k.tpgpc, k.sfacgp, apwalm)
+            ctx.hamiltonandoverlapsetup(system, k.ngk, apwalm, k.igpig, k.vgpc)
+            ts.append(time.perf_counter() - t)
+        u1, r1 = ctx.apwalm_transfers()
+        sec = float(np.median(ts[1:]))
+        pcie = apw_bytes * (1 if reuse else 2) + 4 * 16 * n * n
+        out["reuse" if reuse else "no_reuse"] = {
+            "s_per_kpoint": round(sec, 4), "setups_per_s": round(1.0 / sec, 3),
+            "pcie_bytes": pcie, "pcie_GBps": round(pcie / sec / 1e9, 1),
+            "apwalm_uploads": u1 - u0, "apwalm_reused": r1 - r0}
+    ctx.set_apwalm_reuse(False)
+    log(f"host-pointer drop-in: {out}")
+    return out
 
 
 def cpu_baseline(s, ik: int):
@@ -725,6 +793,9 @@ def main():
                     help="time kinetic_energy on k-point 0 ('only': nothing else)")
     ap.add_argument("--forces", choices=("on", "off", "only"), default="on",
                     help="time forcek on k-point 0 ('only': nothing else)")
+    ap.add_argument("--host-pointer", choices=("on", "off"), default="on",
+                    help="time the host-array drop-in (exh_match + exh_hamiltonandoverlapsetup "
+                         "on NumPy arrays) on k-point 0, with and without apwalm reuse")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, run one device-to-device copy of this "
                          "many MiB (a known-size kernel for calibrating HBM byte counters)")
@@ -867,6 +938,8 @@ def main():
     if args.forces == "on":
         out["forces"] = forces_sample(W, cpu=args.cpu_baseline == "on")
         log(f"forces: {out['forces']}")
+    if args.host_pointer == "on":
+        out["host_pointer"] = host_pointer_sample(W.ctx, s, 0)
     W.free()
 
     if args.uneven == "on":

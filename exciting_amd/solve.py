@@ -75,36 +75,95 @@ def _chebyshev_filter(C, X, degree: int, a: float, b: float, a0: float):
     return Y
 
 
+def _orthonormalize(X):
+    """An orthonormal basis of X's columns: Cholesky QR twice (two Gram
+    matrices and triangular solves, far cheaper than Householder QR for a
+    block of hundreds of columns), accepted only when the first pass left the
+    Gram matrix within 0.5 of the identity (Frobenius), which bounds the
+    second pass's loss of orthogonality at a few units of rounding (Yamamoto
+    et al., ETNA 44 (2015) 306).  Otherwise, or when a Cholesky factorisation
+    fails on a nearly rank-deficient block, Householder QR."""
+    import torch
+    m = X.shape[1]
+    eye = torch.eye(m, dtype=X.dtype, device=X.device)
+    R, info = torch.linalg.cholesky_ex(X.conj().transpose(0, 1) @ X, upper=True)
+    if int(info) == 0:
+        X1 = torch.linalg.solve_triangular(R, X, upper=True, left=False)
+        G = X1.conj().transpose(0, 1) @ X1
+        if float(torch.linalg.matrix_norm(G - eye)) <= 0.5:
+            R, info = torch.linalg.cholesky_ex(G, upper=True)
+            if int(info) == 0:
+                return torch.linalg.solve_triangular(R, X1, upper=True, left=False)
+    return torch.linalg.qr(X)[0]
+
+
+def block_size(nwant: int) -> int:
+    """Vectors the subspace iteration carries for nwant states: 40 % more,
+    at least 16 more (at n = 10364, 400 states: 560 vectors converge in 5
+    filter steps, 500 in 6-8)."""
+    return max(nwant + 16, (7 * nwant + 4) // 5)
+
+
+def filter_degree(nwant: int) -> int:
+    """Chebyshev degree per filter step: 32 for small blocks (whose
+    orthonormalisation and Rayleigh-Ritz cost little next to the filter), 24
+    above 64 states (measured at n = 10364: 40 states 208 ms at degree 32,
+    240 ms at 16; 400 states 1.09 s at 24, 1.17 s at 32)."""
+    return 32 if nwant <= 64 else 24
+
+
 # Record of the last lowest_eigvalsh call: filter steps, final largest
 # residual norm, the tolerance it met, and whether it fell back to the full
 # eigensolve.
 last_subspace: dict = {}
 
 
-def lowest_eigvalsh(C, nwant: int, degree: int = 16, rtol: float = 1e-11, atol: float = 1e-11,
+def lowest_eigvalsh(C, nwant: int, degree: int = 0, rtol: float = 1e-11, atol: float = 1e-11,
                     maxit: int = 40, seed: int = 1234):
     """The nwant lowest eigenvalues of the Hermitian matrix C (n x n CUDA
     tensor), as ZHEGVX's range 'I' asks, by Chebyshev-filtered subspace
-    iteration: a block of nwant + 16 (at least 1.25 nwant) vectors, filtered
-    with the polynomial that damps everything above the block's largest Ritz
-    value, orthonormalised and Rayleigh-Ritz projected until every wanted
-    Ritz pair has residual norm <= min(rtol * max|spectrum|, atol).  The
+    iteration: a block of block_size(nwant) vectors, filtered with the
+    polynomial of degree filter_degree(nwant) (or ``degree``) that damps
+    everything above the block's largest Ritz value, orthonormalised and
+    Rayleigh-Ritz projected until every wanted Ritz pair has residual norm
+    <= max(min(rtol * max|spectrum|, atol), 64 eps max|spectrum|).  The
     residual bounds each eigenvalue's error (Weyl), so atol (Ha) caps the
     error however large the spectrum of C grows with an ill-conditioned
-    overlap.  Small problems, and a run that does not meet the tolerance in
-    maxit filter steps, use the full Hermitian eigensolve instead.
-    Deterministic: the start block comes from a seeded generator."""
+    overlap; the last term is the rounding floor of any FP64 eigensolver on
+    C, ZHEGVX's included.  Small problems, and a run that does not meet the
+    tolerance in maxit filter steps, use the full Hermitian eigensolve
+    instead.  Deterministic: the start block comes from a seeded
+    generator."""
+    return lowest_eigh(C, nwant, False, degree, rtol, atol, maxit, seed)
+
+
+def lowest_eigh(C, nwant: int, vectors: bool = True, degree: int = 0, rtol: float = 1e-11,
+                atol: float = 1e-11, maxit: int = 40, seed: int = 1234):
+    """lowest_eigvalsh with, when vectors is true, the Ritz vectors of the
+    nwant lowest pairs: an (n, nwant) orthonormal block, columns in the order
+    of the eigenvalues.  A residual r = C x - theta x of norm <= tol puts the
+    vector within tol / gap of the exact eigenvector (the invariant subspace
+    of a cluster, Davis-Kahan), as ZHEGVX's own vectors are determined only
+    up to mixing inside clusters.  Returns w, or (w, X) with vectors."""
     import torch
     n = C.shape[0]
-    m = min(n, max(nwant + 16, (5 * nwant + 3) // 4))
+    m = min(n, block_size(nwant))
+    degree = degree or filter_degree(nwant)
     last_subspace.clear()
+
+    def full():
+        if not vectors:
+            return torch.linalg.eigvalsh(C)[:nwant]
+        w, V = torch.linalg.eigh(C)
+        return w[:nwant], V[:, :nwant]
+
     if n < 512 or 3 * m > n:
         last_subspace.update(steps=0, fallback="small")
-        return torch.linalg.eigvalsh(C)[:nwant]
+        return full()
     b = _lanczos_upper_bound(C)
     g = torch.Generator(device=C.device).manual_seed(seed)
     X = torch.randn(n, m, dtype=C.dtype, device=C.device, generator=g)
-    X, _ = torch.linalg.qr(X)
+    X = _orthonormalize(X)
     res = float("inf")
     for it in range(maxit):
         CX = C @ X
@@ -115,18 +174,18 @@ def lowest_eigvalsh(C, nwant: int, degree: int = 16, rtol: float = 1e-11, atol: 
         CX = CX @ Q
         R = CX[:, :nwant] - X[:, :nwant] * theta[:nwant].to(X.dtype)[None, :]
         scale = max(abs(b), float(theta.abs().max()))
-        tol = min(rtol * scale, atol)
+        tol = max(min(rtol * scale, atol), 64 * 2.0 ** -52 * scale)
         res = float(torch.linalg.vector_norm(R, dim=0).max())
         if res <= tol:
             last_subspace.update(steps=it, residual=res, tol=tol, scale=scale, fallback=None)
-            return theta[:nwant]
+            return (theta[:nwant], X[:, :nwant]) if vectors else theta[:nwant]
         a, a0 = float(theta[-1]), float(theta[0])
         if not a < b:
             break
         X = _chebyshev_filter(C, X, degree, a, b, a0)
-        X, _ = torch.linalg.qr(X)
+        X = _orthonormalize(X)
     last_subspace.update(steps=maxit, residual=res, scale=b, fallback="not converged")
-    return torch.linalg.eigvalsh(C)[:nwant]
+    return full()
 
 
 def geneig(H, O, n: int, nstfv: int, vectors: bool = False, method: str = "full"):
@@ -138,10 +197,11 @@ def geneig(H, O, n: int, nstfv: int, vectors: bool = False, method: str = "full"
     eigenvectors as an (n, nstfv) tensor (columns are states, normalised so
     that x^H O x = 1, ZHEGVX's convention).  Raises RuntimeError if O is not
     positive definite (modfvsystem.f90:698-711 stops in that case).
-    method (eigenvalues only): "full" is the full Hermitian eigensolve,
-    "subspace" lowest_eigvalsh, "auto" the latter when the block it iterates
-    is at most n/24 (nstfv = 40 at n = 10364: 0.44 s instead of 1.87 s; at
-    nstfv = 400 the two take the same time)."""
+    method: "full" is the full Hermitian eigensolve, "subspace" the
+    Chebyshev-filtered subspace iteration (lowest_eigh; with vectors its Ritz
+    vectors), "auto" the latter when the block it iterates is at most n/16
+    (n = 10364: 40 states in 0.21 s instead of 1.65 s for all eigenvalues;
+    400 states in 1.1 s)."""
     import torch
     Ht = H.reshape(-1)[:n * n].view(n, n)   # row-major view = H^T = conj(H)
     Ot = O.reshape(-1)[:n * n].view(n, n)
@@ -153,19 +213,23 @@ def geneig(H, O, n: int, nstfv: int, vectors: bool = False, method: str = "full"
     Y = torch.linalg.solve_triangular(L, Ht, upper=False)
     C = torch.linalg.solve_triangular(L, Y.conj().transpose(0, 1), upper=False)
     C = 0.5 * (C + C.conj().transpose(0, 1))
+    few = method == "subspace" or (method == "auto" and 16 * block_size(nstfv) <= n)
     if not vectors:
-        block = max(nstfv + 16, (5 * nstfv + 3) // 4)
-        if method == "subspace" or (method == "auto" and 24 * block <= n):
+        if few:
             return lowest_eigvalsh(C, nstfv)
         return torch.linalg.eigvalsh(C)[:nstfv]
-    w, V = torch.linalg.eigh(C)
-    X = torch.linalg.solve_triangular(L.conj().transpose(0, 1), V[:, :nstfv], upper=True)
+    if few:
+        w, V = lowest_eigh(C, nstfv, True)
+    else:
+        w, V = torch.linalg.eigh(C)
+        w, V = w[:nstfv], V[:, :nstfv]
+    X = torch.linalg.solve_triangular(L.conj().transpose(0, 1), V, upper=True)
     # eigenvectors of the transposed problem are the conjugates of the
     # original ones
     return w[:nstfv], X.conj()
 
 
-def solvewithlapack(H, O, n: int, nstfv: int, nmatmax: int = 0):
+def solvewithlapack(H, O, n: int, nstfv: int, nmatmax: int = 0, method: str = "full"):
     """modfvsystem.f90:585-720 on the GPU: the lowest nstfv eigenpairs of
     H x = e O x with ZHEGVX(1, 'V', 'I', 'U') semantics.
 
@@ -175,10 +239,12 @@ def solvewithlapack(H, O, n: int, nstfv: int, nmatmax: int = 0):
     nmatmax zero).  Normalisation x^H O x = 1 as ZHEGVX's; the phase of each
     vector is the solver's own (the reference's consumers only use
     phase-invariant products).  Raises RuntimeError where ZHEGVX returns
-    INFO > n (O not positive definite; the reference stops)."""
+    INFO > n (O not positive definite; the reference stops).  method as in
+    geneig ("auto": the subspace iteration for few states of a large
+    matrix)."""
     import torch
     nmatmax = max(nmatmax, n)
-    w, X = geneig(H, O, n, nstfv, vectors=True)
+    w, X = geneig(H, O, n, nstfv, vectors=True, method=method)
     evec = torch.zeros((nstfv, nmatmax), dtype=torch.complex128, device=X.device)
     evec[:, :n] = X.transpose(0, 1)
     return w, evec
