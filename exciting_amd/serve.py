@@ -602,6 +602,70 @@ def summarize(reqs: List[Request], wall: float) -> dict:
     }
 
 
+_COUNTERS = ("contexts_created", "contexts_evicted", "bypassed", "hbm_waits")
+
+
+def record_line(r: Request) -> str:
+    """One request as a JSON line: what summarize() needs to pool several
+    segments of a stream."""
+    return json.dumps({"rid": r.rid, "cell": r.cell, "nk": len(r.kpoints), "nmod": r.nmod,
+                       "arrival": r.arrival, "admitted": r.admitted, "finished": r.finished,
+                       "status": r.status, "retries_setup": r.retries_setup,
+                       "retries_solve": r.retries_solve, "launches": r.launches}) + "\n"
+
+
+def write_records(path: str, reqs: List[Request], head: dict) -> None:
+    """One line per request, then the header line (segment, wall time,
+    server counters: the object that has a "segment" key)."""
+    with open(path, "w") as f:
+        for r in reqs:
+            f.write(record_line(r))
+        f.write(json.dumps(head) + "\n")
+
+
+def merge_records(paths: List[str]) -> dict:
+    """Pool the record files of consecutive segments of one stream: latency
+    percentiles over every request, setups/s over the summed segment wall
+    times, server counters summed, peaks maximised."""
+    heads, reqs = [], []
+    for path in paths:
+        with open(path) as f:
+            lines = [json.loads(x) for x in f if x.strip()]
+        head = [d for d in lines if "segment" in d]
+        if len(head) != 1:
+            raise ValueError(f"{path}: no header line (the segment did not finish)")
+        heads.append(head[0])
+        for d in lines:
+            if "segment" in d:
+                continue
+            r = Request(rid=d["rid"], cell=d["cell"], kpoints=list(range(d["nk"])),
+                        nmod=d["nmod"], arrival=d["arrival"])
+            r.admitted, r.finished, r.status = d["admitted"], d["finished"], d["status"]
+            r.retries_setup, r.retries_solve = d["retries_setup"], d["retries_solve"]
+            r.launches = d["launches"]
+            reqs.append(r)
+    if len({r.rid for r in reqs}) != len(reqs):
+        raise ValueError("segments overlap: a request id appears twice")
+    out = summarize(reqs, sum(h["wall_s"] for h in heads))
+    out["segments"] = [h["segment"] for h in heads]
+    for k in _COUNTERS:
+        out[k] = sum(h[k] for h in heads)
+    out["peak_inflight"] = max(h["peak_inflight"] for h in heads)
+    out["peak_context_gib"] = max(h["peak_context_gib"] for h in heads)
+    out["setup_launches"] = sum(h["batching"]["launches"] for h in heads)
+    out["kpoints_per_launch"] = out["setups"] / max(out["setup_launches"], 1)
+    out["max_kpoints_per_launch"] = max(h["batching"]["kpoints_per_launch"]["max"]
+                                        for h in heads)
+    out["max_cells_per_launch"] = max(h["batching"]["cells_per_launch"]["max"] for h in heads)
+    by_cell = {}
+    for r in reqs:
+        if r.status == "done":
+            by_cell.setdefault(r.cell, []).append(r.latency)
+    out["latency_p50_by_cell_s"] = {c: float(np.percentile(v, 50))
+                                    for c, v in sorted(by_cell.items())}
+    return out
+
+
 def main(argv=None) -> int:
     """Serve a Poisson request stream over the 2/8/16/32/64-atom cells on one
     GPU and print one JSON line with p50/p99 latency and setups/s."""
@@ -622,7 +686,19 @@ def main(argv=None) -> int:
                          "allocator working room)")
     ap.add_argument("--cells", default="fcc2,sc8,ortho16,sc32,sc64")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--segment", default="",
+                    help="START:STOP: serve only these requests of the stream (arrival "
+                         "times shifted so the first arrives at 0, starting from an empty "
+                         "server)")
+    ap.add_argument("--records", default="",
+                    help="write a header and one JSON line per request to this file")
+    ap.add_argument("--merge", nargs="+", default=None, metavar="RECORDS",
+                    help="pool the record files of several segments, print the summary "
+                         "and exit (no GPU)")
     args = ap.parse_args(argv)
+    if args.merge:
+        print(json.dumps(merge_records(args.merge)), flush=True)
+        return 0
     from . import solve as _solve
     _solve.configure()
     from . import synth
@@ -639,9 +715,16 @@ def main(argv=None) -> int:
     make, close, execute = ex.make_context, ex.close_context, ex.execute
     budget = (free_hbm(0) - int(args.reserve_gib * 2**30)
               - args.max_solves * _solve.WORKSPACE_BYTES)   # the solvers' hipBLAS workspaces
+    rec = open(args.records, "w") if args.records else None
+    rec_lock = threading.Lock()
+
     def report(r):
         print(f"[serve] request {r.rid} {r.cell} nk={len(r.kpoints)} nmod={r.nmod} "
               f"{r.status} latency {r.latency:.3f} s {r.error}", file=sys.stderr, flush=True)
+        if rec is not None and r.rid >= 0:
+            with rec_lock:
+                rec.write(record_line(r))
+                rec.flush()
 
     srv = Server(cells, execute, make, close, budget, max_inflight=args.max_inflight,
                  on_finish=report)
@@ -657,6 +740,16 @@ def main(argv=None) -> int:
     reqs = poisson_requests(args.requests, args.rate, cells, seed=args.seed, kmax=args.kmax,
                             split_fraction=args.split_fraction, nmod=args.nmod,
                             nstfv=args.nstfv)
+    segment = [0, len(reqs)]
+    if args.segment:
+        a, b = (int(x) for x in args.segment.split(":"))
+        if not 0 <= a < b <= len(reqs):
+            raise SystemExit(f"--segment {args.segment} outside the {len(reqs)} requests")
+        segment = [a, b]
+        reqs = reqs[a:b]
+        t0 = reqs[0].arrival
+        for r in reqs:
+            r.arrival -= t0
     t = time.perf_counter()
     srv.run(reqs)
     wall = time.perf_counter() - t
@@ -671,7 +764,11 @@ def main(argv=None) -> int:
                 "peak_inflight": srv.peak_inflight, "contexts_created": srv.pool.created,
                 "contexts_evicted": srv.pool.evicted, "bypassed": srv.bypassed,
                 "hbm_waits": srv.hbm_waits,
-                "peak_context_gib": srv.peak_bytes / 2**30, "batching": ex.stats()})
+                "peak_context_gib": srv.peak_bytes / 2**30, "batching": ex.stats(),
+                "segment": segment})
+    if rec is not None:
+        rec.write(json.dumps(out) + "\n")
+        rec.close()
     print(json.dumps(out), flush=True)
     srv.pool.close_all()
     ex.close()

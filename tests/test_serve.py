@@ -208,3 +208,36 @@ def test_serving_cells_span_two_to_sixty_four_atoms():
     natoms = sorted(sum(len(p) for p in spec.positions) for spec, _ in specs.values())
     assert natoms == [2, 8, 16, 32, 64]
     assert all(1 <= len(ks) <= 16 for _, ks in specs.values())
+
+
+def test_segment_records_pool_into_one_summary(tmp_path):
+    """Record files of consecutive segments of one stream merge into the
+    summary of the whole stream: percentiles over every request, setups/s
+    over the summed wall time, counters summed; overlapping segments are
+    refused."""
+    cells = {"a": FakeCell("a", 10, 0.0, nk=16), "b": FakeCell("b", 10, 0.0, nk=1)}
+    reqs = serve.poisson_requests(300, rate=50.0, cells=cells, seed=4, kmax=16)
+    for i, r in enumerate(reqs):
+        r.status, r.admitted, r.finished = "done", r.arrival, r.arrival + 0.01 * (i % 100 + 1)
+        r.launches = 1 + i % 3
+    reqs[7].status = "failed"
+    whole = serve.summarize(reqs, wall=12.0)
+    paths = []
+    for j, (a, b, wall) in enumerate(((0, 120, 5.0), (120, 300, 7.0))):
+        head = {"segment": [a, b], "wall_s": wall, "contexts_created": 2,
+                "contexts_evicted": j, "bypassed": 3 * j, "hbm_waits": 1,
+                "peak_inflight": 4 + j, "peak_context_gib": 1.5 - j,
+                "batching": {"launches": 10 + j, "kpoints_per_launch": {"max": 5 + j},
+                             "cells_per_launch": {"max": 2}}}
+        paths.append(str(tmp_path / f"seg{j}.jsonl"))
+        serve.write_records(paths[-1], reqs[a:b], head)
+    m = serve.merge_records(paths)
+    for key in ("requests", "done", "failed", "setups", "setups_per_s", "latency_s",
+                "queue_wait_s", "launches_per_request"):
+        assert m[key] == pytest.approx(whole[key]), key
+    assert m["segments"] == [[0, 120], [120, 300]]
+    assert (m["contexts_evicted"], m["bypassed"], m["hbm_waits"]) == (1, 3, 2)
+    assert m["peak_inflight"] == 5 and m["max_kpoints_per_launch"] == 6
+    assert m["setup_launches"] == 21
+    with pytest.raises(ValueError, match="overlap"):
+        serve.merge_records([paths[0], paths[0]])
