@@ -378,24 +378,19 @@ __device__ __forceinline__ double garner(const int *r) {
 }
 
 
-// ---------------------------------------------------------------------------
-// Residues -> FP64 H and O upper elements (+ interstitial, + mirror).
-// One workgroup per 128x128 tile, walked in 32x32 sub-blocks.  Each thread
-// reconstructs four consecutive rows i of one column j of a sub-block: every
-// (product, modulus) residue of the four is one aligned 4-byte load, and the
-// upper-triangle writes are 64 contiguous bytes per thread.  The sub-block is
-// staged in LDS and written back transposed and conjugated to the lower
-// triangle (coalesced along j).
-// ---------------------------------------------------------------------------
-// x mod p in [0, p) for |x| < 2^23 (exact in float), through the reciprocal
 __device__ __forceinline__ int modp(int x, int k) {
   const int p = c_p[k];
-  int r = x - (int)floorf((float)x * c_rp[k]) * p;
-  r += (r < 0) ? p : 0;
-  r -= (r >= p) ? p : 0;
-  return r;
+  int r = x % p;
+  return r < 0 ? r + p : r;
 }
 
+// ---------------------------------------------------------------------------
+// Residues -> FP64 H and O upper elements (+ interstitial, + mirror).
+// One workgroup per 128x128 tile, walked in 32x32 sub-blocks: each thread
+// reconstructs four elements of a sub-block and writes them to the upper
+// triangle (coalesced along i); the sub-block is staged in LDS and written
+// back transposed and conjugated to the lower triangle (coalesced along j).
+// ---------------------------------------------------------------------------
 template <int NM>
 __global__ __launch_bounds__(256) void ozcombine_kernel(
     int ngp, int npairs, const uint8_t *__restrict__ R, const int *__restrict__ ex,
@@ -410,66 +405,56 @@ __global__ __launch_bounds__(256) void ozcombine_kernel(
   const size_t TT = (size_t)OZ_T * OZ_T;
   __shared__ double2 sH[32][33], sO[32][33];
   const int tid = threadIdx.x, lo = tid & 31, hi8 = tid >> 5;
-  const int i4 = 4 * (tid & 7), jr = tid >> 3;  // rows i4..i4+3 of column jr
-  const uint8_t *Rt = R + (size_t)tile * TT;
   for (int sb = 0; sb < 16; sb++) {
     const int bi = sb & 3, bj = sb >> 2;
     if (I == J && bi > bj) continue;  // wholly below the diagonal (uniform)
-    const int il0 = bi * 32 + i4, jl = bj * 32 + jr;
-    const int i0 = I0 + il0, j = J0 + jl;
-    if (j < ngp && i0 < ngp && i0 <= j) {
-      const int e = il0 + OZ_T * jl;
-      unsigned w[6][NM];
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+      const int il = bi * 32 + lo, jl = bj * 32 + hi8 + 8 * r;
+      const int i = I0 + il, j = J0 + jl;
+      if (i >= ngp || j >= ngp || i > j) continue;
+      const int e = il + OZ_T * jl;
+      int hre[NM], him[NM], ore[NM], oim[NM];
 #pragma unroll
-      for (int k = 0; k < NM; k++)
+      for (int k = 0; k < NM; k++) {
+        int rr[6];
 #pragma unroll
         for (int pr = 0; pr < 6; pr++)
-          w[pr][k] = *(const unsigned *)(Rt + (size_t)(pr * NM + k) * npairs * TT + e);
+          rr[pr] = R[((size_t)(pr * NM + k) * npairs + tile) * TT + e];
+        hre[k] = modp(rr[0] + rr[1], k);
+        him[k] = modp(rr[0] - rr[1] - rr[2], k);
+        ore[k] = modp(rr[3] + rr[4], k);
+        oim[k] = modp(rr[3] - rr[4] - rr[5], k);
+      }
+      const int si = ex[i] == INT_MIN ? 0 : bits - ex[i];
       const int sjb = eb[j] == INT_MIN ? 0 : bits - eb[j];
       const int sjx = ex[j] == INT_MIN ? 0 : bits - ex[j];
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const int i = i0 + t;
-        if (i >= ngp || i > j) continue;
-        int hre[NM], him[NM], ore[NM], oim[NM];
-#pragma unroll
-        for (int k = 0; k < NM; k++) {
-          const int r0 = (w[0][k] >> (8 * t)) & 255, r1 = (w[1][k] >> (8 * t)) & 255;
-          const int r2 = (w[2][k] >> (8 * t)) & 255, r3 = (w[3][k] >> (8 * t)) & 255;
-          const int r4 = (w[4][k] >> (8 * t)) & 255, r5 = (w[5][k] >> (8 * t)) & 255;
-          hre[k] = modp(r0 + r1, k);
-          him[k] = modp(r0 - r1 - r2, k);
-          ore[k] = modp(r3 + r4, k);
-          oim[k] = modp(r3 - r4 - r5, k);
-        }
-        const int si = ex[i] == INT_MIN ? 0 : bits - ex[i];
-        double hr = ldexp(garner<NM>(hre), -(si + sjb));
-        double hv = ldexp(garner<NM>(him), -(si + sjb));
-        double orr = ldexp(garner<NM>(ore), -(si + sjx));
-        double oi = ldexp(garner<NM>(oim), -(si + sjx));
-        const size_t up = (size_t)i + (size_t)j * ldh;
-        if (accumulate) {
-          const double2 h0 = H[up], o0 = O[up];
-          hr += h0.x;
-          hv += h0.y;
-          orr += o0.x;
-          oi += o0.y;
-        }
-        ist_add(box, i, j, hr, hv, orr, oi);
-        H[up] = z2(hr, hv);
-        O[up] = z2(orr, oi);
-        sH[jr][i4 + t] = z2(hr, -hv);
-        sO[jr][i4 + t] = z2(orr, -oi);
+      double hr = ldexp(garner<NM>(hre), -(si + sjb));
+      double hv = ldexp(garner<NM>(him), -(si + sjb));
+      double orr = ldexp(garner<NM>(ore), -(si + sjx));
+      double oi = ldexp(garner<NM>(oim), -(si + sjx));
+      const size_t up = (size_t)i + (size_t)j * ldh;
+      if (accumulate) {
+        const double2 h0 = H[up], o0 = O[up];
+        hr += h0.x;
+        hv += h0.y;
+        orr += o0.x;
+        oi += o0.y;
       }
+      ist_add(box, i, j, hr, hv, orr, oi);
+      H[up] = z2(hr, hv);
+      O[up] = z2(orr, oi);
+      sH[jl - bj * 32][lo] = z2(hr, -hv);
+      sO[jl - bj * 32][lo] = z2(orr, -oi);
     }
     __syncthreads();
     // lower triangle: element (j, i) = conj(upper (i, j)), i < j
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
-      const int jl2 = bj * 32 + lo, il = bi * 32 + hi8 + 8 * r;
-      const int i = I0 + il, j2 = J0 + jl2;
-      if (i >= ngp || j2 >= ngp || i >= j2) continue;
-      const size_t lw = (size_t)j2 + (size_t)i * ldh;
+      const int jl = bj * 32 + lo, il = bi * 32 + hi8 + 8 * r;
+      const int i = I0 + il, j = J0 + jl;
+      if (i >= ngp || j >= ngp || i >= j) continue;
+      const size_t lw = (size_t)j + (size_t)i * ldh;
       H[lw] = sH[lo][hi8 + 8 * r];
       O[lw] = sO[lo][hi8 + 8 * r];
     }
