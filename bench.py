@@ -339,11 +339,15 @@ class Workload:
         K = self.K
         ops = r["_steps"] * sum(6 * nmod * g * (g + 1) / 2 * K * 2.0 for g in self.ngks)
         achieved = ops / (ms * 1e-3) / 1e12
+        # the FP64 work these GEMMs stand in for: the H/O contraction's flops
+        fp64_eq = r["_steps"] * sum(self.f_ho) / (ms * 1e-3) / 1e12
         return {"kernel": f"ozgemm_kernel (int8 MFMA modular GEMMs, {nmod} moduli x 6 "
                           f"products, upper triangle)",
                 "launches": n, "avg_launch_ms": ms / n,
                 "bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                "frac": achieved / INT8_PEAK_TOPS, "traffic": measured_traffic(profile_md)}
+                "frac": achieved / INT8_PEAK_TOPS, "traffic": measured_traffic(profile_md),
+                "fp64_equivalent_tflops": fp64_eq,
+                "fp64_equivalent_vs_fp64_peak": fp64_eq / FP64_PEAK_TFLOPS}
 
 
 def clean(r):

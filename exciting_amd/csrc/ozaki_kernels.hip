@@ -146,30 +146,29 @@ __global__ void residue_kernel(int ngp, int Ktot, int Gpad, const double2 *__res
 }
 
 // ---------------------------------------------------------------------------
-// Upper-triangular int8 GEMMs, one (product pair, modulus, tile) per
-// workgroup.  The H product P_q and the O product Q_q (q = 0, 1, 2) share the
-// row operand (xr, xi or xr+xi), as the FP64 kernel shares conj(A) between H
-// and O, so a workgroup streams three operand panels for two products:
+// Upper-triangular int8 GEMMs.  One workgroup computes the product pair
+// (H_q, O_q) of one modulus (q = 0, 1, 2) for a row panel of the row operand
+// (xr, xi or xr+xi) against one 128-column block J of the H and O column
+// operands:
 //   C_H[i][j] = sum_k A[k][i] BH[k][j],  C_O[i][j] = sum_k A[k][i] BO[k][j]
-// (int32, exact), each stored mod p as bytes.  128x128 tiles, 4 waves of
-// 64x64 (2x2 blocks of v_mfma_i32_32x32x32_i8 per product).
+// (int32, exact), each stored mod p as bytes in the 128x128 tiles on or
+// above the diagonal.  The H and O products share the row operand, as the
+// FP64 kernel shares conj(A) between H and O.  Each wave owns 96 rows x 64
+// columns of both products (3 x 2 blocks of v_mfma_i32_32x32x32_i8 each, 192
+// accumulator registers) and reads 7 operand fragments from LDS per 12
+// MFMAs; NW waves form NW/2 row groups x 2 column groups, so the panel is
+// 48*NW rows.  NW = 8: 384-row panels, one workgroup per CU.  (NW = 4, two
+// 192-row workgroups per CU whose barriers fall at different times, gives
+// the same products and measured 1 % slower at the 64-atom shape.)
+// Operands stream global -> LDS directly (global_load_lds, 16 bytes per
+// lane, one 64-row chunk of one k block per instruction), two stages of 64 k
+// double-buffered.
 // Operand lane layout: row/column = lane%32, 16 k bytes from k block
 // 2*ks + lane/32; result D[8*(r/4) + 4*(lane/32) + r%4][lane%32].
 // ---------------------------------------------------------------------------
-// One workgroup computes the product pair (H_q, O_q) of one modulus for a
-// panel of OZ_P = 3 vertically stacked 128x128 tiles (I = 3*IT .. 3*IT+2,
-// one column block J): a 384-row panel of the row operand against
-// 128-column panels of the H and O column operands, which the three tiles
-// share.  8 waves as 4 row groups x 2 column groups; each wave owns 96 rows
-// x 64 columns of both products (3 x 2 blocks of v_mfma_i32_32x32x32_i8
-// each, 192 accumulator registers), which reads 7 operand fragments from LDS
-// per 12 MFMAs.  Operands stream global -> LDS directly (global_load_lds,
-// 16 bytes per lane, one 64-row chunk of one k block per instruction), two
-// stages of 64 k double-buffered.  Panels walk J-major: the panels IT =
-// 0..J/3 of column block J are consecutive, so co-resident workgroups share
-// the column panels in L2.
-constexpr int OZ_P = 3;
-constexpr int OZ_R = OZ_P * OZ_T;  // rows of the row panel per workgroup
+constexpr int OZ_WR = 3;           // 32-row blocks per wave (96 rows)
+constexpr int OZ_NWG = 8;          // waves per workgroup
+constexpr int OZ_R = 48 * OZ_NWG;  // rows of the row panel per workgroup
 
 #define EXH_LDS(p) ((__attribute__((address_space(3))) void *)(p))
 #define EXH_GLB(p) ((const __attribute__((address_space(1))) void *)(p))
@@ -177,13 +176,13 @@ constexpr int OZ_R = OZ_P * OZ_T;  // rows of the row panel per workgroup
 // Residues mod p of one 32 x 64 strip (block row a of a wave) per product,
 // four consecutive rows packed per 32-bit store, into the 128x128 tile it
 // lies in when that tile is on or above the diagonal.
-__device__ __forceinline__ void store_residues(const v16i (&acc)[OZ_P][2], int p, float rp,
+__device__ __forceinline__ void store_residues(const v16i (&acc)[OZ_WR][2], int p, float rp,
                                                int rg, int wc, int lh, int l32, int IT, int J,
                                                uint8_t *Rp) {
 #pragma unroll
-  for (int a = 0; a < OZ_P; a++) {
-    const int r0 = rg * 32 * OZ_P + a * 32;
-    const int I = OZ_P * IT + (r0 >> 7);
+  for (int a = 0; a < OZ_WR; a++) {
+    const int row = IT * OZ_R + rg * 32 * OZ_WR + a * 32;
+    const int I = row >> 7;
     if (I > J) continue;  // below the diagonal (uniform per wave)
     uint8_t *out = Rp + ((long long)J * (J + 1) / 2 + I) * (OZ_T * OZ_T);
 #pragma unroll
@@ -200,34 +199,37 @@ __device__ __forceinline__ void store_residues(const v16i (&acc)[OZ_P][2], int p
           r -= (r >= p) ? p : 0;
           packed |= (unsigned)r << (8 * t4);
         }
-        const int il = (r0 & 127) + 8 * h + 4 * lh;
+        const int il = (row & 127) + 8 * h + 4 * lh;
         const int jl = wc * 64 + b * 32 + l32;
         *(unsigned *)(out + il + OZ_T * jl) = packed;
       }
   }
 }
 
-// Panels per GEMM: column block J has J/3 + 1 of them; blocks 3c .. 3c+2
-// together hold 3(c+1), starting at 3c(c+1)/2.
+// Row panels column block J needs: those starting at or above its last row.
+__host__ __device__ inline int oz_npan(int j) { return (j * OZ_T + OZ_T - 1) / OZ_R + 1; }
+
+// Panels per GEMM.
 __host__ __device__ inline long long oz_panels(int nt) {
-  const long long C = nt / OZ_P;
-  return OZ_P * C * (C + 1) / 2 + (long long)(nt - OZ_P * C) * (C + 1);
+  long long n = 0;
+  for (int j = 0; j < nt; j++) n += oz_npan(j);
+  return n;
 }
 
 // Panel order: super-columns of OZ_SJ column blocks, each walked in chunks
-// of OZ_SI row panels, so that the 32 workgroups an XCD runs at once share
-// about 4 row panels and 8 column panels (the closed-form column-major order
-// had them share about 1 column panel and 19 row panels); 3.6 % faster at the
-// 64-atom shape, bitwise the same products.
-constexpr int OZ_SJ = 8, OZ_SI = 4;
+// of OZ_SI row panels (4 x 384 rows), so that the workgroups an XCD runs at
+// once share a few row panels and column panels in its L2 (the column-major
+// order had 32 co-resident workgroups share about 1 column panel and 19 row
+// panels); 3.6 % faster at the 64-atom shape, bitwise the same products.
+constexpr int OZ_SJ = 8, OZ_SI = 4 * 384 / OZ_R;
 
-// Panel o of one GEMM -> (column block J, row panel IT), IT <= J / OZ_P.
+// Panel o of one GEMM -> (column block J, row panel IT), IT < oz_npan(J).
 __device__ __forceinline__ void oz_panel_at(int o, int nt, int &J, int &IT) {
   int rem = o, j0 = 0;
   for (;; j0 += OZ_SJ) {
     const int j1 = min(nt, j0 + OZ_SJ);
     int cnt = 0;
-    for (int j = j0; j < j1; j++) cnt += j / OZ_P + 1;
+    for (int j = j0; j < j1; j++) cnt += oz_npan(j);
     if (rem < cnt) break;
     rem -= cnt;
   }
@@ -235,14 +237,14 @@ __device__ __forceinline__ void oz_panel_at(int o, int nt, int &J, int &IT) {
   int i0 = 0;
   for (;; i0 += OZ_SI) {
     int cnt = 0;
-    for (int j = j0; j < j1; j++) cnt += max(0, min(OZ_SI, j / OZ_P + 1 - i0));
+    for (int j = j0; j < j1; j++) cnt += max(0, min(OZ_SI, oz_npan(j) - i0));
     if (rem < cnt) break;
     rem -= cnt;
   }
   J = j0;
   IT = i0;
   for (int j = j0; j < j1; j++) {
-    const int cj = max(0, min(OZ_SI, j / OZ_P + 1 - i0));
+    const int cj = max(0, min(OZ_SI, oz_npan(j) - i0));
     if (rem < cj) {
       J = j;
       IT = i0 + rem;
@@ -252,10 +254,11 @@ __device__ __forceinline__ void oz_panel_at(int o, int nt, int &J, int &IT) {
   }
 }
 
-__global__ __launch_bounds__(512) void ozgemm_kernel(int nt, int nslab, int Gpad, int nmod,
-                                                     const int8_t *__restrict__ QX,
-                                                     const int8_t *__restrict__ QB,
-                                                     size_t qstride, uint8_t *__restrict__ R) {
+// two waves per SIMD whatever the workgroup size: at most 256 VGPRs
+__global__ __launch_bounds__(64 * OZ_NWG) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+ozgemm_kernel(
+    int nt, int nslab, int Gpad, int nmod, const int8_t *__restrict__ QX,
+    const int8_t *__restrict__ QB, size_t qstride, uint8_t *__restrict__ R) {
   // XCD-contiguous remap over the whole grid, then (pair*nmod+m, panel)
   const int bid = blockIdx.x, nb = gridDim.x;
   const int xcd = bid & 7, i8 = bid >> 3, q8 = nb >> 3, r8 = nb & 7;
@@ -279,48 +282,52 @@ __global__ __launch_bounds__(512) void ozgemm_kernel(int nt, int nslab, int Gpad
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int rg = w >> 1, wc = w & 1, l32 = lane & 31, lh = lane >> 5;
-  v16i ah[OZ_P][2], ao[OZ_P][2];
+  v16i ah[OZ_WR][2], ao[OZ_WR][2];
 #pragma unroll
-  for (int a = 0; a < OZ_P; a++)
+  for (int a = 0; a < OZ_WR; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) {
       ah[a][b] = v16i{0};
       ao[a][b] = v16i{0};
     }
 
-  // 40 chunks of 64 rows x 16 bytes per stage: wave w loads A chunks
-  // 3w .. 3w+2 (4 k blocks x 6 row chunks), H chunk w and O chunk w
-  // (4 k blocks x 2); each lane moves its 16 bytes to chunk base + 16 lane
+  // Chunks of 64 rows x 16 bytes per stage: 4 k blocks x OZ_R/64 row chunks
+  // of A (3 per wave), 4 k blocks x 2 of H and of O (8 / OZ_NWG each per
+  // wave); each lane moves its 16 bytes to chunk base + 16 lane
+  constexpr int RC = OZ_R / 64, NBC = 8 / OZ_NWG;
+  static_assert(4 * RC == OZ_WR * OZ_NWG && NBC * OZ_NWG == 8, "chunking");
   const int voff = lane * 16;
   auto issue = [&](int s, v4i (*dA)[OZ_R], v4i (*dH)[OZ_T], v4i (*dO)[OZ_T]) {
 #pragma unroll
-    for (int u = 0; u < 3; u++) {
-      const int ci = w * 3 + u;
-      const int kk = ci / 6, rc = ci - 6 * kk;
+    for (int u = 0; u < OZ_WR; u++) {
+      const int ci = w * OZ_WR + u;
+      const int kk = ci / RC, rc = ci - RC * kk;
       const int8_t *src = Ab + (((size_t)s * 4 + kk) * Gpad + I0 + rc * 64) * 16;
       __builtin_amdgcn_global_load_lds(EXH_GLB(src + voff), EXH_LDS(&dA[kk][rc * 64]), 16, 0, 0);
     }
-    const int kk = w >> 1, rc = w & 1;
-    const size_t boff = (((size_t)s * 4 + kk) * Gpad + J0 + rc * 64) * 16;
-    __builtin_amdgcn_global_load_lds(EXH_GLB(Hb + boff + voff), EXH_LDS(&dH[kk][rc * 64]), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(EXH_GLB(Ob + boff + voff), EXH_LDS(&dO[kk][rc * 64]), 16, 0, 0);
+#pragma unroll
+    for (int u = 0; u < NBC; u++) {
+      const int ci = w * NBC + u;
+      const int kk = ci >> 1, rc = ci & 1;
+      const size_t boff = (((size_t)s * 4 + kk) * Gpad + J0 + rc * 64) * 16;
+      __builtin_amdgcn_global_load_lds(EXH_GLB(Hb + boff + voff), EXH_LDS(&dH[kk][rc * 64]), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(EXH_GLB(Ob + boff + voff), EXH_LDS(&dO[kk][rc * 64]), 16, 0, 0);
+    }
   };
-  // Operand lane layout: row/column = lane%32, 16 k bytes from k block
-  // 2*ks + lane/32; result D[8*(r/4) + 4*(lane/32) + r%4][lane%32].
   auto compute = [&](v4i (*sA)[OZ_R], v4i (*sH)[OZ_T], v4i (*sO)[OZ_T]) {
 #pragma unroll
     for (int ks = 0; ks < 2; ks++) {
       const int kk = 2 * ks + lh;
-      v4i av[OZ_P], hv[2], ov[2];
+      v4i av[OZ_WR], hv[2], ov[2];
 #pragma unroll
-      for (int a = 0; a < OZ_P; a++) av[a] = sA[kk][rg * 32 * OZ_P + a * 32 + l32];
+      for (int a = 0; a < OZ_WR; a++) av[a] = sA[kk][rg * 32 * OZ_WR + a * 32 + l32];
 #pragma unroll
       for (int b = 0; b < 2; b++) {
         hv[b] = sH[kk][wc * 64 + b * 32 + l32];
         ov[b] = sO[kk][wc * 64 + b * 32 + l32];
       }
 #pragma unroll
-      for (int a = 0; a < OZ_P; a++)
+      for (int a = 0; a < OZ_WR; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
           ah[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], hv[b], ah[a][b], 0, 0, 0);
@@ -526,8 +533,11 @@ hipError_t oz_upload_constants(int nmod) {
 
 OzLayout oz_layout(int ngkmax, int Ktot, int nmod) {
   OzLayout L;
-  // rows padded to the 384-row panel of the GEMM workgroups
-  L.Gpad = (ngkmax + OZ_R - 1) / OZ_R * OZ_R;
+  // rows padded to 384: a multiple of the row panel (384 or 192 rows) and of
+  // the 128-column blocks, so every panel and column block loads inside it
+  constexpr int GA = 384;
+  static_assert(GA % OZ_R == 0 && GA % OZ_T == 0, "row padding");
+  L.Gpad = (ngkmax + GA - 1) / GA * GA;
   L.Kpad = (Ktot + OZ_KS - 1) / OZ_KS * OZ_KS;
   L.qstride = (size_t)L.Kpad * L.Gpad;
   const long long nt = (ngkmax + OZ_T - 1) / OZ_T;  // tile columns of the largest basis
@@ -574,7 +584,8 @@ hipError_t launch_ozaki(hipStream_t s, const exh_context *c, int ngp, const doub
   }
   if (stage_mask & 2) {
     const long long per = oz_panels((int)nt);
-    hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)(per * 3 * nmod)), dim3(512), 0, s, (int)nt,
+    hipLaunchKernelGGL(ozgemm_kernel, dim3((unsigned)(per * 3 * nmod)), dim3(64 * OZ_NWG), 0, s,
+                       (int)nt,
                        nslab, L.Gpad, nmod, QX, QB, L.qstride, R);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
