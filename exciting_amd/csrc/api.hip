@@ -3,9 +3,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
+#include <vector>
 
 #include "exh_device.h"
 
@@ -54,6 +57,153 @@ static int upload(exh_context *c, DevBuf &b, const T *src, size_t n) {
   hipError_t e = hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   return e == hipSuccess ? 0 : hip_fail(c, e, "upload");
+}
+
+// ---------------------------------------------------------------------------
+// Host arrays of the host-pointer entry points are pageable (Fortran or NumPy
+// arrays).  A pageable hipMemcpy moves about 12 GB/s; these copies instead go
+// through two pinned buffers of kPinBytes, the host side of each piece split
+// over threads, the DMA of one piece overlapping the host copy of the next.
+// A copy is a 2-D region: `rows` rows of `width` bytes, `hpitch` / `dpitch`
+// bytes apart on the host / device side (a 1-D copy is one row).
+// ---------------------------------------------------------------------------
+constexpr size_t kPinBytes = 64u << 20;
+constexpr size_t kPinDirect = 4u << 20;  // smaller copies: one plain hipMemcpy
+
+static int pin_threads() {
+  static const int n = [] {
+    const char *e = getenv("EXH_COPY_THREADS");
+    const int v = e ? atoi(e) : 8;
+    return std::max(1, std::min(v, 64));
+  }();
+  return n;
+}
+
+static hipError_t pin_ensure(exh_context *c) {
+  if (c->pin_bytes) return hipSuccess;
+  for (int b = 0; b < 2; b++) {
+    hipError_t e = hipHostMalloc(&c->pin[b], kPinBytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+  }
+  c->pin_bytes = kPinBytes;
+  return hipSuccess;
+}
+
+static void pin_release(exh_context *c) {
+  for (int b = 0; b < 2; b++) {
+    if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
+    if (c->pin[b]) (void)hipHostFree(c->pin[b]);
+    c->pin[b] = nullptr;
+    c->pin_ev[b] = nullptr;
+  }
+  c->pin_bytes = 0;
+}
+
+// rows [r0, r1) of `width` bytes from src (pitch sp) to dst (pitch dp), over
+// up to pin_threads() threads (by rows, or by bytes of a single row)
+static void par_copy(char *dst, size_t dp, const char *src, size_t sp, size_t width, size_t nrow) {
+  const size_t total = width * nrow;
+  const int nt = (int)std::min<size_t>(pin_threads(), std::max<size_t>(1, total >> 20));
+  auto part = [&](int t) {
+    if (nrow == 1) {
+      const size_t a = width * t / nt, b = width * (t + 1) / nt;
+      memcpy(dst + a, src + a, b - a);
+      return;
+    }
+    const size_t a = nrow * t / nt, b = nrow * (t + 1) / nt;
+    for (size_t r = a; r < b; r++) memcpy(dst + r * dp, src + r * sp, width);
+  };
+  if (nt == 1) {
+    part(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (int t = 1; t < nt; t++) th.emplace_back(part, t);
+  part(0);
+  for (auto &x : th) x.join();
+}
+
+// one DMA of `rows` rows (a plain copy for one row: pitches may be 0 there)
+static hipError_t dma(void *dst, size_t dp, const void *src, size_t sp, size_t width,
+                      size_t rows, hipMemcpyKind kind, hipStream_t s) {
+  if (rows == 1) return hipMemcpyAsync(dst, src, width, kind, s);
+  return hipMemcpy2DAsync(dst, dp, src, sp, width, rows, kind, s);
+}
+
+// The pieces of a region: whole rows per piece when a row fits a buffer,
+// else byte ranges of one row.
+struct Piece {
+  size_t row, off, len, nrow;
+};
+static std::vector<Piece> pieces(size_t width, size_t rows) {
+  std::vector<Piece> v;
+  if (width <= kPinBytes) {
+    const size_t per = kPinBytes / width;
+    for (size_t r = 0; r < rows; r += per) v.push_back({r, 0, width, std::min(per, rows - r)});
+  } else {
+    for (size_t r = 0; r < rows; r++)
+      for (size_t o = 0; o < width; o += kPinBytes)
+        v.push_back({r, o, std::min(kPinBytes, width - o), 1});
+  }
+  return v;
+}
+
+// host -> device, enqueued on s (returns once every piece is in pinned
+// memory; the DMAs complete in stream order)
+static hipError_t h2d(exh_context *c, hipStream_t s, void *dst, size_t dpitch, const void *src,
+                      size_t hpitch, size_t width, size_t rows) {
+  if (!width || !rows) return hipSuccess;
+  if (width * rows <= kPinDirect)
+    return dma(dst, dpitch, src, hpitch, width, rows, hipMemcpyHostToDevice, s);
+  hipError_t e = pin_ensure(c);
+  if (e != hipSuccess) return e;
+  const auto ps = pieces(width, rows);
+  for (size_t i = 0; i < ps.size() && e == hipSuccess; i++) {
+    const Piece &p = ps[i];
+    const int b = (int)(i & 1);
+    e = hipEventSynchronize(c->pin_ev[b]);  // the last DMA out of this buffer is done
+    if (e != hipSuccess) break;
+    par_copy((char *)c->pin[b], p.len, (const char *)src + p.row * hpitch + p.off, hpitch, p.len,
+             p.nrow);
+    e = dma((char *)dst + p.row * dpitch + p.off, dpitch, c->pin[b], p.len, p.len, p.nrow,
+            hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(c->pin_ev[b], s);
+  }
+  return e;
+}
+
+// device -> host after the work already enqueued on s; returns when dst holds
+// the data
+static hipError_t d2h(exh_context *c, hipStream_t s, void *dst, size_t hpitch, const void *src,
+                      size_t dpitch, size_t width, size_t rows) {
+  if (!width || !rows) return hipSuccess;
+  if (width * rows <= kPinDirect) {
+    hipError_t e = dma(dst, hpitch, src, dpitch, width, rows, hipMemcpyDeviceToHost, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+  }
+  hipError_t e = pin_ensure(c);
+  if (e != hipSuccess) return e;
+  const auto ps = pieces(width, rows);
+  auto issue = [&](size_t i) {
+    const Piece &p = ps[i];
+    const int b = (int)(i & 1);
+    hipError_t r = dma(c->pin[b], p.len, (const char *)src + p.row * dpitch + p.off, dpitch,
+                       p.len, p.nrow, hipMemcpyDeviceToHost, s);
+    return r == hipSuccess ? hipEventRecord(c->pin_ev[b], s) : r;
+  };
+  e = issue(0);
+  for (size_t i = 0; i < ps.size() && e == hipSuccess; i++) {
+    // buffer (i+1)&1 was last read by the host copy of piece i-1, finished
+    if (i + 1 < ps.size()) e = issue(i + 1);
+    if (e == hipSuccess) e = hipEventSynchronize(c->pin_ev[i & 1]);
+    if (e != hipSuccess) break;
+    const Piece &p = ps[i];
+    par_copy((char *)dst + p.row * hpitch + p.off, hpitch, (const char *)c->pin[i & 1], p.len,
+             p.len, p.nrow);
+  }
+  return e;
 }
 
 // polynom.f90 (m >= 0): m-th derivative at x of the degree np-1 polynomial
@@ -319,6 +469,7 @@ void exh_destroy(exh_context *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   batch_release_all(c);
   interstitial_release(c);
+  pin_release(c);
   if (c->done_ev) (void)hipEventDestroy(c->done_ev);
   DevBuf *bufs[] = {&c->colA, &c->katom, &c->atom_species_d, &c->apword_d, &c->omax_d,
                     &c->lorow, &c->lobase_d, &c->rmt_d, &c->mt_match.lu, &c->mt_match.piv,
@@ -793,9 +944,8 @@ static hipError_t stage_apwalm(exh_context *c, hipStream_t s, int ngp, const exh
     return hipSuccess;
   }
   const size_t ld = c->ngkmax, ncol = (size_t)c->apwordmax * c->lmmax * c->natmtot;
-  hipError_t e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm,
-                                  ld * sizeof(double2), ngp * sizeof(double2), ncol,
-                                  hipMemcpyHostToDevice, s);
+  hipError_t e = h2d(c, s, c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
+                     ngp * sizeof(double2), ncol);
   if (e == hipSuccess) {
     c->apw_host = apwalm;
     c->apw_dev = c->s_apwalm.p;
@@ -833,17 +983,15 @@ int exh_overlapsetup(exh_context *c, int ngp, int rank, const exh_z *apwalm, con
     if (e == hipSuccess)
       e = hipMemcpyAsync(c->s_vgpc.p, vgpc, 3 * (size_t)ngp * sizeof(double), hipMemcpyHostToDevice, s);
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(c->s_o.p, o, nn * sizeof(double2), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && h)
-    e = hipMemcpyAsync(c->s_h.p, h, nn * sizeof(double2), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = h2d(c, s, c->s_o.p, 0, o, 0, nn * sizeof(double2), 1);
+  if (e == hipSuccess && h) e = h2d(c, s, c->s_h.p, 0, h, 0, nn * sizeof(double2), 1);
   if (e != hipSuccess) return hip_fail(c, e, "overlapsetup: upload");
   rc = setup_impl(c, s, ngp, rank, (const exh_z *)c->s_apwalm.p, (const int *)c->s_igpig.p,
                   (const double *)c->s_vgpc.p, h ? (exh_z *)c->s_h.p : nullptr,
                   (exh_z *)c->s_o.p, EXH_ACCUMULATE, true);
   if (rc) return rc;
-  e = hipMemcpyAsync(o, c->s_o.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && h)
-    e = hipMemcpyAsync(h, c->s_h.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
+  e = d2h(c, s, o, 0, c->s_o.p, 0, nn * sizeof(double2), 1);
+  if (e == hipSuccess && h) e = d2h(c, s, h, 0, c->s_h.p, 0, nn * sizeof(double2), 1);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "overlapsetup: download");
 }
@@ -870,12 +1018,12 @@ int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
   if (e == hipSuccess)
     e = hipMemcpyAsync(c->s_tpgpc.p, tpgpc, 2 * (size_t)ngp * sizeof(double), hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
-    e = hipMemcpy2DAsync(c->s_sfac.p, ld * sizeof(double2), sfacgp, ld * sizeof(double2),
-                         ngp * sizeof(double2), c->natmtot, hipMemcpyHostToDevice, s);
+    e = h2d(c, s, c->s_sfac.p, ld * sizeof(double2), sfacgp, ld * sizeof(double2),
+            ngp * sizeof(double2), c->natmtot);
   // entries with io >= apword(l) are not written by match: keep the caller's
   if (e == hipSuccess && !c->uniform_apword)
-    e = hipMemcpy2DAsync(c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
-                         ngp * sizeof(double2), ncol, hipMemcpyHostToDevice, s);
+    e = h2d(c, s, c->s_apwalm.p, ld * sizeof(double2), apwalm, ld * sizeof(double2),
+            ngp * sizeof(double2), ncol);
   if (e != hipSuccess) return hip_fail(c, e, "match: upload");
   e = timed(c, EXH_K_MATCH, s, [&] {
     return launch_match(s, c, ngp, (const double *)c->s_gpc.p, (const double *)c->s_tpgpc.p,
@@ -883,8 +1031,8 @@ int exh_match(exh_context *c, int ngp, const double *gpc, const double *tpgpc,
                         (double *)c->mtab.p);
   });
   if (e == hipSuccess)
-    e = hipMemcpy2DAsync(apwalm, ld * sizeof(double2), c->s_apwalm.p, ld * sizeof(double2),
-                         ngp * sizeof(double2), ncol, hipMemcpyDeviceToHost, s);
+    e = d2h(c, s, apwalm, ld * sizeof(double2), c->s_apwalm.p, ld * sizeof(double2),
+            ngp * sizeof(double2), ncol);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     c->apw_host = nullptr;
@@ -925,15 +1073,15 @@ int exh_hamiltonandoverlapsetup_ex(exh_context *c, int ngp, int rank, const exh_
       e = hipMemcpyAsync(c->s_vgpc.p, vgpc, 3 * (size_t)ngp * sizeof(double), hipMemcpyHostToDevice, s);
   }
   if (e == hipSuccess && (flags & EXH_ACCUMULATE)) {
-    e = hipMemcpyAsync(c->s_h.p, h, nn * sizeof(double2), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->s_o.p, o, nn * sizeof(double2), hipMemcpyHostToDevice, s);
+    e = h2d(c, s, c->s_h.p, 0, h, 0, nn * sizeof(double2), 1);
+    if (e == hipSuccess) e = h2d(c, s, c->s_o.p, 0, o, 0, nn * sizeof(double2), 1);
   }
   if (e != hipSuccess) return hip_fail(c, e, "setup: upload");
   rc = exh_setup_device(c, s, ngp, rank, (const exh_z *)c->s_apwalm.p, (const int *)c->s_igpig.p,
                         (const double *)c->s_vgpc.p, (exh_z *)c->s_h.p, (exh_z *)c->s_o.p, flags);
   if (rc) return rc;
-  e = hipMemcpyAsync(h, c->s_h.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(o, c->s_o.p, nn * sizeof(double2), hipMemcpyDeviceToHost, s);
+  e = d2h(c, s, h, 0, c->s_h.p, 0, nn * sizeof(double2), 1);
+  if (e == hipSuccess) e = d2h(c, s, o, 0, c->s_o.p, 0, nn * sizeof(double2), 1);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? 0 : hip_fail(c, e, "setup: download");
 }

@@ -232,6 +232,11 @@ struct exh_context {
   int apw_ngp = -1;
   long long apw_basis_epoch = -1;
   long long apw_uploads = 0, apw_reused = 0;
+  // pinned staging of the host-array entry points: two buffers of
+  // pin_bytes, each with the event of the last DMA that used it
+  void *pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  size_t pin_bytes = 0;
 };
 
 namespace exh {

@@ -251,3 +251,45 @@ def test_sc64_split_eigenvalues_against_the_oracle(sc64, nmod):
     log(f"  sc64 k0 split({nmod}): H {drel(H, ref['H']):.1e} O {drel(O, ref['O']):.1e} "
         f"lowest {NSTATES} max|dE| {de:.2e} Ha")
     assert de <= TOL_SPLIT_HA
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("reuse", [False, True])
+def test_ortho16_host_arrays_equal_the_device_path(ortho16, reuse):
+    """The host-array entry points at a size where every big copy is staged
+    through the pinned buffers in several pieces (apwalm 207 MB as 2-D
+    pieces of whole columns, H and O 207 MB each as byte ranges): match into
+    a NumPy apwalm, then the accumulating setup into prefilled NumPy H and O,
+    bitwise equal to the device entry points on the same inputs (H and O
+    prefilled the same way), with and without apwalm reuse."""
+    import torch
+    c = ortho16
+    s, ik = c.s, 77
+    k = s.kpts[ik]
+    d = c.dk[ik]
+    n = d.rank
+    lmm = (s.lmaxapw + 1) ** 2
+    rng = np.random.default_rng(11)
+    pre = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in range(2)]
+    pre = [np.asfortranarray(m + m.conj().T) for m in pre]
+    A = np.zeros((s.ngkmax, s.apwordmax, lmm, s.natmtot), np.complex128, order="F")
+    Hh, Oh = pre[0].copy(order="F"), pre[1].copy(order="F")
+    c.ctx.set_apwalm_reuse(reuse)
+    u0, r0 = c.ctx.apwalm_transfers()
+    try:
+        c.ctx.match(k.ngk, k.gpc, k.tpgpc, k.sfacgp, A)
+        c.ctx.hamiltonandoverlapsetup(lapw.EvSystem(lapw.HermitianMatrix(n, Hh),
+                                                    lapw.HermitianMatrix(n, Oh)),
+                                      k.ngk, A, k.igpig, k.vgpc)
+        u1, r1 = c.ctx.apwalm_transfers()
+    finally:
+        c.ctx.set_apwalm_reuse(False)
+    assert (u1 - u0, r1 - r0) == ((0, 1) if reuse else (1, 0))
+    Hd, Od = dev(pre[0]), dev(pre[1])
+    device_setup(c.ctx, d, lapw.ACCUMULATE, H=Hd, O=Od)
+    Ad = d.apwalm.cpu().numpy().reshape(A.shape, order="F")
+    assert np.array_equal(A[:k.ngk], Ad[:k.ngk])
+    assert not np.any(A[k.ngk:]), "rows beyond ngp stay untouched"
+    assert np.array_equal(Hh, Hd.cpu().numpy().reshape(n, n, order="F"))
+    assert np.array_equal(Oh, Od.cpu().numpy().reshape(n, n, order="F"))
+    log(f"  ortho16 k{ik} host arrays (reuse {reuse}): n {n}, bitwise the device path")
