@@ -248,6 +248,12 @@ int exh_genveffig_device(exh_context *ctx, void *stream, const double *veffir,
                          exh_z *m2effig);
 
 /* ---- hot path ----------------------------------------------------------- */
+/* Host-array entry points (exh_match, exh_hamiltonandoverlapsetup[_ex],
+ * exh_overlapsetup) copy arrays larger than 4 MiB through two pinned 64 MiB
+ * host buffers that the context allocates on first use and exh_destroy
+ * frees; the host side of each piece runs on EXH_COPY_THREADS threads
+ * (environment variable, default 8).  The caller's arrays stay pageable and
+ * may be reused as soon as the call returns. */
 /* match.f90:13-194.  gpc(ngkmax), tpgpc(2,ngkmax), sfacgp(ngkmax,natmtot) in;
  * apwalm(ngkmax,apwordmax,(lmaxapw+1)^2,natmtot) out (entries with io beyond
  * apword(l) are left untouched, as in the reference). */
