@@ -38,7 +38,9 @@ namespace exh {
 namespace {
 
 constexpr int kPlans = kHoShapes * 4;  // shape x {tile grid, persistent} x {k-major, XCD}
-constexpr int kDefaultPlan = 1;        // 64x64x16, one workgroup per tile, XCD order
+// 64x64x8, one workgroup per tile, XCD order: the autotuner's choice at 64
+// atoms (16 k-points and one) and within 1 % of it at 8 and 16 atoms
+constexpr int kDefaultPlan = 5;
 constexpr size_t kMaxEntries = 8;
 
 struct PlanInfo {

@@ -28,7 +28,11 @@ __device__ __forceinline__ dx4 mfma(double a, double b, dx4 c) {
 }
 
 constexpr int TM = 64;    // output tile edge (rows and columns)
-constexpr int TK = 16;    // K slab staged in LDS per step
+constexpr int TK = 16;    // K slab staged in LDS per step (bt)
+// K slab of the single-k H/O tile: 8 deep, as the grouped kernel's default
+// plan; 183 ms -> 161 ms per 64-atom k-point against 16, same sums in the
+// same order
+constexpr int TKH = 8;
 constexpr int NTH = 256;  // 4 waves; wave w owns a 32x32 quadrant
 
 // ---------------------------------------------------------------------------
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(NTH) void ho_kernel(
   while (J * (J + 1) / 2 > t) J--;
   const int I = t - J * (J + 1) / 2;
   IstBox box{ivgk, vgpc, T, lo0, lo1, lo2, n0, n1, n2};
-  ho_tile<TM, TM, TK, const double2 *, double2 *, IstBox, WH>(
+  ho_tile<TM, TM, TKH, const double2 *, double2 *, IstBox, WH>(
       ngp, Ktot, X, ldx, YB, ldb, YA, lda, box, H, O, ldh, accumulate, I * TM, J * TM);
 }
 

@@ -387,8 +387,8 @@ int exh_setup_multi_device(exh_context *const *ctxs, void *stream, int nk,
 int exh_batch_plan_count(void);
 int exh_batch_plan_info(int plan, int *tile_rows, int *tile_cols, int *k_slab,
                         int *persistent, int *xcd_order);
-/* Plan used by exh_setup_batch_device (-1 = the default: 64x64 tiles with K
- * slabs of 16, one workgroup per tile, XCD order). */
+/* Plan used by exh_setup_batch_device (-1 = the default, plan 5: 64x64 tiles
+ * with K slabs of 8, one workgroup per tile, XCD order). */
 int exh_set_batch_plan(exh_context *ctx, int plan);
 int exh_get_batch_plan(exh_context *ctx);
 /* Autotuner: times every plan on (the first few of) these k-points, with the
