@@ -78,8 +78,9 @@ class Request:
         return self.finished - self.arrival
 
 
-def context_bytes(sys_, nmod_max: int = 12) -> int:
-    """HBM one context holds while it exists, for a cell (LapwSystem)."""
+def context_bytes(sys_, nmod_max: int = 12, slots: int = 1) -> int:
+    """HBM one context holds while it exists, for a cell (LapwSystem), with
+    ``slots`` k-point slots (apwalm, H, O and eigensolve temporaries each)."""
     n = sys_.ngkmax + sys_.nlotot
     lmm = (sys_.lmaxapw + 1) ** 2
     apwalm = 16 * sys_.ngkmax * sys_.apwordmax * lmm * sys_.natmtot
@@ -97,7 +98,7 @@ def context_bytes(sys_, nmod_max: int = 12) -> int:
     # Cholesky factor, the two triangular solves' results, the reduced matrix
     # and its Hermitian average, and the eigensolver's copy and workspace
     solve = 8 * 16 * n * n
-    return apwalm + ho + scratch + perk + solve
+    return slots * (apwalm + ho + solve) + scratch + perk
 
 
 class ContextPool:
@@ -302,18 +303,27 @@ def _solver_pool(device: int, max_solves: int):
 
 
 @dataclasses.dataclass
+class _Slot:
+    """apwalm, H and O of one k-point of a context (CUDA tensors)."""
+    apwalm: object
+    H: object
+    O: object
+
+
+@dataclasses.dataclass
 class _Setup:
     req: Request
     ctx: object
     d: object            # lapw.DeviceKPoint
+    slot: _Slot
     flags: int
     fut: concurrent.futures.Future
 
 
 class Batcher:
     """Merges the k-point setups of every request in flight into grouped
-    launches.  Request threads submit one k-point at a time (match + H/O
-    setup into their context's apwalm, H and O); the batcher thread takes
+    launches.  Request threads submit the k-points their context's slots
+    can hold (match + H/O setup into a slot's apwalm, H and O); the batcher thread takes
     whatever is pending -- up to ``max_k`` k-points, of any cells -- and
     runs it as one exh_setup_multi_device call per precision on its own
     stream, then waits for that launch.  Setups that arrive meanwhile queue
@@ -344,14 +354,14 @@ class Batcher:
         self.thread = threading.Thread(target=self._loop, name="exh-batcher", daemon=True)
         self.thread.start()
 
-    def submit(self, req: Request, ctx, d, flags: int) -> concurrent.futures.Future:
-        """Queue match + setup of one k-point; the future gives the event
-        that completes with it."""
+    def submit(self, req: Request, ctx, d, slot: _Slot, flags: int) -> concurrent.futures.Future:
+        """Queue match + setup of one k-point into ``slot``; the future gives
+        the event that completes with it."""
         fut: concurrent.futures.Future = concurrent.futures.Future()
         with self.cv:
             if self.closed:
                 raise RuntimeError("batcher closed")
-            self.q.append(_Setup(req, ctx, d, flags, fut))
+            self.q.append(_Setup(req, ctx, d, slot, flags, fut))
             self.cv.notify()
         return fut
 
@@ -391,7 +401,8 @@ class Batcher:
     def _launch(self, its: List[_Setup], flags: int):
         import torch
         from . import lapw
-        kp = [lapw.kpoint_struct(it.d, it.ctx.H, it.ctx.O, apwalm=it.ctx.apwalm) for it in its]
+        kp = [lapw.kpoint_struct(it.d, it.slot.H, it.slot.O, apwalm=it.slot.apwalm)
+              for it in its]
         ctxs = [it.ctx for it in its]
         lead = ctxs[0]
         t0 = time.perf_counter()
@@ -427,9 +438,10 @@ class Batcher:
         self.max_kpoints = max(self.max_kpoints, len(its))
         self.cells += ncell
         self.max_cells = max(self.max_cells, ncell)
-        self.requests += len({it.req.rid for it in its})
-        for it in its:
-            it.req.launches += 1
+        reqs = {id(it.req): it.req for it in its}
+        self.requests += len(reqs)
+        for r in reqs.values():
+            r.launches += 1
         return ev
 
     def stats(self) -> dict:
@@ -451,13 +463,20 @@ class GpuExecutor:
     small cells)."""
 
     def __init__(self, device: int = 0, digests: bool = False, max_solves: int = 4,
-                 max_batch_k: int = 32, plan: int = -1, linger_s: float = 0.002):
+                 max_batch_k: int = 32, plan: int = -1, linger_s: float = 0.002,
+                 slots: int = 1):
         from . import solve as _solve
         _solve.configure()
         self.device = device
         self.dev = f"cuda:{device}"
         self.digests = digests
         self.solvers, self._local = _solver_pool(device, max_solves)
+        # k-point slots per context: with two, a request's next setup runs
+        # while the previous k-point's eigensolve does, and its first two
+        # setups share a launch.  One is the default: at 64 atoms two slots
+        # took a 16-k-point request from 10.50 to 10.29 s (the eigensolves
+        # already keep the GPU busy) for twice the context memory.
+        self.slots = max(1, slots)
         self.batcher = Batcher(device, max_k=max_batch_k, plan=plan, linger_s=linger_s)
 
     def _solve_on_worker(self, req, ready, H, O, n, nst):
@@ -494,37 +513,73 @@ class GpuExecutor:
         lmm = (sys_.lmaxapw + 1) ** 2
         ctx.tstream = torch.cuda.ExternalStream(ctx.stream, device=self.dev)
         with torch.cuda.stream(ctx.tstream):
-            ctx.apwalm = torch.zeros(sys_.natmtot * lmm * sys_.apwordmax * sys_.ngkmax,
-                                     dtype=torch.complex128, device=self.dev)
-            ctx.H = torch.empty(n * n, dtype=torch.complex128, device=self.dev)
-            ctx.O = torch.empty(n * n, dtype=torch.complex128, device=self.dev)
+            ctx.slots = [_Slot(torch.zeros(sys_.natmtot * lmm * sys_.apwordmax * sys_.ngkmax,
+                                           dtype=torch.complex128, device=self.dev),
+                               torch.empty(n * n, dtype=torch.complex128, device=self.dev),
+                               torch.empty(n * n, dtype=torch.complex128, device=self.dev))
+                         for _ in range(self.slots)]
         ctx.tstream.synchronize()
         return ctx
 
-    def execute(self, req: Request, ctx):
+    def _kpoint(self, ctx, ik):
+        """The per-k inputs of k-point ik of the context's cell, uploaded on
+        first use."""
         import torch
         from . import lapw
-        s = ctx.sys
-        out, dg = [], []
+        d = ctx.kp.get(ik)
+        if d is None:
+            with torch.cuda.stream(ctx.tstream):
+                d = lapw.DeviceKPoint(ctx.sys, ik, device=self.dev, with_apwalm=False)
+            ctx.tstream.synchronize()     # uploaded before the batcher reads it
+            ctx.kp[ik] = d
+        return d
+
+    def execute(self, req: Request, ctx):
+        """The request's k-points through the context's slots: every free
+        slot takes the next k-point, and the setups queued together reach the
+        batcher together (one grouped launch); each eigensolve starts when its
+        setup is done, and its slot takes the next k-point when it returns.
+        On a failure every setup and solve already started is waited for
+        before the error propagates, so no GPU work is left on the slots."""
+        from . import lapw
         flags = lapw.SPLIT(req.nmod) if req.nmod else 0
-        for ik in req.kpoints:
-            d = ctx.kp.get(ik)
-            if d is None:
-                with torch.cuda.stream(ctx.tstream):
-                    d = lapw.DeviceKPoint(s, ik, device=self.dev, with_apwalm=False)
-                ctx.tstream.synchronize()     # uploaded before the batcher reads it
-                ctx.kp[ik] = d
-            # one k-point of this request in flight: its H and O are free
-            # again once the previous eigensolve has returned
-            ready = self.batcher.submit(req, ctx, d, flags).result()
-            nn = d.rank * d.rank
-            if self.digests:
-                ready.synchronize()
-                h = hashlib.sha1(ctx.H[:nn].cpu().numpy().tobytes())
-                h.update(ctx.O[:nn].cpu().numpy().tobytes())
-                dg.append(h.hexdigest())
-            out.append(self.solvers.submit(self._solve_on_worker, req, ready, ctx.H, ctx.O,
-                                           d.rank, min(req.nstfv, d.rank)).result())
+        ks = list(req.kpoints)
+        out, dg = [None] * len(ks), [None] * len(ks)
+        free = list(ctx.slots)
+        inflight = {}                  # solve future -> (index, slot)
+        started = []                   # every setup and solve future
+        nxt = 0
+        try:
+            while nxt < len(ks) or inflight:
+                subs = []
+                while free and nxt < len(ks):
+                    slot = free.pop(0)
+                    d = self._kpoint(ctx, ks[nxt])
+                    fut = self.batcher.submit(req, ctx, d, slot, flags)
+                    started.append(fut)
+                    subs.append((nxt, slot, d, fut))
+                    nxt += 1
+                for i, slot, d, fut in subs:
+                    ready = fut.result()
+                    if self.digests:
+                        nn = d.rank * d.rank
+                        ready.synchronize()
+                        h = hashlib.sha1(slot.H[:nn].cpu().numpy().tobytes())
+                        h.update(slot.O[:nn].cpu().numpy().tobytes())
+                        dg[i] = h.hexdigest()
+                    sf = self.solvers.submit(self._solve_on_worker, req, ready, slot.H, slot.O,
+                                             d.rank, min(req.nstfv, d.rank))
+                    started.append(sf)
+                    inflight[sf] = (i, slot)
+                done, _ = concurrent.futures.wait(
+                    list(inflight), return_when=concurrent.futures.FIRST_COMPLETED)
+                for sf in done:
+                    i, slot = inflight.pop(sf)
+                    out[i] = sf.result()
+                    free.append(slot)
+        except BaseException:
+            concurrent.futures.wait(started)
+            raise
         if self.digests:
             req.digests = dg
         return out
@@ -534,11 +589,15 @@ class GpuExecutor:
         ctx.tstream.synchronize()
         ctx.close()
         ctx.kp.clear()
-        ctx.apwalm = ctx.H = ctx.O = None
+        ctx.slots = None
         torch.cuda.empty_cache()
 
     def stats(self) -> dict:
         return self.batcher.stats()
+
+    def footprint(self, sys_) -> int:
+        """context_bytes for this executor's slot count (Server footprint)."""
+        return context_bytes(sys_, slots=self.slots)
 
     def close(self) -> None:
         """Stop the batcher thread (the solver threads stay for the process)."""
@@ -684,6 +743,8 @@ def main(argv=None) -> int:
     ap.add_argument("--reserve-gib", type=float, default=48.0,
                     help="HBM left outside the admission budget (library and "
                          "allocator working room)")
+    ap.add_argument("--slots", type=int, default=1,
+                    help="k-point slots per context (apwalm, H and O each)")
     ap.add_argument("--cells", default="fcc2,sc8,ortho16,sc32,sc64")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--segment", default="",
@@ -711,7 +772,8 @@ def main(argv=None) -> int:
     print(f"[serve] cells built in {time.perf_counter() - t:.1f} s: "
           + ", ".join(f"{k} ngk<={c.ngkmax} nk={len(c.kpts)}" for k, c in cells.items()),
           file=sys.stderr, flush=True)
-    ex = GpuExecutor(0, max_solves=args.max_solves, max_batch_k=args.max_batch_k)
+    ex = GpuExecutor(0, max_solves=args.max_solves, max_batch_k=args.max_batch_k,
+                     slots=args.slots)
     make, close, execute = ex.make_context, ex.close_context, ex.execute
     budget = (free_hbm(0) - int(args.reserve_gib * 2**30)
               - args.max_solves * _solve.WORKSPACE_BYTES)   # the solvers' hipBLAS workspaces
@@ -727,7 +789,7 @@ def main(argv=None) -> int:
                 rec.flush()
 
     srv = Server(cells, execute, make, close, budget, max_inflight=args.max_inflight,
-                 on_finish=report)
+                 on_finish=report, footprint=ex.footprint)
     # warm-up, untimed: one context per cell, one FP64 and one split k-point
     warm = [Request(rid=-1 - i, cell=c, kpoints=[0], nmod=m, nstfv=args.nstfv)
             for i, (c, m) in enumerate((c, m) for c in cells for m in (0, args.nmod))]

@@ -1,10 +1,11 @@
 """Serving on the GPU: each result is the same alone and in the mix.
 
 A fixed list of setup + eigensolve requests over three cells (FP64 and split
-precision, 1-4 k-points each) is served twice: one request at a time, then
-all at once with up to six in flight, their k-point setups merged into
-grouped launches over several cells (exh_setup_multi_device) and their
-eigensolves overlapping on the solver threads.  Tolerance: the H
+precision, 1-4 k-points each) is served twice: one request at a time with
+one k-point slot per context, then all at once with up to six in flight and
+two slots per context, their k-point setups merged into grouped launches
+over several cells (exh_setup_multi_device) and their eigensolves
+overlapping on the solver threads.  Tolerance: the H
 and O matrices must be bitwise identical (SHA-1 of their bytes per k-point),
 and the eigenvalues identical to 1e-12 Ha (they come from the same library
 on bitwise identical inputs).  The FP64 eigenvalues are also checked against
@@ -56,10 +57,11 @@ def served():
     print("[gpu-serve] mix of 10 requests, up to 6 in flight", flush=True)
     mix = _requests()
     # the ten requests arrive within 20 ms: a 20 ms linger lets the setups
-    # of concurrent requests of different cells share launches
-    ex2 = serve.GpuExecutor(0, digests=True, linger_s=0.02)
+    # of concurrent requests of different cells share launches; two k-point
+    # slots per context let a request's own k-points share them too
+    ex2 = serve.GpuExecutor(0, digests=True, linger_s=0.02, slots=2)
     srv2 = serve.Server(cells, ex2.execute, ex2.make_context, ex2.close_context, budget,
-                        max_inflight=6)
+                        max_inflight=6, footprint=ex2.footprint)
     srv2.run(mix)
     srv2.pool.close_all()
     mix_stats = ex2.stats()
@@ -71,9 +73,12 @@ def served():
 @pytest.mark.gpu
 def test_every_request_succeeds_and_overlaps(served):
     cells, alone, mix, srv, alone_stats, mix_stats = served
-    for r in alone + mix:
+    for r in alone:
         assert r.status == "done", (r.rid, r.error)
         assert r.launches == len(r.kpoints)
+    for r in mix:
+        assert r.status == "done", (r.rid, r.error)
+        assert 1 <= r.launches <= len(r.kpoints)
     assert srv.peak_inflight > 1
 
 
@@ -81,7 +86,8 @@ def test_every_request_succeeds_and_overlaps(served):
 def test_requests_of_several_cells_share_launches(served):
     """In the mix the setups of concurrent requests run as grouped launches
     over several cells: fewer launches than k-points, and at least one
-    launch holding k-points of two or more cells."""
+    launch holding k-points of two or more cells.  Alone, with one slot per
+    context, every launch holds one k-point."""
     cells, alone, mix, srv, alone_stats, mix_stats = served
     nk = sum(len(r.kpoints) for r in mix)
     assert alone_stats["kpoints_per_launch"]["max"] == 1

@@ -201,6 +201,8 @@ def test_context_footprint_covers_setup_and_solve():
     b = serve.context_bytes(s)
     assert b > 7 * 16 * n * n          # H, O and the eigensolve temporaries
     assert serve.context_bytes(s, nmod_max=0) < b
+    # every slot holds its own apwalm, H, O and eigensolve temporaries
+    assert serve.context_bytes(s, slots=2) - b >= (2 * 16 + 8 * 16) * n * n
 
 
 def test_serving_cells_span_two_to_sixty_four_atoms():
