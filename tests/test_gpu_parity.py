@@ -1,6 +1,6 @@
 """HIP path vs the oracle on identical seeded inputs (through the C ABI).
 
-Tolerances: relative Frobenius error <= 1e-12 for
+Tolerances (the FP64 drop-in bar): relative Frobenius error <= 1e-12 for
 apwalm, H and O; <= 1e-8 Ha for the eigenvalues of H x = e O x.
 """
 import numpy as np
