@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexh.so")
 SOURCES = ["api.hip", "setup_kernels.hip", "match_kernel.hip", "ozaki_kernels.hip", "batch.hip",
-           "interstitial.hip", "hmlint.hip"]
+           "interstitial.hip", "hmlint.hip", "dense_kernels.hip"]
 # hipFFT for the 3-D transforms of the interstitial producers
 LINK = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
 ARCH = os.environ.get("EXH_OFFLOAD_ARCH", "gfx950")

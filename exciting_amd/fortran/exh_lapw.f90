@@ -59,7 +59,7 @@ module exh_lapw
             exh_set_mt_device, exh_overlapsetup, exh_overlapsetup_device, exh_mt_kinetic_device, &
             exh_mt_pot, exh_mt_pot_device, exh_mt_so, exh_mt_so_device, exh_stream, &
             exh_match_device, exh_setup_device, exh_setup_device_noalloc, &
-            exh_set_apwalm_reuse, exh_apwalm_transfers, &
+            exh_set_apwalm_reuse, exh_apwalm_transfers, exh_zgemm_cn_device, &
             exh_batch_plan_count, exh_batch_plan_info, &
             exh_profile_enable, exh_profile_read, exh_profile_reset
 
@@ -400,6 +400,19 @@ module exh_lapw
       type(c_ptr), value :: ctx, stream
       integer(c_int), value :: nk, flags
       type(exh_kpoint), intent(in) :: kpts(*)
+    end function
+
+    ! y = alpha op(a) x + beta y0 + gamma z on the device, op(a) = a^H
+    ! (conj_a /= 0) or a^T (the eigensolver's filter product; y0 and z may
+    ! be c_null_ptr)
+    integer(c_int) function exh_zgemm_cn_device(stream, conj_a, nout, m, k, a, lda, x, ldx, &
+        alpha, beta, y0, ldy0, gamma, z, ldz, y, ldy, split, work) &
+        bind(C, name="exh_zgemm_cn_device")
+      import :: c_int, c_ptr, c_long_long, c_double
+      type(c_ptr), value :: stream, a, x, y0, z, y, work
+      integer(c_int), value :: conj_a, nout, m, k, split
+      integer(c_long_long), value :: lda, ldx, ldy0, ldz, ldy
+      real(c_double), value :: alpha, beta, gamma
     end function
 
     ! k-points of several structures in one grouped launch: kpts(i) belongs

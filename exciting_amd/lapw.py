@@ -78,6 +78,9 @@ def lib():
     L.exh_setup_device.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
     L.exh_setup_device_noalloc.argtypes = [_p, _p, _i, _i, _p, _p, _p, _p, _p, _i]
     L.exh_set_apwalm_reuse.argtypes = [_p, _i]
+    _ll = C.c_longlong
+    L.exh_zgemm_cn_device.argtypes = [_p, _i, _i, _i, _i, _p, _ll, _p, _ll, _d, _d, _p, _ll,
+                                      _d, _p, _ll, _p, _ll, _i, _p]
     L.exh_apwalm_transfers.argtypes = [_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.exh_profile_enable.argtypes = [_p, _i]
     L.exh_profile_read.argtypes = [_p, _i, C.POINTER(_d), C.POINTER(_i)]
@@ -129,7 +132,7 @@ EXPORTS = (
     "exh_hmlint_device", "exh_overlapsetup", "exh_overlapsetup_device",
     "exh_mt_kinetic_device", "exh_mt_pot", "exh_mt_pot_device", "exh_mt_so",
     "exh_mt_so_device", "exh_setup_device_noalloc", "exh_set_apwalm_reuse",
-    "exh_apwalm_transfers", "exh_setup_multi_device",
+    "exh_apwalm_transfers", "exh_setup_multi_device", "exh_zgemm_cn_device",
 )
 
 KERNELS = ("match", "ivgk", "bt", "a2t", "ho", "lo_border", "lolo", "pack", "oz_residues",
@@ -724,6 +727,45 @@ def setup_multi(ctxs, kpts, flags: int = ACCUMULATE, stream=None):
     lead._check(lead._device_call(stream, lambda st: lib().exh_setup_multi_device(
         handles, st, len(kpts), arr, flags)), "exh_setup_multi_device")
     return arr
+
+
+def zgemm_cn(a, x, alpha: float = 1.0, beta: float = 0.0, y0=None, gamma: float = 0.0, z=None,
+             split: int = 0, stream=None, hermitian: bool = False):
+    """exh_zgemm_cn_device on complex128 CUDA tensors: returns
+    alpha a^H x + beta y0 + gamma z (nout x m) for a (k x nout) and x (k x m).
+    With hermitian=True, ``a`` is a Hermitian matrix C and the result is
+    alpha C x + ...; C is read in place whether it is stored row-major or
+    column-major (no copy of C).  ``split`` chunks of K (0: enough for about
+    4096 workgroups, at most 32)."""
+    import torch
+    if a.dtype != torch.complex128 or x.dtype != torch.complex128 or not a.is_cuda:
+        raise ExhError("zgemm_cn: complex128 CUDA tensors expected")
+    conj_a = 1
+    if hermitian and not a.is_contiguous() and a.t().is_contiguous():
+        a, conj_a = a.t(), 0      # row-major C^T: C x = (C^T)^T x
+    a = a.contiguous()
+    x = x.contiguous()
+    k, nout = a.shape
+    if x.shape[0] != k:
+        raise ExhError("zgemm_cn: a and x disagree on k")
+    m = x.shape[1]
+    tiles = -(-nout // 64) * -(-m // 64)
+    split = split or max(1, min(32, -(-4096 // max(tiles, 1))))
+    out = torch.empty((nout, m), dtype=torch.complex128, device=a.device)
+    work = torch.empty(split * nout * m, dtype=torch.complex128, device=a.device)
+    for t in (y0, z):
+        if t is not None and (t.shape != (nout, m) or t.dtype != torch.complex128):
+            raise ExhError("zgemm_cn: y0 and z must be complex128 nout x m")
+    y0 = y0.contiguous() if y0 is not None else None
+    z = z.contiguous() if z is not None else None
+    st = stream if stream is not None else torch.cuda.current_stream(a.device).cuda_stream
+    rc = lib().exh_zgemm_cn_device(st, conj_a, nout, m, k, _ptr(a), nout, _ptr(x), m, alpha,
+                                   beta, _ptr(y0) if y0 is not None else None, m, gamma,
+                                   _ptr(z) if z is not None else None, m, _ptr(out), m, split,
+                                   _ptr(work))
+    if rc != 0:
+        raise ExhError(f"exh_zgemm_cn_device failed ({rc})")
+    return out
 
 
 def kpoint_struct(d: "DeviceKPoint", h, o, apwalm=None) -> KPoint:

@@ -51,7 +51,8 @@ enum {
   EXH_ESTATE = -2,   /* required set_* call missing           */
   EXH_ERANGE = -3,   /* size beyond what the kernels support  */
   EXH_ENOTSUP = -4,  /* option not implemented                */
-  EXH_EMATCH = -5    /* singular boundary-derivative matrix   */
+  EXH_EMATCH = -5,   /* singular boundary-derivative matrix   */
+  EXH_EHIP = -6      /* HIP runtime error (stateless calls)   */
 };
 
 /* Flags for exh_setup_device / exh_hamiltonandoverlapsetup_ex. */
@@ -326,6 +327,24 @@ int exh_overlapsetup(exh_context *ctx, int ngp, int rank, const exh_z *apwalm,
 int exh_overlapsetup_device(exh_context *ctx, void *stream, int ngp, int rank,
                             const exh_z *apwalm, const int *igpig, const double *vgpc,
                             exh_z *h, exh_z *o, int flags);
+
+/* ---- the eigensolve after the setup ------------------------------------- */
+/* No counterpart in the reference as a call (it is inside ZHEGVX,
+ * modfvsystem.f90:585-720): the product of the Chebyshev-filtered subspace
+ * iteration that solves the reduced problem for the lowest states.
+ * y = alpha * op(a) x + beta * y0 + gamma * z, all device memory, complex,
+ * enqueued on `stream`, with op(a) = a^H (conj_a != 0) or a^T (conj_a = 0)
+ * for a(k, i) at a[k*lda + i] (k < k, i < nout).  A Hermitian matrix C gives
+ * C x either way without a copy: row-major storage with conj_a = 1,
+ * column-major storage with conj_a = 0.  x(k, j) at x[k*ldx + j] (j < m);
+ * y0, z and y row-major nout x m (y0 and z may be NULL, y may be y0).  K is split into at most `split` chunks, each summed by its own
+ * workgroups into work (split * nout * m complex), then the chunks are
+ * added in order: the result does not depend on the launch.  Returns
+ * EXH_EINVAL for bad sizes, EXH_EHIP if a launch fails. */
+int exh_zgemm_cn_device(void *stream, int conj_a, int nout, int m, int k, const exh_z *a,
+                        long long lda, const exh_z *x, long long ldx, double alpha, double beta,
+                        const exh_z *y0, long long ldy0, double gamma, const exh_z *z,
+                        long long ldz, exh_z *y, long long ldy, int split, exh_z *work);
 
 /* ---- batched k-points ---------------------------------------------------- */
 /* No reference counterpart as one call: the reference runs match and

@@ -168,3 +168,47 @@ def test_subspace_vectors_at_64_atoms(nst):
         assert np.abs(sv - 1).max() <= 1e-8, (a, b, sv)
         off = np.concatenate([M[a:b, :a], M[a:b, b:]], axis=1)
         assert off.size == 0 or np.abs(off).max() <= 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,m", [(1000, 1), (1000, 7), (1000, 64), (1000, 65), (777, 130),
+                                 (10364, 56)])
+def test_split_k_product_matches_torch(n, m):
+    """exh_zgemm_cn_device (the subspace iteration's C X) against PyTorch's
+    product on the same Hermitian C, stored row-major and column-major:
+    alpha C X + beta Y0 + gamma Z within 1e-13 of the largest entry (FP64, a
+    different summation order), and the same bits on a second call (the K
+    chunks are added in a fixed order); a^H x for a general rectangular a."""
+    import torch
+    from exciting_amd import lapw
+    g = torch.Generator(device="cuda").manual_seed(n + m)
+    B = torch.randn(n, n, dtype=torch.complex128, device="cuda", generator=g)
+    C = B + B.conj().T
+    X = torch.randn(n, m, dtype=torch.complex128, device="cuda", generator=g)
+    Y0 = torch.randn(n, m, dtype=torch.complex128, device="cuda", generator=g)
+    Z = torch.randn(n, m, dtype=torch.complex128, device="cuda", generator=g)
+    print(f"[gpu-zgemm-cn] n {n} m {m}", flush=True)
+    for alpha, beta, gamma, y0, z in ((1.0, 0.0, 0.0, None, None), (0.7, -1.3, 0.0, Y0, None),
+                                      (0.7, -1.3, 2.1, Y0, Z)):
+        want = alpha * (C @ X)
+        if y0 is not None:
+            want = want + beta * y0
+        if z is not None:
+            want = want + gamma * z
+        # row-major C, and the column-major copy of the same C read in place
+        for Cs in (C, C.t().contiguous().t()):
+            got = lapw.zgemm_cn(Cs, X, alpha=alpha, beta=beta, y0=y0, gamma=gamma, z=z,
+                                hermitian=True)
+            again = lapw.zgemm_cn(Cs, X, alpha=alpha, beta=beta, y0=y0, gamma=gamma, z=z,
+                                  hermitian=True)
+            torch.cuda.synchronize()
+            err = float((got - want).abs().max()) / float(want.abs().max())
+            assert err <= 1e-13, (alpha, beta, gamma, Cs.stride(), err)
+            assert torch.equal(got, again)
+    # a general (non-Hermitian) a: a^H x
+    A = B[:, : max(1, n // 2)].contiguous()
+    got = lapw.zgemm_cn(A, X)
+    want = A.conj().T @ X
+    assert float((got - want).abs().max()) <= 1e-13 * float(want.abs().max())
+    with pytest.raises(lapw.ExhError):
+        lapw.zgemm_cn(C, X[:-1])
