@@ -17,7 +17,7 @@ __device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
 }
 
 constexpr int CT = 64;    // output tile edge (rows i and columns j)
-constexpr int CK = 8;     // K slab staged in LDS per step
+constexpr int CK = 8;     // K slab staged in LDS per step (16: 2 %, 32: 6 % slower)
 constexpr int CNT = 256;  // 4 waves, each a 32 x 32 quadrant of 2 x 2 MFMA blocks
 
 // part[s][i][j] = sum over k in chunk s of conj(A[k][i]) X[k][j] (asign = -1)
