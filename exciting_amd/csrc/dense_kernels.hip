@@ -175,5 +175,6 @@ extern "C" int exh_zgemm_cn_device(void *stream, int conj_a, int nout, int m, in
   hipLaunchKernelGGL(cx_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      nout, m, S, (const double2 *)work, alpha, beta, (const double2 *)y0, ldy0,
                      gamma, (const double2 *)z, ldz, (double2 *)y, ldy);
-  return hipGetLastError() == hipSuccess ? EXH_OK : EXH_EHIP;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? EXH_OK : (int)e;
 }

@@ -51,8 +51,7 @@ enum {
   EXH_ESTATE = -2,   /* required set_* call missing           */
   EXH_ERANGE = -3,   /* size beyond what the kernels support  */
   EXH_ENOTSUP = -4,  /* option not implemented                */
-  EXH_EMATCH = -5,   /* singular boundary-derivative matrix   */
-  EXH_EHIP = -6      /* HIP runtime error (stateless calls)   */
+  EXH_EMATCH = -5    /* singular boundary-derivative matrix   */
 };
 
 /* Flags for exh_setup_device / exh_hamiltonandoverlapsetup_ex. */
@@ -340,7 +339,8 @@ int exh_overlapsetup_device(exh_context *ctx, void *stream, int ngp, int rank,
  * y0, z and y row-major nout x m (y0 and z may be NULL, y may be y0).  K is split into at most `split` chunks, each summed by its own
  * workgroups into work (split * nout * m complex), then the chunks are
  * added in order: the result does not depend on the launch.  Returns
- * EXH_EINVAL for bad sizes, EXH_EHIP if a launch fails. */
+ * EXH_EINVAL for bad sizes and the HIP error code (> 0) if a launch
+ * fails. */
 int exh_zgemm_cn_device(void *stream, int conj_a, int nout, int m, int k, const exh_z *a,
                         long long lda, const exh_z *x, long long ldx, double alpha, double beta,
                         const exh_z *y0, long long ldy0, double gamma, const exh_z *z,
