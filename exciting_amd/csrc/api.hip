@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -79,17 +80,6 @@ static int pin_threads() {
   return n;
 }
 
-static hipError_t pin_ensure(exh_context *c) {
-  if (c->pin_bytes) return hipSuccess;
-  for (int b = 0; b < 2; b++) {
-    hipError_t e = hipHostMalloc(&c->pin[b], kPinBytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming);
-    if (e != hipSuccess) return e;
-  }
-  c->pin_bytes = kPinBytes;
-  return hipSuccess;
-}
-
 static void pin_release(exh_context *c) {
   for (int b = 0; b < 2; b++) {
     if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
@@ -98,6 +88,23 @@ static void pin_release(exh_context *c) {
     c->pin_ev[b] = nullptr;
   }
   c->pin_bytes = 0;
+}
+
+// both buffers and their events, or none (a failed call frees what it made)
+static hipError_t pin_ensure(exh_context *c) {
+  if (c->pin_bytes) return hipSuccess;
+  for (int b = 0; b < 2; b++) {
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, kPinBytes, hipHostMallocDefault);
+    if (e == hipSuccess) c->pin[b] = p;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      pin_release(c);
+      return e;
+    }
+  }
+  c->pin_bytes = kPinBytes;
+  return hipSuccess;
 }
 
 // rows [r0, r1) of `width` bytes from src (pitch sp) to dst (pitch dp), over
@@ -120,7 +127,12 @@ static void par_copy(char *dst, size_t dp, const char *src, size_t sp, size_t wi
   }
   std::vector<std::thread> th;
   th.reserve(nt - 1);
-  for (int t = 1; t < nt; t++) th.emplace_back(part, t);
+  int t = 1;
+  try {
+    for (; t < nt; t++) th.emplace_back(part, t);
+  } catch (const std::system_error &) {
+    for (; t < nt; t++) part(t);  // no thread to be had: this one copies the rest
+  }
   part(0);
   for (auto &x : th) x.join();
 }
