@@ -119,6 +119,41 @@ __global__ __launch_bounds__(CNT) void cx_part_kernel(int nout, int m, int K, in
       }
 }
 
+// One column (m = 1: the matrix-vector products of the Lanczos bound): the
+// same partial sums without MFMA, which would spend 63 of 64 columns on
+// padding.  One thread per output row i, its reads of A coalesced along i;
+// each chunk's x is staged in LDS 256 entries at a time.
+constexpr int VT = 256;
+__global__ __launch_bounds__(VT) void cx_part_vec_kernel(int nout, int K, int kchunk,
+                                                         const double2 *__restrict__ A,
+                                                         long long lda, double asign,
+                                                         const double2 *__restrict__ X,
+                                                         long long ldx,
+                                                         double2 *__restrict__ part) {
+  const int i = blockIdx.x * VT + threadIdx.x, s = blockIdx.y;
+  const int k0 = s * kchunk, k1 = min(K, k0 + kchunk);
+  __shared__ double2 sx[VT];
+  const double2 *ap = A + min(i, nout - 1);
+  double re = 0.0, im = 0.0;
+  for (int kb = k0; kb < k1; kb += VT) {
+    const int kl = min(VT, k1 - kb);
+    __syncthreads();  // the previous block of x is no longer read
+    if ((int)threadIdx.x < kl) sx[threadIdx.x] = X[(size_t)(kb + threadIdx.x) * ldx];
+    __syncthreads();
+#pragma unroll 8
+    for (int t = 0; t < kl; t++) {
+      const double2 a = ap[(size_t)(kb + t) * lda];
+      const double2 x = sx[t];
+      const double ai = asign * a.y;  // conj(A): the imaginary part negated
+      re = fma(a.x, x.x, re);
+      re = fma(-ai, x.y, re);
+      im = fma(a.x, x.y, im);
+      im = fma(ai, x.x, im);
+    }
+  }
+  if (i < nout) part[(size_t)s * nout + i] = z2(re, im);
+}
+
 // Y[i][j] = alpha * sum_s part[s][i][j] (s in order) + beta * Y0[i][j] +
 // gamma * Z[i][j]; Y may be Y0 (each element is read before it is written).
 __global__ void cx_reduce_kernel(int nout, int m, int S, const double2 *__restrict__ part,
@@ -167,10 +202,18 @@ extern "C" int exh_zgemm_cn_device(void *stream, int conj_a, int nout, int m, in
   kchunk = (kchunk + CK - 1) / CK * CK;
   const int S = (k + kchunk - 1) / kchunk;
   hipStream_t s = (hipStream_t)stream;
-  const dim3 grid((unsigned)((nout + CT - 1) / CT), (unsigned)((m + CT - 1) / CT), (unsigned)S);
-  hipLaunchKernelGGL(cx_part_kernel, grid, dim3(CNT), 0, s, nout, m, k, kchunk,
-                     (const double2 *)a, lda, conj_a ? -1.0 : 1.0, (const double2 *)x, ldx,
-                     (double2 *)work);
+  if (m == 1) {
+    const dim3 grid((unsigned)((nout + VT - 1) / VT), (unsigned)S);
+    hipLaunchKernelGGL(cx_part_vec_kernel, grid, dim3(VT), 0, s, nout, k, kchunk,
+                       (const double2 *)a, lda, conj_a ? -1.0 : 1.0, (const double2 *)x, ldx,
+                       (double2 *)work);
+  } else {
+    const dim3 grid((unsigned)((nout + CT - 1) / CT), (unsigned)((m + CT - 1) / CT),
+                    (unsigned)S);
+    hipLaunchKernelGGL(cx_part_kernel, grid, dim3(CNT), 0, s, nout, m, k, kchunk,
+                       (const double2 *)a, lda, conj_a ? -1.0 : 1.0, (const double2 *)x, ldx,
+                       (double2 *)work);
+  }
   const size_t total = (size_t)nout * m;
   hipLaunchKernelGGL(cx_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      nout, m, S, (const double2 *)work, alpha, beta, (const double2 *)y0, ldy0,

@@ -736,7 +736,8 @@ def zgemm_cn(a, x, alpha: float = 1.0, beta: float = 0.0, y0=None, gamma: float 
     With hermitian=True, ``a`` is a Hermitian matrix C and the result is
     alpha C x + ...; C is read in place whether it is stored row-major or
     column-major (no copy of C).  ``split`` chunks of K (0: enough for about
-    4096 workgroups, at most 32)."""
+    4096 output tiles x chunks, with chunks of at least 64 of K: the Gram
+    matrices of a block, 56 x 56 over K = 10364, get 162 chunks)."""
     import torch
     if a.dtype != torch.complex128 or x.dtype != torch.complex128 or not a.is_cuda:
         raise ExhError("zgemm_cn: complex128 CUDA tensors expected")
@@ -750,7 +751,7 @@ def zgemm_cn(a, x, alpha: float = 1.0, beta: float = 0.0, y0=None, gamma: float 
         raise ExhError("zgemm_cn: a and x disagree on k")
     m = x.shape[1]
     tiles = -(-nout // 64) * -(-m // 64)
-    split = split or max(1, min(32, -(-4096 // max(tiles, 1))))
+    split = split or max(1, min(4096, -(-4096 // max(tiles, 1)), -(-k // 64)))
     out = torch.empty((nout, m), dtype=torch.complex128, device=a.device)
     work = torch.empty(split * nout * m, dtype=torch.complex128, device=a.device)
     for t in (y0, z):

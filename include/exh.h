@@ -336,9 +336,11 @@ int exh_overlapsetup_device(exh_context *ctx, void *stream, int ngp, int rank,
  * for a(k, i) at a[k*lda + i] (k < k, i < nout).  A Hermitian matrix C gives
  * C x either way without a copy: row-major storage with conj_a = 1,
  * column-major storage with conj_a = 0.  x(k, j) at x[k*ldx + j] (j < m);
- * y0, z and y row-major nout x m (y0 and z may be NULL, y may be y0).  K is split into at most `split` chunks, each summed by its own
- * workgroups into work (split * nout * m complex), then the chunks are
- * added in order: the result does not depend on the launch.  Returns
+ * y0, z and y row-major nout x m (y0 and z may be NULL, y may be y0).  K is
+ * split into at most `split` chunks, each summed by its own workgroups into
+ * work (split * nout * m complex; FP64 MFMA tiles, or for m = 1 one thread
+ * per output row), then the chunks are added in order: the result does not
+ * depend on the launch.  Returns
  * EXH_EINVAL for bad sizes and the HIP error code (> 0) if a launch
  * fails. */
 int exh_zgemm_cn_device(void *stream, int conj_a, int nout, int m, int k, const exh_z *a,

@@ -172,13 +172,14 @@ def test_subspace_vectors_at_64_atoms(nst):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,m", [(1000, 1), (1000, 7), (1000, 64), (1000, 65), (777, 130),
-                                 (10364, 56)])
+                                 (10364, 1), (10364, 56)])
 def test_split_k_product_matches_torch(n, m):
     """exh_zgemm_cn_device (the subspace iteration's C X) against PyTorch's
     product on the same Hermitian C, stored row-major and column-major:
     alpha C X + beta Y0 + gamma Z within 1e-13 of the largest entry (FP64, a
     different summation order), and the same bits on a second call (the K
-    chunks are added in a fixed order); a^H x for a general rectangular a."""
+    chunks are added in a fixed order); a^H x for a general rectangular a and
+    for the Gram matrix of two blocks.  m = 1 runs the one-column kernel."""
     import torch
     from exciting_amd import lapw
     g = torch.Generator(device="cuda").manual_seed(n + m)
@@ -210,5 +211,10 @@ def test_split_k_product_matches_torch(n, m):
     got = lapw.zgemm_cn(A, X)
     want = A.conj().T @ X
     assert float((got - want).abs().max()) <= 1e-13 * float(want.abs().max())
+    # the Gram matrix of two blocks (m x m over K = n: many K chunks, few tiles)
+    got = lapw.zgemm_cn(X, Y0)
+    want = X.conj().T @ Y0
+    assert float((got - want).abs().max()) <= 1e-13 * float(want.abs().max())
+    assert torch.equal(got, lapw.zgemm_cn(X, Y0))
     with pytest.raises(lapw.ExhError):
         lapw.zgemm_cn(C, X[:-1])
