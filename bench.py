@@ -391,7 +391,8 @@ def serving_sample(args):
     out.update({"sample": (f"{args.serve_requests} Poisson requests at {args.serve_rate}/s over "
                            f"fcc2/sc8/ortho16/sc32/sc64, 1..{args.serve_kmax} k-points each, "
                            f"half split ({args.serve_moduli} moduli), {args.serve_nstfv} states, "
-                           f"up to {args.serve_solves} eigensolves at once, admission budget "
+                           f"one eigensolve at a time ({args.serve_solves} solver threads), "
+                           f"admission budget "
                            f"{budget / 2**30:.0f} GiB"),
                 "budget_gib": budget / 2**30,
                 "peak_inflight": srv.peak_inflight, "contexts": srv.pool.created,
@@ -779,8 +780,8 @@ def main():
                          "and a few small ones; 0 = free HBM minus the reserve)")
     ap.add_argument("--serve-kmax", type=int, default=4)
     ap.add_argument("--serve-nstfv", type=int, default=40)
-    ap.add_argument("--serve-solves", type=int, default=4,
-                    help="eigensolves the serving sample runs at once")
+    ap.add_argument("--serve-solves", type=int, default=1,
+                    help="solver threads of the serving sample (the solves run one at a time)")
     ap.add_argument("--serve-moduli", type=int, default=9,
                     help="moduli of the split requests (9 keeps every cell of the mix within "
                          "1 uHa)")

@@ -286,13 +286,23 @@ class Server:
 # ---------------------------------------------------------------------------
 _SOLVER_POOLS: Dict[tuple, tuple] = {}
 _SOLVER_LOCK = threading.Lock()
+# One eigensolve at a time per process.  PyTorch's linear algebra (rocBLAS,
+# rocSOLVER) called from several host threads on their own streams at once
+# gave wrong results here: four threads solving the same 8-atom H and O
+# concurrently (Cholesky, triangular solves, GEMMs, eigh) differed from the
+# one-thread eigenvalues in 5 of 160 solves, by up to 1e-4 Ha, whether the
+# subspace or the full eigensolve ran; with every library call serialized
+# and synchronized, none of 320 did.  Setups (this library's own kernels on
+# the batcher's stream) still overlap the solve.
+_SOLVE_GATE = threading.Lock()
 
 
 def _solver_pool(device: int, max_solves: int):
     """The process's solver threads for one device, created once and shared by
     every executor: at most ``max_solves`` (thread, stream) pairs run BLAS,
     so PyTorch holds at most that many hipBLAS workspaces however many
-    executors, contexts and request threads come and go."""
+    executors, contexts and request threads come and go.  The solves
+    themselves run one at a time (_SOLVE_GATE); more threads only queue."""
     key = (device, max(1, max_solves))
     with _SOLVER_LOCK:
         if key not in _SOLVER_POOLS:
@@ -456,13 +466,13 @@ class Batcher:
 class GpuExecutor:
     """Runs requests on ``cuda:device``: the k-point setups of all requests
     in flight go through one Batcher (grouped multi-structure launches), the
-    eigensolves through the process's solver threads for the device, at most
-    ``max_solves`` at once.  ``make_context``/``close_context``/``execute``
+    eigensolves through the process's solver threads for the device
+    (``max_solves`` threads; the solves run one at a time, see _SOLVE_GATE).  ``make_context``/``close_context``/``execute``
     plug into Server.  With ``digests`` each request also records a SHA-1 of
     the bytes of H and O per k-point (copies them to the host: for tests on
     small cells)."""
 
-    def __init__(self, device: int = 0, digests: bool = False, max_solves: int = 4,
+    def __init__(self, device: int = 0, digests: bool = False, max_solves: int = 1,
                  max_batch_k: int = 32, plan: int = -1, linger_s: float = 0.002,
                  slots: int = 1):
         from . import solve as _solve
@@ -486,7 +496,7 @@ class GpuExecutor:
         if st is None:
             st = self._local.stream = torch.cuda.Stream(device=self.dev)
         st.wait_event(ready)                   # the setup of H and O is done
-        with torch.cuda.stream(st):
+        with _SOLVE_GATE, torch.cuda.stream(st):
             try:
                 w = _solve.geneig(H, O, n, nst, method="auto")
             except torch.cuda.OutOfMemoryError:
@@ -498,6 +508,7 @@ class GpuExecutor:
                 w = _solve.geneig(H, O, n, nst, method="auto")
             out = w.cpu().numpy()              # waits for the solve
             del w
+            st.synchronize()                   # nothing of it left running
         # the solve's n x n temporaries are free now; hand them back so that
         # they do not sit in PyTorch's cache in pieces that the admission
         # budget cannot see and hipMalloc cannot use
@@ -736,8 +747,8 @@ def main(argv=None) -> int:
     ap.add_argument("--nmod", type=int, default=9)
     ap.add_argument("--split-fraction", type=float, default=0.5)
     ap.add_argument("--max-inflight", type=int, default=8)
-    ap.add_argument("--max-solves", type=int, default=4,
-                    help="eigensolves allowed to run at once")
+    ap.add_argument("--max-solves", type=int, default=1,
+                    help="solver threads (the eigensolves themselves run one at a time)")
     ap.add_argument("--max-batch-k", type=int, default=32,
                     help="k-points one grouped setup launch takes at most")
     ap.add_argument("--reserve-gib", type=float, default=48.0,
