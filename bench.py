@@ -128,6 +128,10 @@ def eigvec_timing(H, O, n, nsts):
     subspace iteration, and how far apart their eigenvalues are."""
     import torch
     from exciting_amd import solve
+    # one untimed subspace solve first: the process's first one also pays
+    # one-time costs (kernel and library initialisation, allocator growth)
+    solve.solvewithlapack(H, O, n, nsts[0], method="subspace")
+    torch.cuda.synchronize()
     out = []
     for nv in nsts:
         row = {"nstates": nv}
