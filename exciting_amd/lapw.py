@@ -741,6 +741,8 @@ def zgemm_cn(a, x, alpha: float = 1.0, beta: float = 0.0, y0=None, gamma: float 
     import torch
     if a.dtype != torch.complex128 or x.dtype != torch.complex128 or not a.is_cuda:
         raise ExhError("zgemm_cn: complex128 CUDA tensors expected")
+    if any(t is not None and t.device != a.device for t in (x, y0, z)):
+        raise ExhError("zgemm_cn: all tensors must be on the device of a")
     conj_a = 1
     if hermitian and not a.is_contiguous() and a.t().is_contiguous():
         a, conj_a = a.t(), 0      # row-major C^T: C x = (C^T)^T x
