@@ -395,7 +395,8 @@ def serving_sample(args):
     out.update({"sample": (f"{args.serve_requests} Poisson requests at {args.serve_rate}/s over "
                            f"fcc2/sc8/ortho16/sc32/sc64, 1..{args.serve_kmax} k-points each, "
                            f"half split ({args.serve_moduli} moduli), {args.serve_nstfv} states, "
-                           f"one eigensolve at a time ({args.serve_solves} solver threads), "
+                           f"one eigensolve at a time ({args.serve_solves} solver "
+                           f"thread{'s' if args.serve_solves != 1 else ''}), "
                            f"admission budget "
                            f"{budget / 2**30:.0f} GiB"),
                 "budget_gib": budget / 2**30,
