@@ -467,8 +467,8 @@ class GpuExecutor:
     """Runs requests on ``cuda:device``: the k-point setups of all requests
     in flight go through one Batcher (grouped multi-structure launches), the
     eigensolves through the process's solver threads for the device
-    (``max_solves`` threads; the solves run one at a time, see _SOLVE_GATE).  ``make_context``/``close_context``/``execute``
-    plug into Server.  With ``digests`` each request also records a SHA-1 of
+    (``max_solves`` threads; the solves run one at a time, see _SOLVE_GATE).
+    ``make_context``/``close_context``/``execute`` plug into Server.  With ``digests`` each request also records a SHA-1 of
     the bytes of H and O per k-point (copies them to the host: for tests on
     small cells)."""
 
