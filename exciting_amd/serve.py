@@ -286,15 +286,19 @@ class Server:
 # ---------------------------------------------------------------------------
 _SOLVER_POOLS: Dict[tuple, tuple] = {}
 _SOLVER_LOCK = threading.Lock()
-# One eigensolve at a time per process.  PyTorch's linear algebra (rocBLAS,
-# rocSOLVER) called from several host threads on their own streams at once
-# gave wrong results here: four threads solving the same 8-atom H and O
-# concurrently (Cholesky, triangular solves, GEMMs, eigh) differed from the
-# one-thread eigenvalues in 5 of 160 solves, by up to 1e-4 Ha, whether the
-# subspace or the full eigensolve ran; with every library call serialized
-# and synchronized, none of 320 did.  Setups (this library's own kernels on
-# the batcher's stream) still overlap the solve.
-_SOLVE_GATE = threading.Lock()
+# Setups and eigensolves take turns on the GPU, one at a time per process.
+# PyTorch's linear algebra (rocBLAS, rocSOLVER) called from several host
+# threads on their own streams at once gave wrong results here: four threads
+# solving the same 8-atom H and O concurrently (Cholesky, triangular solves,
+# GEMMs, eigh) differed from the one-thread eigenvalues in 5 of 160 solves,
+# by up to 1e-4 Ha, whether the subspace or the full eigensolve ran; with
+# every library call serialized and synchronized, none of 320 did.  With
+# the solves serialized but a grouped setup launch (this library's kernels,
+# on the batcher's stream) allowed to run beside a solve, 2 of 30 served
+# mixes still gave other eigenvalues from bitwise equal H and O; with the
+# setup launches taking turns too, none of 36 did.  Each holder waits for
+# its own stream before releasing the lock.
+_GPU_GATE = threading.Lock()
 
 
 def _solver_pool(device: int, max_solves: int):
@@ -302,7 +306,7 @@ def _solver_pool(device: int, max_solves: int):
     every executor: at most ``max_solves`` (thread, stream) pairs run BLAS,
     so PyTorch holds at most that many hipBLAS workspaces however many
     executors, contexts and request threads come and go.  The solves
-    themselves run one at a time (_SOLVE_GATE); more threads only queue."""
+    themselves run one at a time (_GPU_GATE); more threads only queue."""
     key = (device, max(1, max_solves))
     with _SOLVER_LOCK:
         if key not in _SOLVER_POOLS:
@@ -423,24 +427,26 @@ class Batcher:
             lapw.setup_multi(ctxs, kp, flags=flags | lapw.MATCH,
                              stream=self.stream.cuda_stream)
 
-        try:
-            run()
-        except lapw.ExhError as e:
-            # The contexts' scratch grows on first use of a larger setup (the
-            # first split one, say).  Blocks PyTorch's caching allocator holds
-            # free for the eigensolves are not visible to hipMalloc: release
-            # them and run the launch once more (it overwrites H and O).
-            if "out of memory" not in str(e):
-                raise
-            for it in its:
-                it.req.retries_setup += 1
-            self.retries += 1
-            torch.cuda.synchronize(self.device)
-            torch.cuda.empty_cache()
-            run()
-        ev = torch.cuda.Event()
-        ev.record(self.stream)
-        ev.synchronize()        # setups arriving meanwhile form the next launch
+        with _GPU_GATE:         # no eigensolve runs beside the launch
+            try:
+                run()
+            except lapw.ExhError as e:
+                # The contexts' scratch grows on first use of a larger setup
+                # (the first split one, say).  Blocks PyTorch's caching
+                # allocator holds free for the eigensolves are not visible to
+                # hipMalloc: release them and run the launch once more (it
+                # overwrites H and O).
+                if "out of memory" not in str(e):
+                    raise
+                for it in its:
+                    it.req.retries_setup += 1
+                self.retries += 1
+                torch.cuda.synchronize(self.device)
+                torch.cuda.empty_cache()
+                run()
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+            ev.synchronize()    # setups arriving meanwhile form the next launch
         self.busy_s += time.perf_counter() - t0
         ncell = len({id(it.ctx.sys) for it in its})
         self.launches += 1
@@ -467,7 +473,8 @@ class GpuExecutor:
     """Runs requests on ``cuda:device``: the k-point setups of all requests
     in flight go through one Batcher (grouped multi-structure launches), the
     eigensolves through the process's solver threads for the device
-    (``max_solves`` threads; the solves run one at a time, see _SOLVE_GATE).
+    (``max_solves`` threads; solves and setup launches take turns, see
+    _GPU_GATE).
     ``make_context``/``close_context``/``execute`` plug into Server.  With ``digests`` each request also records a SHA-1 of
     the bytes of H and O per k-point (copies them to the host: for tests on
     small cells)."""
@@ -496,7 +503,7 @@ class GpuExecutor:
         if st is None:
             st = self._local.stream = torch.cuda.Stream(device=self.dev)
         st.wait_event(ready)                   # the setup of H and O is done
-        with _SOLVE_GATE, torch.cuda.stream(st):
+        with _GPU_GATE, torch.cuda.stream(st):
             try:
                 w = _solve.geneig(H, O, n, nst, method="auto")
             except torch.cuda.OutOfMemoryError:
