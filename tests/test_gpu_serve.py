@@ -5,7 +5,8 @@ precision, 1-4 k-points each) is served twice: one request at a time with
 one k-point slot per context, then all at once with up to six in flight and
 two slots per context, their k-point setups merged into grouped launches
 over several cells (exh_setup_multi_device) and their eigensolves
-queued for the solver thread, overlapping those setups.  Tolerance: the H
+queued for the solver thread (setup launches and solves take turns on the
+GPU).  Tolerance: the H
 and O matrices must be bitwise identical (SHA-1 of their bytes per k-point),
 and the eigenvalues identical to 1e-12 Ha (they come from the same library
 on bitwise identical inputs).  The FP64 eigenvalues are also checked against
